@@ -1,0 +1,82 @@
+// Shared definitions of the CK-MPM HIP engine (device and host).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ckmpm {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kDirections = 27;      // new block - old block in {-1,0,1}^3
+constexpr int kMoveShift = 27;       // ordering word: slot (27 bits) | move << 27
+constexpr uint32_t kSlotMask = (1u << kMoveShift) - 1u;
+constexpr int kNodesPerBlock = 128;  // 2 sub-grids x 4^3 nodes
+constexpr int kMChannels = 4;        // m, px, py, pz
+constexpr int kVChannels = 3;        // vx, vy, vz
+constexpr int kMFloats = kMChannels * kNodesPerBlock;  // 512 floats = 2 KiB per block
+constexpr int kVFloats = kVChannels * kNodesPerBlock;  // 384 floats = 1.5 KiB per block
+
+enum Kind : int { kFixedCorotated = 0, kNeoHookean = 1, kDruckerPrager = 2, kFluid = 3 };
+
+enum Flag : uint32_t {
+    kFlagOutOfWindow = 1u << 0,
+    kFlagOutOfDomain = 1u << 1,
+    kFlagNonFinite = 1u << 2,
+    kFlagCapacity = 1u << 3,
+    kFlagMissingBlock = 1u << 4,
+};
+
+// Device-resident counters of one block layout.
+struct LayoutCounts {
+    uint32_t A;   // particle blocks: [0, A)
+    uint32_t Nb;  // + neighbour blocks b + {0,1}^3: [A, Nb)  (the grid)
+    uint32_t Ex;  // + exterior blocks b + {-1,0,1}^3: [Nb, Ex)
+    uint32_t pad;
+};
+
+// Device-resident engine status.
+struct Status {
+    uint32_t flags;
+    uint32_t vmax2_bits;  // max |v|^2 of the last grid update (non-negative float bits)
+    uint32_t pad[2];
+};
+
+struct DomainDev {
+    int dim[3];
+    int off[3];
+    uint32_t nblocks;
+    float dx, inv_dx;
+};
+
+struct MaterialDev {
+    int kind;
+    float lambda, mu;
+    float bulk, gamma, viscosity, J0;
+    float sin_phi;
+    float mass, volume;
+};
+
+__host__ __device__ inline uint32_t KeyOf(const DomainDev& d, int bx, int by, int bz) {
+    return static_cast<uint32_t>((bx * d.dim[1] + by) * d.dim[2] + bz);
+}
+__host__ __device__ inline void CoordsOf(const DomainDev& d, uint32_t key, int& bx, int& by, int& bz) {
+    bz = static_cast<int>(key % static_cast<uint32_t>(d.dim[2]));
+    const uint32_t r = key / static_cast<uint32_t>(d.dim[2]);
+    by = static_cast<int>(r % static_cast<uint32_t>(d.dim[1]));
+    bx = static_cast<int>(r / static_cast<uint32_t>(d.dim[1]));
+}
+__host__ __device__ inline bool InDomain(const DomainDev& d, int bx, int by, int bz) {
+    return bx >= 0 && by >= 0 && bz >= 0 && bx < d.dim[0] && by < d.dim[1] && bz < d.dim[2];
+}
+__host__ __device__ inline int FloorDiv4(int v) { return v >> 2; }  // arithmetic shift = floor
+
+// Grid-0 cell of an FP32 coordinate: int(x * invDx + 0.25f) with the multiply and
+// the add rounded separately (mpm_algorithm.cuh:59-67).
+__device__ inline int Cell0(float x, float inv_dx) {
+    const float t = __fmul_rn(x, inv_dx);
+    return static_cast<int>(floorf(__fadd_rn(t, 0.25f)));
+}
+
+__device__ inline void SetFlag(Status* st, uint32_t f) { atomicOr(&st->flags, f); }
+
+}  // namespace ckmpm

@@ -1,0 +1,651 @@
+// Host orchestration of the CK-MPM substep and the C-ABI of libckmpm.so.
+//
+// One engine = one HIP stream.  Every per-substep quantity that the reference reads
+// back to the host (block counts, bin offsets, max |v|) stays in device memory:
+// kernels read the counts they need, so a substep is enqueued without a host sync
+// (the reference needs about six, mpm_engine.cuh:537-542,650,724-737,783,803-807,846-850).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/ckmpm.h"
+#include "kernels.h"
+
+using namespace ckmpm;
+
+namespace {
+
+struct ModelBuf {
+    MaterialDev mat{};
+    int kind = 0;
+    int nattr = 0;
+    uint32_t n = 0;
+    float* st[2] = {nullptr, nullptr};
+    uint32_t* perm[2] = {nullptr, nullptr};
+    uint32_t* off[2] = {nullptr, nullptr};
+    uint32_t* code = nullptr;
+    uint32_t* cnt = nullptr;
+    uint32_t* base = nullptr;
+    uint32_t* cnt_e = nullptr;
+    uint32_t* cntn = nullptr;
+    int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
+    // setup inputs (host), released after the initial setup
+    std::vector<float> x0, v0, lam0, mu0;
+    uint32_t nvel = 0;
+};
+
+struct LayoutBuf {
+    uint32_t* keys = nullptr;
+    uint32_t* table = nullptr;
+    LayoutCounts* lc = nullptr;
+};
+
+int AttrCount(int kind) {
+    return kind == kDruckerPrager ? 13 : (kind == kFluid ? 4 : 12);
+}
+
+}  // namespace
+
+struct ckmpm_engine {
+    DomainDev dom{};
+    ckmpm_sim_desc sim{};
+    uint32_t cap = 0;
+    hipStream_t stream = nullptr;
+    LayoutBuf L[2];
+    int cur = 0;          // current layout; 1 - cur is the previous one
+    float* M[2] = {nullptr, nullptr};
+    int mprev = 0;        // M[mprev] holds the previous layout's mass / momentum
+    float* V = nullptr;
+    Status* status = nullptr;
+    uint32_t* bitmap = nullptr;
+    uint32_t nwords = 0;
+    uint32_t* word_off = nullptr;
+    uint32_t* occ = nullptr;
+    uint32_t* newidx = nullptr;
+    uint32_t* scan_tmp = nullptr;
+    uint32_t* total = nullptr;
+    std::vector<ModelBuf> models;
+    int sc = 0;           // state / ordering buffer index of the current step
+    bool setup_done = false;
+    uint32_t a_hint = 0;  // particle-block count at the last host read
+    uint64_t substeps = 0;
+    double dt_default = 0;
+    double time = 0, cur_dt = 0;
+    std::string err;
+    std::vector<void*> allocs;
+    // phase timing: 4 events per substep (before grid, after grid, after G2P2G, after rebin)
+    bool profile = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+int Fail(ckmpm_engine* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+#define CK_HIP(e, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t _r = (call);                                                                          \
+        if (_r != hipSuccess) return Fail((e), CKMPM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_r)); \
+    } while (0)
+
+template <typename T>
+int Alloc(ckmpm_engine* e, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t r = hipMalloc(&q, bytes);
+    if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(r));
+    e->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return CKMPM_OK;
+}
+
+#define CK_TRY(x)                   \
+    do {                            \
+        int _rc = (x);              \
+        if (_rc != CKMPM_OK) return _rc; \
+    } while (0)
+
+uint32_t GridFor(uint32_t blocks) {
+    uint32_t g = std::max<uint32_t>(blocks, 8u);
+    g = std::min<uint32_t>(g, 16384u);
+    return (g + 7u) & ~7u;
+}
+
+int FlagsToCode(uint32_t f) {
+    if (f & kFlagNonFinite) return CKMPM_ERR_NONFINITE;
+    if (f & kFlagCapacity) return CKMPM_ERR_CAPACITY;
+    if (f & kFlagOutOfDomain) return CKMPM_ERR_OUT_OF_DOMAIN;
+    if (f & (kFlagOutOfWindow | kFlagMissingBlock)) return CKMPM_ERR_OUT_OF_WINDOW;
+    return CKMPM_OK;
+}
+
+int ReadStatus(ckmpm_engine* e, Status& st, LayoutCounts& lc) {
+    CK_HIP(e, hipMemcpyAsync(&st, e->status, sizeof(Status), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipMemcpyAsync(&lc, e->L[e->cur].lc, sizeof(LayoutCounts), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    e->a_hint = lc.A;
+    return CKMPM_OK;
+}
+
+// Neighbour and exterior extension of layout L (its A keys already written).
+void ExtendLayout(ckmpm_engine* e, LayoutBuf& L) {
+    hipStream_t s = e->stream;
+    LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
+    for (int pass = 0; pass < 2; ++pass) {
+        LaunchMarkExtension(e->dom, L.keys, L.lc, e->cap, L.table, e->bitmap, pass == 0 ? 0 : -1, s);
+        ExclusiveScan(e->bitmap, e->word_off, nullptr, e->nwords, e->nwords, e->total, true, e->scan_tmp, s);
+        LaunchBitmapEmit(e->bitmap, e->nwords, e->word_off, e->total, L.keys, L.table, L.lc, e->cap, pass, e->status, s);
+    }
+}
+
+void Mark(ckmpm_engine* e) {
+    if (!e->profile) return;
+    if (e->ev_used == e->events.size()) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return;
+        e->events.push_back(ev);
+    }
+    (void)hipEventRecord(e->events[e->ev_used++], e->stream);
+}
+
+// Grid update with dt: velocities of the current layout from the previous layout's
+// mass / momentum; clears the next P2G target.
+void PhaseGrid(ckmpm_engine* e, double dt) {
+    hipStream_t s = e->stream;
+    Mark(e);
+    LaunchBeginSubstep(e->status, s);
+    GridUpdateArgs g{};
+    g.dom = e->dom;
+    g.keys = e->L[e->cur].keys;
+    g.lc = e->L[e->cur].lc;
+    g.table_prev = e->L[1 - e->cur].table;
+    g.lc_prev = e->L[1 - e->cur].lc;
+    g.M_prev = e->M[e->mprev];
+    g.V = e->V;
+    g.M_zero = e->M[1 - e->mprev];
+    g.dt = static_cast<float>(dt);
+    g.gravity = e->sim.gravity;
+    g.mass_clamp = e->sim.mass_clamp;
+    g.cap = e->cap;
+    g.st = e->status;
+    LaunchGridUpdate(g, GridFor(std::max<uint32_t>(e->a_hint * 2u, 64u)), s);
+    Mark(e);
+}
+
+// G2P2G(dt, next_dt) for every model, then the rebin into the next layout.
+void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
+    hipStream_t s = e->stream;
+    LayoutBuf& C = e->L[e->cur];
+    LayoutBuf& P = e->L[1 - e->cur];
+    float* Mout = e->M[1 - e->mprev];
+    const int sc = e->sc, sn = 1 - e->sc;
+    const uint32_t grid = GridFor(e->a_hint + e->a_hint / 4 + 8);
+    for (ModelBuf& m : e->models) {
+        G2P2GArgs a{};
+        a.dom = e->dom;
+        a.mat = m.mat;
+        a.keys = C.keys;
+        a.table = C.table;
+        a.lc = C.lc;
+        a.V = e->V;
+        a.M = Mout;
+        a.st_in = m.st[sc];
+        a.st_out = m.st[sn];
+        a.n = m.n;
+        a.perm = m.perm[sc];
+        a.off = m.off[sc];
+        a.code = m.code;
+        a.cnt = m.cnt;
+        a.pp_attr = m.pp_attr;
+        a.dt = static_cast<float>(dt);
+        a.next_dt = static_cast<float>(next_dt);
+        a.st = e->status;
+        LaunchG2P2G(a, grid, s);
+    }
+    Mark(e);
+    // ---- rebin: new layout in P (the previous layout's buffers are free now)
+    RebinModels rm{};
+    rm.nmodels = static_cast<int>(e->models.size());
+    for (int i = 0; i < rm.nmodels; ++i) {
+        ModelBuf& m = e->models[i];
+        rm.cnt[i] = m.cnt;
+        rm.cnt_e[i] = m.cnt_e;
+        rm.cntn[i] = m.cntn;
+        rm.base[i] = m.base;
+        rm.off_next[i] = m.off[sn];
+    }
+    LaunchTableClear(P.keys, P.lc, e->cap, P.table, s);
+    LaunchRebinCount(e->dom, C.keys, C.table, C.lc, rm, e->occ, e->cap, s);
+    ExclusiveScan(e->occ, e->newidx, &C.lc->Ex, 0, e->cap, e->total, false, e->scan_tmp, s);
+    LaunchSetCount(P.lc, 0, e->total, e->cap, e->status, s);
+    LaunchRebinCompact(C.keys, C.lc, e->occ, e->newidx, rm, P.keys, e->cap, s);
+    for (int i = 0; i < rm.nmodels; ++i) {
+        ModelBuf& m = e->models[i];
+        ExclusiveScan(m.cntn, m.off[sn], &P.lc->A, 0, e->cap, nullptr, false, e->scan_tmp, s);
+        LaunchFinishOffsets(m.off[sn], P.lc, m.n, s);
+    }
+    LaunchRebinBase(e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap, s);
+    for (ModelBuf& m : e->models) LaunchRebinScatter(C.lc, m.off[sc], m.code, m.base, m.perm[sn], m.n, grid, s);
+    ExtendLayout(e, P);
+    Mark(e);
+    e->cur = 1 - e->cur;
+    e->sc = sn;
+    e->mprev = 1 - e->mprev;
+    e->substeps++;
+}
+
+int CheckReady(ckmpm_engine* e) {
+    if (!e) return CKMPM_ERR_INVALID;
+    if (!e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "call ckmpm_initial_setup first");
+    return CKMPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckmpm_engine** out) {
+    if (!domain || !sim || !out) return CKMPM_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CKMPM_ERR_NO_DEVICE;
+    for (int a = 0; a < 3; ++a)
+        if (domain->dim[a] <= 0 || domain->dim[a] > 1024) return CKMPM_ERR_INVALID;
+    if (!(sim->dx > 0.0f) || sim->block_capacity == 0) return CKMPM_ERR_INVALID;
+    const uint64_t nblocks = static_cast<uint64_t>(domain->dim[0]) * domain->dim[1] * domain->dim[2];
+    if (nblocks >= (1ull << 31)) return CKMPM_ERR_INVALID;
+    ckmpm_engine* e = new ckmpm_engine();
+    for (int a = 0; a < 3; ++a) {
+        e->dom.dim[a] = domain->dim[a];
+        e->dom.off[a] = domain->offset[a];
+    }
+    e->dom.nblocks = static_cast<uint32_t>(nblocks);
+    e->dom.dx = sim->dx;
+    e->dom.inv_dx = static_cast<float>(1.0 / static_cast<double>(sim->dx));  // mpm_config.h:22-25
+    e->sim = *sim;
+    e->cap = std::min<uint32_t>(sim->block_capacity, e->dom.nblocks);
+    e->dt_default = sim->dt;
+    if (sim->fps) e->dt_default = std::min<double>(e->dt_default, 1.0 / sim->fps);  // mpm_engine.cuh:178
+    int rc = CKMPM_OK;
+    auto check = [&](int r) { if (rc == CKMPM_OK) rc = r; };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = Fail(e, CKMPM_ERR_HIP, "stream");
+    for (int i = 0; i < 2 && rc == CKMPM_OK; ++i) {
+        check(Alloc(e, &e->L[i].keys, e->cap));
+        check(Alloc(e, &e->L[i].table, nblocks));
+        check(Alloc(e, &e->L[i].lc, 1));
+        check(Alloc(e, &e->M[i], static_cast<size_t>(e->cap) * kMFloats));
+    }
+    e->nwords = static_cast<uint32_t>((nblocks + 31) / 32);
+    check(Alloc(e, &e->V, static_cast<size_t>(e->cap) * kVFloats));
+    check(Alloc(e, &e->status, 1));
+    check(Alloc(e, &e->bitmap, e->nwords));
+    check(Alloc(e, &e->word_off, e->nwords));
+    check(Alloc(e, &e->occ, e->cap));
+    check(Alloc(e, &e->newidx, e->cap));
+    check(Alloc(e, &e->total, 4));
+    check(Alloc(e, &e->scan_tmp, ScanTempWords(std::max<size_t>(e->cap, e->nwords))));
+    if (rc == CKMPM_OK) {
+        for (int i = 0; i < 2; ++i) {
+            (void)hipMemsetAsync(e->L[i].table, 0xFF, nblocks * sizeof(uint32_t), e->stream);
+            (void)hipMemsetAsync(e->L[i].lc, 0, sizeof(LayoutCounts), e->stream);
+        }
+        (void)hipMemsetAsync(e->bitmap, 0, e->nwords * sizeof(uint32_t), e->stream);
+        (void)hipMemsetAsync(e->status, 0, sizeof(Status), e->stream);
+        if (hipStreamSynchronize(e->stream) != hipSuccess) rc = Fail(e, CKMPM_ERR_HIP, "init");
+    }
+    if (rc != CKMPM_OK) {
+        ckmpm_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return CKMPM_OK;
+}
+
+int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass, float volume, const float* xyz,
+                    uint64_t n, const float* vel, uint64_t nvel, const float* lambda_p, const float* mu_p,
+                    int32_t* model_index) {
+    if (!e || !mat || !xyz || !vel) return CKMPM_ERR_INVALID;
+    if (e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "models must be added before ckmpm_initial_setup");
+    if (e->models.size() >= static_cast<size_t>(kMaxModels)) return Fail(e, CKMPM_ERR_INVALID, "too many models");
+    if (n == 0 || n >= (1ull << 31)) return Fail(e, CKMPM_ERR_INVALID, "particle count");
+    if (nvel != 1 && nvel != n) return Fail(e, CKMPM_ERR_INVALID, "velocity count must be 1 or n");
+    if (mat->kind < 0 || mat->kind > 3) return Fail(e, CKMPM_ERR_INVALID, "material kind");
+    if ((lambda_p == nullptr) != (mu_p == nullptr)) return Fail(e, CKMPM_ERR_INVALID, "lambda_p and mu_p go together");
+    if (lambda_p && mat->kind == kFluid) return Fail(e, CKMPM_ERR_INVALID, "per-particle Lame parameters need a solid");
+    ModelBuf m;
+    m.kind = mat->kind;
+    m.nattr = AttrCount(m.kind);
+    if (lambda_p) {
+        m.pp_attr = m.nattr;  // two extra state attributes travel with the particle
+        m.nattr += 2;
+        m.lam0.assign(lambda_p, lambda_p + n);
+        m.mu0.assign(mu_p, mu_p + n);
+    }
+    m.n = static_cast<uint32_t>(n);
+    m.mat.kind = mat->kind;
+    m.mat.lambda = mat->lambda;
+    m.mat.mu = mat->mu;
+    m.mat.bulk = mat->bulk;
+    m.mat.gamma = mat->gamma;
+    m.mat.viscosity = mat->viscosity;
+    m.mat.J0 = mat->J0;
+    m.mat.sin_phi = static_cast<float>(std::sin(static_cast<double>(mat->friction_angle_deg) * 3.14159265358979323846 / 180.0));
+    m.mat.mass = mass;
+    m.mat.volume = volume;
+    m.x0.assign(xyz, xyz + 3 * n);
+    m.v0.assign(vel, vel + 3 * nvel);
+    m.nvel = static_cast<uint32_t>(nvel);
+    // default dt from the initial speeds: mpm_engine.cuh:109-117
+    double vmax = 0.0;
+    for (uint64_t i = 0; i < nvel; ++i) {
+        const double s = std::sqrt(double(vel[3 * i]) * vel[3 * i] + double(vel[3 * i + 1]) * vel[3 * i + 1] +
+                                   double(vel[3 * i + 2]) * vel[3 * i + 2]);
+        vmax = std::max(vmax, s);
+    }
+    if (vmax > 0.0 && e->sim.cfl > 0.0f) e->dt_default = std::min(e->dt_default, double(e->sim.dx) * e->sim.cfl / vmax);
+    const size_t N = m.n;
+    for (int i = 0; i < 2; ++i) {
+        CK_TRY(Alloc(e, &m.st[i], N * m.nattr));
+        CK_TRY(Alloc(e, &m.perm[i], N));
+        CK_TRY(Alloc(e, &m.off[i], static_cast<size_t>(e->cap) + 1));
+    }
+    CK_TRY(Alloc(e, &m.code, N));
+    CK_TRY(Alloc(e, &m.cnt, static_cast<size_t>(e->cap) * kDirections));
+    CK_TRY(Alloc(e, &m.base, static_cast<size_t>(e->cap) * kDirections));
+    CK_TRY(Alloc(e, &m.cnt_e, e->cap));
+    CK_TRY(Alloc(e, &m.cntn, e->cap));
+    e->models.push_back(std::move(m));
+    if (model_index) *model_index = static_cast<int32_t>(e->models.size() - 1);
+    return CKMPM_OK;
+}
+
+int ckmpm_initial_setup(ckmpm_engine* e) {
+    if (!e) return CKMPM_ERR_INVALID;
+    if (e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "setup already done");
+    if (e->models.empty()) return Fail(e, CKMPM_ERR_INVALID, "no models");
+    hipStream_t s = e->stream;
+    uint32_t nmax = 0;
+    for (ModelBuf& m : e->models) nmax = std::max(nmax, m.n);
+    float *xyz = nullptr, *vel = nullptr, *lam = nullptr, *mu = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr, *keys_alt = nullptr, *vals_alt = nullptr, *rtmp = nullptr;
+    std::vector<uint32_t*> sorted(e->models.size(), nullptr);
+    const size_t mark = e->allocs.size();
+    CK_TRY(Alloc(e, &xyz, 3 * static_cast<size_t>(nmax)));
+    CK_TRY(Alloc(e, &vel, 3 * static_cast<size_t>(nmax)));
+    CK_TRY(Alloc(e, &lam, nmax));
+    CK_TRY(Alloc(e, &mu, nmax));
+    CK_TRY(Alloc(e, &vals_alt, nmax));
+    CK_TRY(Alloc(e, &keys_alt, nmax));
+    CK_TRY(Alloc(e, &rtmp, RadixTempWords(nmax)));
+    int key_bits = 1;
+    while ((1ull << key_bits) < e->dom.nblocks) ++key_bits;
+    LayoutBuf& L = e->L[0];
+    // particle blocks from FP32 positions; per-model stable sort by block key
+    for (size_t i = 0; i < e->models.size(); ++i) {
+        ModelBuf& m = e->models[i];
+        CK_TRY(Alloc(e, &keys, m.n));
+        sorted[i] = keys;
+        vals = m.perm[0];
+        CK_HIP(e, hipMemcpyAsync(xyz, m.x0.data(), 3 * sizeof(float) * m.n, hipMemcpyHostToDevice, s));
+        LaunchSetupKeys(e->dom, xyz, m.n, keys, vals, e->bitmap, e->status, s);
+        RadixSortPairs(keys, vals, keys_alt, vals_alt, m.n, key_bits, rtmp, s);
+        if (m.pp_attr) {
+            CK_HIP(e, hipMemcpyAsync(lam, m.lam0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
+            CK_HIP(e, hipMemcpyAsync(mu, m.mu0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
+        }
+        LaunchInitState(e->dom, m.kind, xyz, lam, mu, m.pp_attr, vals, m.n, m.mat.J0, m.st[0], s);
+        CK_HIP(e, hipStreamSynchronize(s));
+    }
+    ExclusiveScan(e->bitmap, e->word_off, nullptr, e->nwords, e->nwords, e->total, true, e->scan_tmp, s);
+    LaunchBitmapToKeysA(e->bitmap, e->nwords, e->word_off, e->total, L.keys, L.lc, e->cap, e->status, s);
+    for (size_t i = 0; i < e->models.size(); ++i)
+        LaunchSetupOffsets(sorted[i], e->models[i].n, L.keys, L.lc, e->cap, e->models[i].off[0], s);
+    ExtendLayout(e, L);
+    CK_HIP(e, hipMemsetAsync(e->M[0], 0, static_cast<size_t>(e->cap) * kMFloats * sizeof(float), s));
+    for (ModelBuf& m : e->models) {
+        CK_HIP(e, hipMemcpyAsync(xyz, m.x0.data(), 3 * sizeof(float) * m.n, hipMemcpyHostToDevice, s));
+        CK_HIP(e, hipMemcpyAsync(vel, m.v0.data(), 3 * sizeof(float) * m.nvel, hipMemcpyHostToDevice, s));
+        LaunchRasterize(e->dom, xyz, vel, m.n, m.nvel, m.mat.mass, L.table, L.lc, e->M[0], e->status, s);
+        CK_HIP(e, hipStreamSynchronize(s));
+    }
+    // previous layout = setup layout (the first grid update reads M[0] through it)
+    CK_HIP(e, hipMemcpyAsync(e->L[1].keys, L.keys, e->cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    CK_HIP(e, hipMemcpyAsync(e->L[1].table, L.table, static_cast<size_t>(e->dom.nblocks) * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, s));
+    CK_HIP(e, hipMemcpyAsync(e->L[1].lc, L.lc, sizeof(LayoutCounts), hipMemcpyDeviceToDevice, s));
+    // the stored state was written in the setup ordering: it is also the "previous" one
+    for (ModelBuf& m : e->models)
+        CK_HIP(e, hipMemcpyAsync(m.off[1], m.off[0], (static_cast<size_t>(e->cap) + 1) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, s));
+    e->cur = 0;
+    e->mprev = 0;
+    e->sc = 0;
+    Status st;
+    LayoutCounts lc;
+    CK_TRY(ReadStatus(e, st, lc));
+    // release the setup temporaries
+    for (size_t i = mark; i < e->allocs.size(); ++i) (void)hipFree(e->allocs[i]);
+    e->allocs.resize(mark);
+    for (ModelBuf& m : e->models) {
+        std::vector<float>().swap(m.x0);
+        std::vector<float>().swap(m.v0);
+        std::vector<float>().swap(m.lam0);
+        std::vector<float>().swap(m.mu0);
+    }
+    e->setup_done = true;
+    e->time = 0.0;
+    e->cur_dt = 0.0;
+    const int code = FlagsToCode(st.flags);
+    if (code != CKMPM_OK) return Fail(e, code, "initial setup flagged 0x" + std::to_string(st.flags));
+    return CKMPM_OK;
+}
+
+int ckmpm_substep(ckmpm_engine* e, double dt, double next_dt) {
+    CK_TRY(CheckReady(e));
+    PhaseGrid(e, dt);
+    PhaseParticles(e, dt, next_dt);
+    const hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
+    return CKMPM_OK;
+}
+
+// Simulate's frame loop (mpm_engine.cuh:514-876) with its dt rules.
+int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
+    CK_TRY(CheckReady(e));
+    const double spf = e->sim.fps ? 1.0 / e->sim.fps : std::numeric_limits<double>::infinity();
+    for (uint32_t f = 0; f < frames; ++f) {
+        const double frame_end = e->time + spf;
+        while (e->time < frame_end) {
+            PhaseGrid(e, e->cur_dt);
+            Status st;
+            LayoutCounts lc;
+            CK_TRY(ReadStatus(e, st, lc));
+            const float vmax = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits));
+            if (std::isinf(vmax) || (st.flags & kFlagNonFinite))
+                return Fail(e, CKMPM_ERR_NONFINITE, "velocity reached infinity");
+            double next_dt = std::min(e->dt_default, frame_end - e->time);
+            if (vmax > 0.0f) next_dt = std::min(next_dt, double(e->sim.dx) * e->sim.cfl / vmax);
+            PhaseParticles(e, e->cur_dt, next_dt);
+            e->cur_dt = next_dt;
+            e->time += e->cur_dt;
+        }
+    }
+    return ckmpm_synchronize(e);
+}
+
+int ckmpm_synchronize(ckmpm_engine* e) {
+    if (!e) return CKMPM_ERR_INVALID;
+    Status st;
+    LayoutCounts lc;
+    CK_TRY(ReadStatus(e, st, lc));
+    const int code = FlagsToCode(st.flags);
+    if (code != CKMPM_OK) return Fail(e, code, "device flags 0x" + std::to_string(st.flags));
+    return CKMPM_OK;
+}
+
+int ckmpm_get_counters(ckmpm_engine* e, ckmpm_counters* out) {
+    if (!e || !out) return CKMPM_ERR_INVALID;
+    Status st;
+    LayoutCounts lc;
+    CK_TRY(ReadStatus(e, st, lc));
+    out->active_blocks = lc.A;
+    out->neighbor_blocks = lc.Nb;
+    out->exterior_blocks = lc.Ex;
+    out->flags = st.flags;
+    out->max_speed = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits));
+    out->reserved = 0;
+    out->substeps = e->substeps;
+    return CKMPM_OK;
+}
+
+int ckmpm_clear_flags(ckmpm_engine* e) {
+    if (!e) return CKMPM_ERR_INVALID;
+    CK_HIP(e, hipMemsetAsync(&e->status->flags, 0, sizeof(uint32_t), e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    return CKMPM_OK;
+}
+
+int ckmpm_model_info(ckmpm_engine* e, int32_t model, uint64_t* n, int32_t* nattr) {
+    if (!e || model < 0 || model >= static_cast<int32_t>(e->models.size())) return CKMPM_ERR_INVALID;
+    if (n) *n = e->models[model].n;
+    if (nattr) *nattr = e->models[model].nattr;
+    return CKMPM_OK;
+}
+
+int ckmpm_get_positions(ckmpm_engine* e, int32_t model, float* xyz, uint64_t capacity, uint64_t* n) {
+    CK_TRY(CheckReady(e));
+    if (model < 0 || model >= static_cast<int32_t>(e->models.size()) || !xyz) return CKMPM_ERR_INVALID;
+    ModelBuf& m = e->models[model];
+    if (capacity < m.n) return Fail(e, CKMPM_ERR_INVALID, "capacity below particle count");
+    float* tmp = nullptr;
+    CK_HIP(e, hipMalloc(&tmp, 3 * sizeof(float) * static_cast<size_t>(m.n)));
+    const LayoutBuf& P = e->L[1 - e->cur];
+    LaunchExportPositions(e->dom, P.keys, P.lc, m.off[1 - e->sc], m.st[e->sc], m.n, tmp, GridFor(e->a_hint + 8), e->stream);
+    hipError_t r = hipMemcpyAsync(xyz, tmp, 3 * sizeof(float) * static_cast<size_t>(m.n), hipMemcpyDeviceToHost, e->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    (void)hipFree(tmp);
+    if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, std::string("get_positions: ") + hipGetErrorString(r));
+    if (n) *n = m.n;
+    return CKMPM_OK;
+}
+
+int ckmpm_export_layout(ckmpm_engine* e, int32_t which, uint32_t* keys, uint32_t capacity, uint32_t counts[3]) {
+    CK_TRY(CheckReady(e));
+    const LayoutBuf& L = e->L[which ? 1 - e->cur : e->cur];
+    LayoutCounts lc;
+    CK_HIP(e, hipMemcpyAsync(&lc, L.lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    if (counts) {
+        counts[0] = lc.A;
+        counts[1] = lc.Nb;
+        counts[2] = lc.Ex;
+    }
+    if (keys) {
+        if (capacity < lc.Ex) return Fail(e, CKMPM_ERR_INVALID, "capacity below exterior count");
+        CK_HIP(e, hipMemcpy(keys, L.keys, lc.Ex * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    return CKMPM_OK;
+}
+
+int ckmpm_export_ordering(ckmpm_engine* e, int32_t model, int32_t which, uint32_t* perm, uint32_t* offsets,
+                          uint32_t offsets_capacity) {
+    CK_TRY(CheckReady(e));
+    if (model < 0 || model >= static_cast<int32_t>(e->models.size())) return CKMPM_ERR_INVALID;
+    ModelBuf& m = e->models[model];
+    const int li = which ? 1 - e->cur : e->cur;
+    const int oi = which ? 1 - e->sc : e->sc;
+    LayoutCounts lc;
+    CK_HIP(e, hipMemcpyAsync(&lc, e->L[li].lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    if (perm) CK_HIP(e, hipMemcpy(perm, m.perm[oi], m.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (offsets) {
+        if (offsets_capacity < lc.A + 1) return Fail(e, CKMPM_ERR_INVALID, "offsets capacity below A + 1");
+        CK_HIP(e, hipMemcpy(offsets, m.off[oi], (lc.A + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    return CKMPM_OK;
+}
+
+int ckmpm_export_state(ckmpm_engine* e, int32_t model, float* soa) {
+    CK_TRY(CheckReady(e));
+    if (model < 0 || model >= static_cast<int32_t>(e->models.size()) || !soa) return CKMPM_ERR_INVALID;
+    ModelBuf& m = e->models[model];
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    CK_HIP(e, hipMemcpy(soa, m.st[e->sc], static_cast<size_t>(m.n) * m.nattr * sizeof(float), hipMemcpyDeviceToHost));
+    return CKMPM_OK;
+}
+
+int ckmpm_import_state(ckmpm_engine* e, int32_t model, const float* soa) {
+    CK_TRY(CheckReady(e));
+    if (model < 0 || model >= static_cast<int32_t>(e->models.size()) || !soa) return CKMPM_ERR_INVALID;
+    ModelBuf& m = e->models[model];
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    CK_HIP(e, hipMemcpy(m.st[e->sc], soa, static_cast<size_t>(m.n) * m.nattr * sizeof(float), hipMemcpyHostToDevice));
+    return CKMPM_OK;
+}
+
+int ckmpm_export_grid_m(ckmpm_engine* e, float* grid, uint64_t capacity_blocks) {
+    CK_TRY(CheckReady(e));
+    LayoutCounts lc;
+    CK_HIP(e, hipMemcpyAsync(&lc, e->L[1 - e->cur].lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    if (capacity_blocks < lc.Nb || !grid) return Fail(e, CKMPM_ERR_INVALID, "capacity below neighbour count");
+    CK_HIP(e, hipMemcpy(grid, e->M[e->mprev], static_cast<size_t>(lc.Nb) * kMFloats * sizeof(float), hipMemcpyDeviceToHost));
+    return CKMPM_OK;
+}
+
+int ckmpm_export_grid_v(ckmpm_engine* e, float* grid, uint64_t capacity_blocks) {
+    CK_TRY(CheckReady(e));
+    LayoutCounts lc;
+    CK_HIP(e, hipMemcpyAsync(&lc, e->L[1 - e->cur].lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    if (capacity_blocks < lc.Nb || !grid) return Fail(e, CKMPM_ERR_INVALID, "capacity below neighbour count");
+    CK_HIP(e, hipMemcpy(grid, e->V, static_cast<size_t>(lc.Nb) * kVFloats * sizeof(float), hipMemcpyDeviceToHost));
+    return CKMPM_OK;
+}
+
+int ckmpm_set_profiling(ckmpm_engine* e, int32_t enable) {
+    if (!e) return CKMPM_ERR_INVALID;
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    e->profile = enable != 0;
+    e->ev_used = 0;
+    return CKMPM_OK;
+}
+
+int ckmpm_get_phase_times(ckmpm_engine* e, double ms[3], uint64_t* substeps) {
+    if (!e || !ms) return CKMPM_ERR_INVALID;
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    ms[0] = ms[1] = ms[2] = 0.0;
+    const size_t n = e->ev_used / 4;
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            float t = 0.0f;
+            CK_HIP(e, hipEventElapsedTime(&t, e->events[4 * i + k], e->events[4 * i + k + 1]));
+            ms[k] += t;
+        }
+    }
+    if (substeps) *substeps = n;
+    e->ev_used = 0;
+    return CKMPM_OK;
+}
+
+int ckmpm_destroy(ckmpm_engine* e) {
+    if (!e) return CKMPM_ERR_INVALID;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
+    for (void* p : e->allocs) (void)hipFree(p);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return CKMPM_OK;
+}
+
+const char* ckmpm_last_error(const ckmpm_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// Host-side launchers of the CK-MPM kernels (implemented in kernels.hip).
+#pragma once
+
+#include "common.h"
+
+namespace ckmpm {
+
+constexpr int kMaxModels = 8;
+
+// ---------------------------------------------------------------- scans
+// Exclusive scan of n u32 values (n read from n_dev when non-null, else n_imm; the
+// launch is sized for n_cap).  When popcount is set the scanned values are the
+// population counts of the input words.  total (optional) receives the sum.
+// tmp must hold ScanTempWords(n_cap) words.
+size_t ScanTempWords(size_t n_cap);
+void ExclusiveScan(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_imm, uint32_t n_cap,
+                   uint32_t* total, bool popcount, uint32_t* tmp, hipStream_t s);
+
+// ---------------------------------------------------------------- setup
+// Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
+size_t RadixTempWords(size_t n);
+void RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int key_bits,
+                    uint32_t* tmp, hipStream_t s);
+
+void LaunchSetupKeys(const DomainDev& d, const float* xyz, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* bitmap,
+                     Status* st, hipStream_t s);
+void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t* akeys, const LayoutCounts* lc,
+                        uint32_t a_cap, uint32_t* off, hipStream_t s);
+// order: in = setup sort permutation (processing position -> input index),
+// out = identity ordering with move 13.
+// pp_attr: attribute index of per-particle (lambda, mu), 0 when the material's are used.
+void LaunchInitState(const DomainDev& d, int kind, const float* xyz, const float* lam_p, const float* mu_p, int pp_attr,
+                     uint32_t* order, uint32_t n, float J0, float* st, hipStream_t s);
+void LaunchExportPositions(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, const uint32_t* off,
+                           const float* st, uint32_t n, float* xyz, uint32_t grid, hipStream_t s);
+void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uint32_t n, uint32_t nvel, float mass,
+                     const uint32_t* table, const LayoutCounts* lc, float* M, Status* st, hipStream_t s);
+
+// ---------------------------------------------------------------- layout
+// Emit the set bits of a domain bitmap in ascending key order after position
+// base (= counts.A for the neighbour pass, counts.Nb for the exterior pass),
+// register them in the table and clear the bitmap.  pass 0 sets Nb, pass 1 sets Ex.
+void LaunchTableClear(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
+void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
+void LaunchMarkExtension(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
+                         const uint32_t* table, uint32_t* bitmap, int lo, hipStream_t s);
+void LaunchBitmapEmit(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
+                      uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap, int pass, Status* st,
+                      hipStream_t s);
+void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
+                         uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
+
+// ---------------------------------------------------------------- substep
+struct GridUpdateArgs {
+    DomainDev dom;
+    const uint32_t* keys;        // current layout
+    const LayoutCounts* lc;
+    const uint32_t* table_prev;  // layout of M_prev
+    const LayoutCounts* lc_prev;
+    const float* M_prev;
+    float* V;
+    float* M_zero;               // cleared for the current layout (next P2G target)
+    float dt, gravity, mass_clamp;
+    uint32_t cap;
+    Status* st;
+};
+void LaunchBeginSubstep(Status* st, hipStream_t s);
+void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s);
+
+struct G2P2GArgs {
+    DomainDev dom;
+    MaterialDev mat;
+    const uint32_t* keys;
+    const uint32_t* table;
+    const LayoutCounts* lc;
+    const float* V;
+    float* M;
+    const float* st_in;
+    float* st_out;
+    uint32_t n;
+    const uint32_t* perm;
+    const uint32_t* off;
+    uint32_t* code;
+    uint32_t* cnt;
+    int pp_attr;  // attribute index of per-particle (lambda, mu) in the state, 0: none
+    float dt, next_dt;
+    Status* st;
+};
+void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, hipStream_t s);
+
+struct RebinModels {
+    int nmodels;
+    const uint32_t* cnt[kMaxModels];   // [A_cur][27] direction counts per old block
+    uint32_t* cnt_e[kMaxModels];       // [Ex_cur] particles entering each exterior block
+    uint32_t* cntn[kMaxModels];        // [A_new] same, compacted
+    uint32_t* base[kMaxModels];        // [A_cur][27] first destination of (old block, direction)
+    const uint32_t* off_next[kMaxModels];
+};
+void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
+                      const RebinModels& rm, uint32_t* occ, uint32_t cap, hipStream_t s);
+void LaunchRebinCompact(const uint32_t* keys, const LayoutCounts* lc, const uint32_t* occ, const uint32_t* newidx,
+                        const RebinModels& rm, uint32_t* keys_next, uint32_t cap, hipStream_t s);
+void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
+                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap, hipStream_t s);
+void LaunchRebinScatter(const LayoutCounts* lc, const uint32_t* off, const uint32_t* code, const uint32_t* base,
+                        uint32_t* perm_next, uint32_t n, uint32_t grid, hipStream_t s);
+void LaunchSetCount(LayoutCounts* lc, int field, const uint32_t* total, uint32_t cap, Status* st, hipStream_t s);
+void LaunchFinishOffsets(uint32_t* off, const LayoutCounts* lc, uint32_t n, hipStream_t s);
+
+}  // namespace ckmpm

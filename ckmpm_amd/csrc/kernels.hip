@@ -1,0 +1,1015 @@
+// CK-MPM kernels for gfx950 (wave64).
+//
+// Data layout in HBM (all FP32 / u32, structure of arrays):
+//   particle state   st[attr][n]      u (3: position relative to the origin 4 b dx of the
+//                                     block b that processed the particle last), then
+//                                     F (9, row-major) | q = J - 1 (fluid), then logJp (DP)
+//   ordering         perm[t] = slot | move << 27: slot of the previous state array and
+//                    the particle's block move since then; off[j] block ranges
+//   grid mass/mom.   M[block][ch 4][node 128]   node = g*64 + lx*16 + ly*4 + lz
+//   grid velocity    V[block][ch 3][node 128]
+//   layout           keys[list index] -> linear block key, table[key] -> list index
+#include "kernels.h"
+#include "material.h"
+
+namespace ckmpm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr float kInv2Pi = 0.15915494309189535f;
+
+__device__ __forceinline__ uint32_t LaneId() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
+    const uint32_t lane = LaneId();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+__device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t& total, uint32_t* sh) {
+    const uint32_t lane = LaneId(), wave = threadIdx.x >> 6;
+    const uint32_t inc = WaveInclusiveScan(v);
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kThreads / 64; ++w) {
+        const uint32_t x = sh[w];
+        wbase += (w < wave) ? x : 0u;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return wbase + inc - v;
+}
+
+// Lanes of the wave holding the same small integer value (bits significant bits).
+template <int bits>
+__device__ __forceinline__ uint64_t MatchPeers(uint32_t v) {
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < bits; ++b) {
+        const uint64_t m = __ballot((v >> b) & 1u);
+        peers &= ((v >> b) & 1u) ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ uint32_t ReadCount(const uint32_t* n_dev, uint32_t n_imm) { return n_dev ? *n_dev : n_imm; }
+
+// ============================================================== scans
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kThreads * kScanItems;
+
+template <bool kPopc>
+__device__ __forceinline__ uint32_t ScanValue(const uint32_t* in, uint32_t i, uint32_t n) {
+    if (i >= n) return 0u;
+    const uint32_t v = in[i];
+    return kPopc ? static_cast<uint32_t>(__popc(v)) : v;
+}
+
+template <bool kPopc>
+__global__ __launch_bounds__(kThreads) void ScanReduceKernel(const uint32_t* in, const uint32_t* n_dev, uint32_t n_imm,
+                                                             uint32_t* partial) {
+    __shared__ uint32_t sh[kThreads / 64];
+    const uint32_t n = ReadCount(n_dev, n_imm);
+    const uint32_t base = blockIdx.x * kScanTile;
+    if (base >= n) return;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += ScanValue<kPopc>(in, base + k * kThreads + threadIdx.x, n);
+    uint32_t tot;
+    BlockExclusiveScan(s, tot, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void ScanPartialsKernel(uint32_t* partial, const uint32_t* n_dev, uint32_t n_imm,
+                                                               uint32_t* total) {
+    __shared__ uint32_t sh[kThreads / 64];
+    const uint32_t n = ReadCount(n_dev, n_imm);
+    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < ntiles; c += kThreads) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t v = i < ntiles ? partial[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = BlockExclusiveScan(v, tot, sh);
+        if (i < ntiles) partial[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <bool kPopc>
+__global__ __launch_bounds__(kThreads) void ScanDownsweepKernel(const uint32_t* in, uint32_t* out, const uint32_t* n_dev,
+                                                                uint32_t n_imm, const uint32_t* partial) {
+    __shared__ uint32_t sh[kThreads / 64];
+    const uint32_t n = ReadCount(n_dev, n_imm);
+    const uint32_t base = blockIdx.x * kScanTile;
+    if (base >= n) return;
+    // thread-contiguous items: thread i owns [base + i*8, base + i*8 + 8)
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = ScanValue<kPopc>(in, base + threadIdx.x * kScanItems + k, n);
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = partial[blockIdx.x] + BlockExclusiveScan(s, tot, sh);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t i = base + threadIdx.x * kScanItems + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+}
+
+// The reduce kernel sums strided items and the downsweep contiguous ones; both cover
+// the same tile, so the tile sums agree.
+
+// ============================================================== radix sort (setup)
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile = kThreads * kRadixItems;
+
+__global__ __launch_bounds__(kThreads) void RadixHistKernel(const uint32_t* keys, uint32_t n, int shift, uint32_t* hist,
+                                                            uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kRadixTile;
+#pragma unroll
+    for (int k = 0; k < kRadixItems; ++k) {
+        const uint32_t i = base + k * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void RadixScatterKernel(const uint32_t* keys, const uint32_t* vals,
+                                                               uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                                               int shift, const uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kThreads / 64][257];
+    run[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
+    const uint32_t lane = LaneId(), wave = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * kRadixTile;
+    for (int k = 0; k < kRadixItems; ++k) {
+        for (int i = threadIdx.x; i < (kThreads / 64) * 257; i += kThreads) (&wcnt[0][0])[i] = 0;
+        __syncthreads();
+        const uint32_t i = base + k * kThreads + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys[i] : 0u;
+        const uint32_t val = valid ? vals[i] : 0u;
+        const uint32_t digit = valid ? ((key >> shift) & 255u) : 256u;
+        const uint64_t peers = MatchPeers<9>(digit);
+        const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
+        if (rank == 0) wcnt[wave][digit] = static_cast<uint32_t>(__popcll(peers));
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[digit] + rank;
+            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][digit];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t w = 0; w < kThreads / 64; ++w) add += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// ============================================================== setup kernels
+__global__ void SetupKeysKernel(DomainDev d, const float* xyz, uint32_t n, uint32_t* keys, uint32_t* vals,
+                                uint32_t* bitmap, Status* st) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        int b[3];
+        bool out = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            b[a] = FloorDiv4(Cell0(xyz[3 * static_cast<size_t>(p) + a], d.inv_dx) - 2);
+            const int c = min(max(b[a], 0), d.dim[a] - 1);
+            out |= c != b[a];
+            b[a] = c;
+        }
+        if (out) SetFlag(st, kFlagOutOfDomain);
+        const uint32_t key = KeyOf(d, b[0], b[1], b[2]);
+        keys[p] = key;
+        vals[p] = p;
+        atomicOr(&bitmap[key >> 5], 1u << (key & 31u));
+    }
+}
+
+__global__ void SetupOffsetsKernel(const uint32_t* sorted, uint32_t n, const uint32_t* akeys, const LayoutCounts* lc,
+                                   uint32_t* off) {
+    const uint32_t A = lc->A;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j <= A; j += gridDim.x * blockDim.x) {
+        if (j == A) {
+            off[j] = n;
+            continue;
+        }
+        const uint32_t key = akeys[j];
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sorted[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        off[j] = lo;
+    }
+}
+
+// Initial state in processing order (CopyParticleArrayToParticleBuffer,
+// mpm_algorithm.cuh:973-1081): slot t holds input particle order[t], its position
+// relative to its block origin 4 b dx (exact for power-of-two dx: the product is exact
+// and the difference of two nearby numbers is exact); F = I, logJp = 0, J = J0.  The
+// ordering becomes the identity with move 13 (no move), like InitializeAdvectionBucket
+// (mpm_algorithm.cuh:949-964).
+__global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const float* lam_p, const float* mu_p,
+                                int pp_attr, uint32_t* order, uint32_t n, float J0, float* st) {
+    const size_t N = n;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t src = order[p];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float x = xyz[3 * static_cast<size_t>(src) + a];
+            const int b = min(max(FloorDiv4(Cell0(x, d.inv_dx) - 2), 0), d.dim[a] - 1);
+            st[a * N + p] = __fsub_rn(x, __fmul_rn(static_cast<float>(4 * b), d.dx));
+        }
+        order[p] = p | (13u << kMoveShift);
+        if (pp_attr) {
+            st[pp_attr * N + p] = lam_p[src];
+            st[(pp_attr + 1) * N + p] = mu_p[src];
+        }
+        if (kind == kFluid) {
+            st[3 * N + p] = J0 - 1.0f;  // the fluid state is q = J - 1
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) st[(3 + k) * N + p] = (k % 4 == 0) ? 1.0f : 0.0f;
+            if (kind == kDruckerPrager) st[12 * N + p] = 0.0f;
+        }
+    }
+}
+
+// Stencil of one sub-grid at an FP32 position (xs = x * invDx): base cell c,
+// fractional offset t and node-0 weight w0 per axis.  mpm_algorithm.cuh:1555-1569,
+// mpm_kernel.cuh:91-114 (w0 = 1 - t + sin(2 pi t) / (2 pi)).
+struct Stencil32 {
+    int c[3];
+    float t[3];
+    float w0[3];
+};
+template <int g>
+__device__ __forceinline__ void MakeStencil(const float (&xs)[3], Stencil32& s) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float shift = g == 0 ? 0.25f : -0.25f;
+        const int c = static_cast<int>(floorf(__fadd_rn(xs[a], shift)));
+        const float t = __fadd_rn(xs[a] - static_cast<float>(c), shift);
+        s.c[a] = c;
+        s.t[a] = t;
+        s.w0[a] = 1.0f - t + sinpif(2.0f * t) * kInv2Pi;
+    }
+}
+__device__ __forceinline__ float NodeWeight(const Stencil32& s, int dx, int dy, int dz) {
+    const float wx = dx ? 1.0f - s.w0[0] : s.w0[0];
+    const float wy = dy ? 1.0f - s.w0[1] : s.w0[1];
+    const float wz = dz ? 1.0f - s.w0[2] : s.w0[2];
+    return wx * wy * wz;
+}
+
+// mpm_algorithm.cuh:871-947 (Rasterize): m w and m w v0 on both grids, global atomics.
+__global__ void RasterizeKernel(DomainDev d, const float* xyz, const float* vel, uint32_t n, uint32_t nvel, float mass,
+                                const uint32_t* table, const LayoutCounts* lc, float* M, Status* st) {
+    const uint32_t Nb = lc->Nb;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        float xs[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xs[a] = __fmul_rn(xyz[3 * static_cast<size_t>(p) + a], d.inv_dx);
+        const float* v = nvel == 1 ? vel : vel + 3 * static_cast<size_t>(p);
+        const float v0 = v[0], v1 = v[1], v2 = v[2];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            Stencil32 s;
+            if (g == 0) MakeStencil<0>(xs, s);
+            else MakeStencil<1>(xs, s);
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
+                const int cx = s.c[0] + ox, cy = s.c[1] + oy, cz = s.c[2] + oz;
+                const int bx = FloorDiv4(cx), by = FloorDiv4(cy), bz = FloorDiv4(cz);
+                uint32_t bi = kEmpty;
+                if (InDomain(d, bx, by, bz)) bi = table[KeyOf(d, bx, by, bz)];
+                if (bi == kEmpty || bi >= Nb) {
+                    SetFlag(st, kFlagMissingBlock);
+                    continue;
+                }
+                const int node = g * 64 + (cx - 4 * bx) * 16 + (cy - 4 * by) * 4 + (cz - 4 * bz);
+                const float wm = NodeWeight(s, ox, oy, oz) * mass;
+                float* blk = M + static_cast<size_t>(bi) * kMFloats;
+                unsafeAtomicAdd(blk + node, wm);
+                unsafeAtomicAdd(blk + 128 + node, wm * v0);
+                unsafeAtomicAdd(blk + 256 + node, wm * v1);
+                unsafeAtomicAdd(blk + 384 + node, wm * v2);
+            }
+        }
+    }
+}
+
+// Global positions of the stored state: slot t was written while processing block j of
+// the previous ordering (off_prev), relative to that block's origin.
+__global__ __launch_bounds__(kThreads) void ExportPositionsKernel(DomainDev d, const uint32_t* keys,
+                                                                  const LayoutCounts* lc, const uint32_t* off,
+                                                                  const float* st, uint32_t n, float* xyz) {
+    const uint32_t A = lc->A;
+    const size_t N = n;
+    for (uint32_t j = blockIdx.x; j < A; j += gridDim.x) {
+        int b[3];
+        CoordsOf(d, keys[j], b[0], b[1], b[2]);
+        for (uint32_t t = off[j] + threadIdx.x; t < off[j + 1] && t < n; t += kThreads)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                xyz[3 * static_cast<size_t>(t) + k] = __fadd_rn(__fmul_rn(static_cast<float>(4 * b[k]), d.dx), st[k * N + t]);
+    }
+}
+
+// ============================================================== layout kernels
+__global__ void TableClearKernel(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table) {
+    const uint32_t ex = min(lc->Ex, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ex; i += gridDim.x * blockDim.x) table[keys[i]] = kEmpty;
+}
+
+__global__ void TableInsertAKernel(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table) {
+    const uint32_t A = min(lc->A, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) table[keys[i]] = i;
+}
+
+// Neighbour (lo = 0: b + {0,1}^3) or exterior (lo = -1: b + {-1,0,1}^3) candidates of
+// the particle blocks that are not yet in the table: mpm_algorithm.cuh:657-695.
+__global__ void MarkExtensionKernel(DomainDev d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
+                                    const uint32_t* table, uint32_t* bitmap, int lo) {
+    const uint32_t A = min(lc->A, cap);
+    const int span = 2 - lo;
+    const int per = span * span * span;
+    const uint32_t total = A * static_cast<uint32_t>(per);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const uint32_t i = q / per;
+        const int o = static_cast<int>(q % per);
+        int bx, by, bz;
+        CoordsOf(d, keys[i], bx, by, bz);
+        bx += o / (span * span) + lo;
+        by += (o / span) % span + lo;
+        bz += o % span + lo;
+        if (!InDomain(d, bx, by, bz)) continue;
+        const uint32_t key = KeyOf(d, bx, by, bz);
+        if (table[key] == kEmpty) atomicOr(&bitmap[key >> 5], 1u << (key & 31u));
+    }
+}
+
+// Emit set bits in ascending key order at position base + word_off[w] + rank.
+// pass 0 / 1: append after A / after Nb and register in the table; pass 2: particle
+// blocks of the setup (positions from 0, no table).
+__global__ void BitmapEmitKernel(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, uint32_t* keys,
+                                 uint32_t* table, const LayoutCounts* lc, uint32_t cap, int pass, Status* st) {
+    const uint32_t base = pass == 0 ? lc->A : (pass == 1 ? lc->Nb : 0u);
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+        uint32_t bits = bitmap[w];
+        if (!bits) continue;
+        uint32_t pos = base + word_off[w];
+        bitmap[w] = 0;
+        while (bits) {
+            const uint32_t b = static_cast<uint32_t>(__ffs(bits) - 1);
+            bits &= bits - 1u;
+            const uint32_t key = w * 32u + b;
+            if (pos < cap) {
+                keys[pos] = key;
+                if (table) table[key] = pos;
+            } else {
+                SetFlag(st, kFlagCapacity);
+            }
+            ++pos;
+        }
+    }
+}
+
+// counts.field = base + total (field 0: A = total, 1: Nb = A + total, 2: Ex = Nb + total)
+__global__ void SetCountKernel(LayoutCounts* lc, int field, const uint32_t* total, uint32_t cap, Status* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t v = *total;
+    if (field == 1) v += lc->A;
+    if (field == 2) v += lc->Nb;
+    if (v > cap) {
+        SetFlag(st, kFlagCapacity);
+        v = cap;
+    }
+    if (field == 0) lc->A = v;
+    if (field == 1) lc->Nb = v;
+    if (field == 2) lc->Ex = v;
+}
+
+// ============================================================== grid update
+__global__ void BeginSubstepKernel(Status* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->vmax2_bits = 0u;
+}
+
+// mpm_algorithm.cuh:738-869 fused with CopySelectedGridBlocks (:1920-1945): each grid
+// block of the current layout reads its mass / momentum from the previous layout
+// through the previous key table, writes velocities (p/m, gravity on y, sticky
+// boundary within 2 blocks of the domain edge), clears the next P2G target and
+// reduces max |v|^2 (NaN -> +inf).  128 threads per block, 2 blocks per workgroup.
+__global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
+    const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
+    const uint32_t n = threadIdx.x & 127u;
+    float vmax = 0.0f;
+    for (uint32_t i = blockIdx.x * 2 + (threadIdx.x >> 7); i < Nb; i += gridDim.x * 2) {
+        const uint32_t key = a.keys[i];
+        int b[3];
+        CoordsOf(a.dom, key, b[0], b[1], b[2]);
+        bool oob = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) oob |= (b[k] + a.dom.off[k] < 2) || (b[k] + a.dom.off[k] + 2 >= a.dom.dim[k]);
+        const uint32_t io = a.table_prev[key];
+        float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
+        if (io != kEmpty && io < Nbp) {
+            const float* src = a.M_prev + static_cast<size_t>(io) * kMFloats;
+            m = src[n];
+            p[0] = src[128 + n];
+            p[1] = src[256 + n];
+            p[2] = src[384 + n];
+        }
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        float v2 = 0.0f;
+        if (m > a.mass_clamp) {
+            const float im = 1.0f / m;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = p[k] * im;
+            v[1] += a.gravity * a.dt;
+            if (oob) v[0] = v[1] = v[2] = 0.0f;
+            v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        }
+        float* dst = a.V + static_cast<size_t>(i) * kVFloats;
+        dst[n] = v[0];
+        dst[128 + n] = v[1];
+        dst[256 + n] = v[2];
+        float* mz = a.M_zero + static_cast<size_t>(i) * kMFloats;
+        mz[n] = 0.0f;
+        mz[128 + n] = 0.0f;
+        mz[256 + n] = 0.0f;
+        mz[384 + n] = 0.0f;
+        if (isnan(v2)) {
+            v2 = INFINITY;
+            SetFlag(a.st, kFlagNonFinite);
+        }
+        vmax = fmaxf(vmax, v2);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+    if (LaneId() == 0 && vmax > 0.0f) atomicMax(&a.st->vmax2_bits, __float_as_uint(vmax));
+}
+
+// ============================================================== G2P2G
+// Fused G2P / constitutive update / P2G per particle block (mpm_algorithm.cuh:1365-1845):
+// one workgroup per particle block; the 2x2x2-block velocity halo and the P2G
+// accumulator live in LDS as 8^3 node windows of both sub-grids; each non-zero LDS
+// node is flushed once with a global float atomic.  The kernel also emits each
+// particle's block move (direction) and its deterministic rank among the block's
+// particles with the same move, for the counting-sort rebin.
+constexpr int kWin = 512;  // 8^3 nodes of one sub-grid window
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
+    __shared__ float sV[kVChannels * 2 * kWin];
+    __shared__ float sM[kMChannels * 2 * kWin];
+    __shared__ uint32_t sHalo[8];
+    __shared__ uint32_t sWave[kThreads / 64][32];
+    __shared__ uint32_t sRun[32];
+
+    const uint32_t A = a.lc->A, Nb = a.lc->Nb;
+    const uint32_t tid = threadIdx.x, lane = LaneId(), wave = tid >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const float dx = a.dom.dx, inv_dx = a.dom.inv_dx;
+    const size_t N = a.n;
+
+    // XCD-aware mapping: workgroup w runs on XCD w % 8; each XCD walks a contiguous
+    // eighth of the block list so that neighbouring blocks share its L2.
+    const uint32_t nx = 8u, xcd = blockIdx.x % nx, per = gridDim.x / nx;
+    const uint32_t lo = static_cast<uint32_t>((static_cast<uint64_t>(A) * xcd) / nx);
+    const uint32_t hi = static_cast<uint32_t>((static_cast<uint64_t>(A) * (xcd + 1)) / nx);
+
+    for (uint32_t j = lo + blockIdx.x / nx; j < hi; j += per) {
+        int bb[3];
+        CoordsOf(a.dom, a.keys[j], bb[0], bb[1], bb[2]);
+        if (tid < 8) {
+            const int bx = bb[0] + ((tid >> 2) & 1), by = bb[1] + ((tid >> 1) & 1), bz = bb[2] + (tid & 1);
+            uint32_t bi = kEmpty;
+            if (InDomain(a.dom, bx, by, bz)) bi = a.table[KeyOf(a.dom, bx, by, bz)];
+            if (bi != kEmpty && bi >= Nb) bi = kEmpty;
+            sHalo[tid] = bi;
+        }
+        if (tid < 32) sRun[tid] = 0;
+        for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) sM[i] = 0.0f;
+        __syncthreads();
+        // velocity halo: 8 blocks x 3 channels x 128 nodes
+        for (int i = tid; i < 8 * kVFloats; i += kThreads) {
+            const int o = i / kVFloats, r = i % kVFloats;
+            const int ch = r >> 7, node = r & 127, g = node >> 6;
+            const int lx = ((node >> 4) & 3) + 4 * ((o >> 2) & 1);
+            const int ly = ((node >> 2) & 3) + 4 * ((o >> 1) & 1);
+            const int lz = (node & 3) + 4 * (o & 1);
+            const uint32_t bi = sHalo[o];
+            sV[(ch * 2 + g) * kWin + lx * 64 + ly * 8 + lz] = bi != kEmpty ? a.V[static_cast<size_t>(bi) * kVFloats + r] : 0.0f;
+        }
+        if (tid == 0) {
+            bool miss = false;
+            for (int o = 0; o < 8; ++o) miss |= sHalo[o] == kEmpty;
+            if (miss) SetFlag(a.st, kFlagMissingBlock);
+        }
+        __syncthreads();
+
+        const uint32_t beg = a.off[j], end = a.off[j + 1];
+        for (uint32_t c0 = beg; c0 < end; c0 += kThreads) {
+            const uint32_t t = c0 + tid;
+            const bool active = t < end;
+            uint32_t dir = 31u;
+            if (active) {
+                const uint32_t pe = a.perm[t];
+                uint32_t s = pe & kSlotMask;
+                int mv = static_cast<int>(pe >> kMoveShift);
+                if (s >= a.n || mv >= kDirections) {
+                    SetFlag(a.st, kFlagMissingBlock);
+                    s = 0;
+                    mv = 13;
+                }
+                // position relative to this block's origin: the stored value is relative
+                // to the block that wrote it, which sits at this block - move
+                const int mov[3] = {mv / 9 - 1, (mv / 3) % 3 - 1, mv % 3 - 1};
+                float x[3], F[9], q = 0.0f, logJp = 0.0f;  // q = J - 1 (fluid)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    x[k] = __fsub_rn(a.st_in[k * N + s], __fmul_rn(static_cast<float>(4 * mov[k]), dx));
+                if (KIND == kFluid) {
+                    q = a.st_in[3 * N + s];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) F[k] = a.st_in[(3 + k) * N + s];
+                    if (KIND == kDruckerPrager) logJp = a.st_in[12 * N + s];
+                }
+                float lam = a.mat.lambda, mu = a.mat.mu;
+                if (a.pp_attr) {
+                    lam = a.st_in[a.pp_attr * N + s];
+                    mu = a.st_in[(a.pp_attr + 1) * N + s];
+                }
+                // ---------------- G2P at x (both sub-grids, 16 nodes)
+                float xs[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
+                float v[3] = {0.f, 0.f, 0.f}, B[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                bool bad = false;
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    Stencil32 sg;
+                    if (g == 0) MakeStencil<0>(xs, sg);
+                    else MakeStencil<1>(xs, sg);
+                    int l[3];
+                    float m1[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int lc = sg.c[k];
+                        bad |= lc < 0 || lc > 6;
+                        l[k] = min(max(lc, 0), 6);
+                        // D closed form per axis: second moment on the diagonal, products of
+                        // first moments off the diagonal (units of dx^2)
+                        const float w0 = sg.w0[k], w1 = 1.0f - w0, tt = sg.t[k];
+                        m1[k] = w1 - tt;
+                        D[k * 4] += w0 * tt * tt + w1 * (1.0f - tt) * (1.0f - tt);
+                    }
+                    D[1] += m1[0] * m1[1];
+                    D[2] += m1[0] * m1[2];
+                    D[5] += m1[1] * m1[2];
+                    const float* base = sV + g * kWin + l[0] * 64 + l[1] * 8 + l[2];
+#pragma unroll
+                    for (int o = 0; o < 8; ++o) {
+                        const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
+                        const float w = NodeWeight(sg, ox, oy, oz);
+                        const int off = ox * 64 + oy * 8 + oz;
+                        const float vi[3] = {base[off], base[2 * kWin + off], base[4 * kWin + off]};
+                        const float wx[3] = {w * (ox - sg.t[0]) * dx, w * (oy - sg.t[1]) * dx, w * (oz - sg.t[2]) * dx};
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            v[r] += w * vi[r];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) B[r * 3 + c] += vi[r] * wx[c];
+                        }
+                    }
+                }
+                if (bad) SetFlag(a.st, kFlagOutOfWindow);
+                const float dx2 = dx * dx;
+                D[0] *= dx2;
+                D[4] *= dx2;
+                D[8] *= dx2;
+                D[1] *= dx2;
+                D[2] *= dx2;
+                D[5] *= dx2;
+                D[3] = D[1];
+                D[6] = D[2];
+                D[7] = D[5];
+                float Di[9], C[9];
+                InverseSym3(D, Di);
+                MatMul3(B, Di, C);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    v[k] *= 0.5f;
+                    x[k] += a.dt * v[k];
+                }
+                // ---------------- deformation update and stress
+                float tau[9];
+                if (KIND == kFluid) {
+                    StressFluid(a.mat, C, a.dt, q, tau);
+                } else {
+                    float dW[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) dW[k] = C[k] * a.dt + ((k % 4 == 0) ? 1.0f : 0.0f);
+                    MatMul3(dW, F, F);
+                    if (KIND == kFixedCorotated) StressFixedCorotated(lam, mu, F, tau);
+                    else if (KIND == kNeoHookean) StressNeoHookean(lam, mu, F, tau);
+                    else StressDruckerPrager(lam, mu, a.mat.sin_phi, F, logJp, tau);
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) a.st_out[k * N + t] = x[k];
+                if (KIND == kFluid) {
+                    a.st_out[3 * N + t] = q;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) a.st_out[(3 + k) * N + t] = F[k];
+                    if (KIND == kDruckerPrager) a.st_out[12 * N + t] = logJp;
+                }
+                if (a.pp_attr) {
+                    a.st_out[a.pp_attr * N + t] = lam;
+                    a.st_out[(a.pp_attr + 1) * N + t] = mu;
+                }
+                // ---------------- block move for the rebin (FP32 grid-0 binning)
+                int dlt[3];
+                bool clamped = false;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int dd = FloorDiv4(Cell0(x[k], inv_dx) - 2);
+                    clamped |= dd < -1 || dd > 1;
+                    dlt[k] = min(max(dd, -1), 1);
+                }
+                if (clamped) SetFlag(a.st, kFlagOutOfWindow);
+                if (!InDomain(a.dom, bb[0] + dlt[0], bb[1] + dlt[1], bb[2] + dlt[2])) {
+                    SetFlag(a.st, kFlagOutOfDomain);
+                    dlt[0] = dlt[1] = dlt[2] = 0;
+                }
+                dir = static_cast<uint32_t>((dlt[0] + 1) * 9 + (dlt[1] + 1) * 3 + (dlt[2] + 1));
+                // ---------------- P2G at the new x
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
+                Stencil32 sn[2];
+                MakeStencil<0>(xs, sn[0]);
+                MakeStencil<1>(xs, sn[1]);
+                bool inwin = true;
+                float Dn[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    float m1[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int lc = sn[g].c[k];
+                        inwin &= g == 0 ? (lc >= 1 && lc <= 6) : (lc >= 0 && lc <= 6);
+                        const float w0 = sn[g].w0[k], w1 = 1.0f - w0, tt = sn[g].t[k];
+                        m1[k] = w1 - tt;
+                        Dn[k * 4] += w0 * tt * tt + w1 * (1.0f - tt) * (1.0f - tt);
+                    }
+                    Dn[1] += m1[0] * m1[1];
+                    Dn[2] += m1[0] * m1[2];
+                    Dn[5] += m1[1] * m1[2];
+                }
+                if (!inwin) {
+                    SetFlag(a.st, kFlagOutOfWindow);
+                } else {
+                    // D^ = (0.5 D')^-1 = 2 D'^-1; K = m C - nextDt V D^ tau
+                    const float h = 0.5f * dx2;
+                    Dn[0] *= h;
+                    Dn[4] *= h;
+                    Dn[8] *= h;
+                    Dn[1] *= h;
+                    Dn[2] *= h;
+                    Dn[5] *= h;
+                    Dn[3] = Dn[1];
+                    Dn[6] = Dn[2];
+                    Dn[7] = Dn[5];
+                    float Dh[9], DT[9], K[9];
+                    InverseSym3(Dn, Dh);
+                    MatMul3(Dh, tau, DT);
+                    const float sv = a.next_dt * a.mat.volume;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) K[k] = C[k] * a.mat.mass - sv * DT[k];
+                    const float mv[3] = {a.mat.mass * v[0], a.mat.mass * v[1], a.mat.mass * v[2]};
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        const Stencil32& sg = sn[g];
+                        float* base = sM + g * kWin + sg.c[0] * 64 + sg.c[1] * 8 + sg.c[2];
+#pragma unroll
+                        for (int o = 0; o < 8; ++o) {
+                            const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
+                            const float w = NodeWeight(sg, ox, oy, oz);
+                            const float xx[3] = {(ox - sg.t[0]) * dx, (oy - sg.t[1]) * dx, (oz - sg.t[2]) * dx};
+                            const int off = ox * 64 + oy * 8 + oz;
+                            atomicAdd(base + off, w * a.mat.mass);
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) {
+                                const float kx = K[r * 3 + 0] * xx[0] + K[r * 3 + 1] * xx[1] + K[r * 3 + 2] * xx[2];
+                                atomicAdd(base + (2 * (r + 1)) * kWin + off, w * (mv[r] + kx));
+                            }
+                        }
+                    }
+                }
+            }
+            // ---------------- deterministic rank among this block's particles with the same move
+            if (tid < kThreads / 64 * 32) (&sWave[0][0])[tid] = 0;
+            __syncthreads();
+            const uint64_t peers = MatchPeers<5>(dir);
+            const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
+            if (rank == 0 && dir < 27u) sWave[wave][dir] = static_cast<uint32_t>(__popcll(peers));
+            __syncthreads();
+            if (active) {
+                uint32_t r = sRun[dir] + rank;
+                for (uint32_t w = 0; w < wave; ++w) r += sWave[w][dir];
+                a.code[t] = dir | (r << 5);
+            }
+            __syncthreads();
+            if (tid < 27) {
+                uint32_t add = 0;
+                for (uint32_t w = 0; w < kThreads / 64; ++w) add += sWave[w][tid];
+                sRun[tid] += add;
+            }
+            __syncthreads();
+        }
+        // ---------------- flush: one global atomic per non-zero node and channel
+        for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) {
+            const float val = sM[i];
+            if (val == 0.0f) continue;
+            const int ch = i / (2 * kWin), g = (i / kWin) & 1, l = i & (kWin - 1);
+            const int lx = l >> 6, ly = (l >> 3) & 7, lz = l & 7;
+            const int o = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
+            const uint32_t bi = sHalo[o];
+            if (bi == kEmpty) continue;
+            const int node = g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
+            unsafeAtomicAdd(a.M + static_cast<size_t>(bi) * kMFloats + ch * 128 + node, val);
+        }
+        if (tid < 27) a.cnt[static_cast<size_t>(j) * kDirections + tid] = sRun[tid];
+        __syncthreads();
+    }
+}
+
+// ============================================================== rebin
+__device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
+    ox = d / 9 - 1;
+    oy = (d / 3) % 3 - 1;
+    oz = d % 3 - 1;
+}
+
+// Particles entering each exterior block e: sum over the 27 possible source blocks
+// (e - delta, particle blocks of the current layout) of their count for move delta.
+__global__ void RebinCountKernel(DomainDev d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
+                                 RebinModels rm, uint32_t* occ, uint32_t cap) {
+    const uint32_t A = lc->A, Ex = min(lc->Ex, cap);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < Ex; e += gridDim.x * blockDim.x) {
+        int bx, by, bz;
+        CoordsOf(d, keys[e], bx, by, bz);
+        uint32_t tot[kMaxModels];
+        for (int m = 0; m < rm.nmodels; ++m) tot[m] = 0;
+        for (int dd = 0; dd < kDirections; ++dd) {
+            int ox, oy, oz;
+            DirOffset(dd, ox, oy, oz);
+            const int sx = bx - ox, sy = by - oy, sz = bz - oz;
+            if (!InDomain(d, sx, sy, sz)) continue;
+            const uint32_t js = table[KeyOf(d, sx, sy, sz)];
+            if (js >= A) continue;  // also kEmpty
+            for (int m = 0; m < rm.nmodels; ++m) tot[m] += rm.cnt[m][static_cast<size_t>(js) * kDirections + dd];
+        }
+        uint32_t any = 0;
+        for (int m = 0; m < rm.nmodels; ++m) {
+            rm.cnt_e[m][e] = tot[m];
+            any |= tot[m];
+        }
+        occ[e] = any ? 1u : 0u;
+    }
+}
+
+__global__ void RebinCompactKernel(const uint32_t* keys, const LayoutCounts* lc, const uint32_t* occ,
+                                   const uint32_t* newidx, RebinModels rm, uint32_t* keys_next, uint32_t cap) {
+    const uint32_t Ex = min(lc->Ex, cap);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < Ex; e += gridDim.x * blockDim.x) {
+        if (!occ[e]) continue;
+        const uint32_t i = newidx[e];
+        if (i >= cap) continue;
+        keys_next[i] = keys[e];
+        for (int m = 0; m < rm.nmodels; ++m) rm.cntn[m][i] = rm.cnt_e[m][e];
+    }
+}
+
+// First destination of every (source block, move) pair entering an occupied block:
+// sources are ordered by their list index, so the destination order is the stable
+// order of the previous processing position.  Half a wave (32 lanes) per exterior
+// block: lane d owns move d; the ordering sum is done with lane shuffles.
+__global__ __launch_bounds__(kThreads) void RebinBaseKernel(DomainDev d, const uint32_t* keys, const uint32_t* table,
+                                                            const LayoutCounts* lc, const uint32_t* occ,
+                                                            const uint32_t* newidx, RebinModels rm, uint32_t cap) {
+    const uint32_t A = lc->A, Ex = min(lc->Ex, cap);
+    const uint32_t lane = LaneId(), half = lane >> 5, dd = lane & 31u;
+    const uint32_t waves = blockDim.x >> 6;
+    const uint32_t step = gridDim.x * waves * 2;
+    for (uint32_t e0 = (blockIdx.x * waves + (threadIdx.x >> 6)) * 2; e0 < Ex; e0 += step) {
+        const uint32_t e = e0 + half;
+        const bool valid = e < Ex && occ[e] != 0u;
+        uint32_t js = kEmpty;
+        if (valid && dd < static_cast<uint32_t>(kDirections)) {
+            int bx, by, bz, ox, oy, oz;
+            CoordsOf(d, keys[e], bx, by, bz);
+            DirOffset(static_cast<int>(dd), ox, oy, oz);
+            const int sx = bx - ox, sy = by - oy, sz = bz - oz;
+            if (InDomain(d, sx, sy, sz)) {
+                const uint32_t v = table[KeyOf(d, sx, sy, sz)];
+                js = v < A ? v : kEmpty;
+            }
+        }
+        const uint32_t ni = valid ? newidx[e] : 0u;
+        for (int m = 0; m < rm.nmodels; ++m) {
+            const uint32_t c = js != kEmpty ? rm.cnt[m][static_cast<size_t>(js) * kDirections + dd] : 0u;
+            uint32_t before = 0;
+            for (int k = 0; k < kDirections; ++k) {
+                const int src = static_cast<int>(half << 5) + k;
+                const uint32_t jk = __shfl(js, src, 64);
+                const uint32_t ck = __shfl(c, src, 64);
+                before += jk < js ? ck : 0u;
+            }
+            if (js != kEmpty && ni < cap) rm.base[m][static_cast<size_t>(js) * kDirections + dd] = rm.off_next[m][ni] + before;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void RebinScatterKernel(const LayoutCounts* lc, const uint32_t* off,
+                                                               const uint32_t* code, const uint32_t* base,
+                                                               uint32_t* perm_next, uint32_t n) {
+    const uint32_t A = lc->A;
+    for (uint32_t j = blockIdx.x; j < A; j += gridDim.x) {
+        const uint32_t beg = off[j], end = off[j + 1];
+        const uint32_t* bj = base + static_cast<size_t>(j) * kDirections;
+        for (uint32_t t = beg + threadIdx.x; t < end; t += kThreads) {
+            const uint32_t c = code[t];
+            if ((c & 31u) >= static_cast<uint32_t>(kDirections)) continue;
+            const uint32_t pos = bj[c & 31u] + (c >> 5);
+            if (pos < n) perm_next[pos] = t | ((c & 31u) << kMoveShift);
+        }
+    }
+}
+
+__global__ void FinishOffsetsKernel(uint32_t* off, const LayoutCounts* lc, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) off[lc->A] = n;
+}
+
+uint32_t Blocks(uint64_t n, uint32_t threads, uint32_t maxb = 4096) {
+    uint64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > maxb) b = maxb;
+    return static_cast<uint32_t>(b);
+}
+
+}  // namespace
+
+// ============================================================== launchers
+size_t ScanTempWords(size_t n_cap) { return (n_cap + kScanTile - 1) / kScanTile + 1; }
+
+void ExclusiveScan(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_imm, uint32_t n_cap,
+                   uint32_t* total, bool popcount, uint32_t* tmp, hipStream_t s) {
+    const uint32_t tiles = static_cast<uint32_t>((static_cast<uint64_t>(n_cap) + kScanTile - 1) / kScanTile);
+    const uint32_t grid = tiles ? tiles : 1;
+    if (popcount) ScanReduceKernel<true><<<grid, kThreads, 0, s>>>(in, n_dev, n_imm, tmp);
+    else ScanReduceKernel<false><<<grid, kThreads, 0, s>>>(in, n_dev, n_imm, tmp);
+    ScanPartialsKernel<<<1, kThreads, 0, s>>>(tmp, n_dev, n_imm, total);
+    if (popcount) ScanDownsweepKernel<true><<<grid, kThreads, 0, s>>>(in, out, n_dev, n_imm, tmp);
+    else ScanDownsweepKernel<false><<<grid, kThreads, 0, s>>>(in, out, n_dev, n_imm, tmp);
+}
+
+size_t RadixTempWords(size_t n) {
+    const size_t tiles = (n + kRadixTile - 1) / kRadixTile;
+    const size_t hist = 256 * (tiles ? tiles : 1);
+    return hist + ScanTempWords(hist);
+}
+
+void RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int key_bits,
+                    uint32_t* tmp, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t tiles = (n + kRadixTile - 1) / kRadixTile;
+    uint32_t* hist = tmp;
+    uint32_t* scan_tmp = tmp + 256 * static_cast<size_t>(tiles);
+    uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+    int passes = 0;
+    for (int shift = 0; shift < key_bits; shift += 8, ++passes) {
+        RadixHistKernel<<<tiles, kThreads, 0, s>>>(ki, n, shift, hist, tiles);
+        ExclusiveScan(hist, hist, nullptr, 256 * tiles, 256 * tiles, nullptr, false, scan_tmp, s);
+        RadixScatterKernel<<<tiles, kThreads, 0, s>>>(ki, vi, ko, vo, n, shift, hist, tiles);
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    if (passes & 1) {
+        (void)hipMemcpyAsync(keys, ki, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(vals, vi, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+    }
+}
+
+void LaunchSetupKeys(const DomainDev& d, const float* xyz, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* bitmap,
+                     Status* st, hipStream_t s) {
+    SetupKeysKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, xyz, n, keys, vals, bitmap, st);
+}
+void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t* akeys, const LayoutCounts* lc,
+                        uint32_t a_cap, uint32_t* off, hipStream_t s) {
+    SetupOffsetsKernel<<<Blocks(a_cap + 1, kThreads), kThreads, 0, s>>>(sorted_keys, n, akeys, lc, off);
+}
+void LaunchInitState(const DomainDev& d, int kind, const float* xyz, const float* lam_p, const float* mu_p, int pp_attr,
+                     uint32_t* order, uint32_t n, float J0, float* st, hipStream_t s) {
+    InitStateKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, kind, xyz, lam_p, mu_p, pp_attr, order, n, J0, st);
+}
+void LaunchExportPositions(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, const uint32_t* off,
+                           const float* st, uint32_t n, float* xyz, uint32_t grid, hipStream_t s) {
+    ExportPositionsKernel<<<grid, kThreads, 0, s>>>(d, keys, lc, off, st, n, xyz);
+}
+void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uint32_t n, uint32_t nvel, float mass,
+                     const uint32_t* table, const LayoutCounts* lc, float* M, Status* st, hipStream_t s) {
+    RasterizeKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, xyz, vel, n, nvel, mass, table, lc, M, st);
+}
+void LaunchTableClear(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s) {
+    TableClearKernel<<<Blocks(cap, kThreads, 1024), kThreads, 0, s>>>(keys, lc, cap, table);
+}
+void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s) {
+    TableInsertAKernel<<<Blocks(cap, kThreads, 1024), kThreads, 0, s>>>(keys, lc, cap, table);
+}
+void LaunchMarkExtension(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
+                         const uint32_t* table, uint32_t* bitmap, int lo, hipStream_t s) {
+    const uint32_t per = lo == 0 ? 8 : 27;
+    MarkExtensionKernel<<<Blocks(static_cast<uint64_t>(cap) * per, kThreads), kThreads, 0, s>>>(d, keys, lc, cap, table,
+                                                                                                 bitmap, lo);
+}
+void LaunchBitmapEmit(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
+                      uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap, int pass, Status* st,
+                      hipStream_t s) {
+    BitmapEmitKernel<<<Blocks(nwords, kThreads), kThreads, 0, s>>>(bitmap, nwords, word_off, keys, table, lc, cap, pass,
+                                                                    st);
+    SetCountKernel<<<1, 64, 0, s>>>(lc, pass + 1, total, cap, st);
+}
+void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
+                         uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s) {
+    BitmapEmitKernel<<<Blocks(nwords, kThreads), kThreads, 0, s>>>(bitmap, nwords, word_off, keys, nullptr, lc, cap, 2,
+                                                                    st);
+    SetCountKernel<<<1, 64, 0, s>>>(lc, 0, total, cap, st);
+}
+void LaunchBeginSubstep(Status* st, hipStream_t s) { BeginSubstepKernel<<<1, 64, 0, s>>>(st); }
+void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s) {
+    GridUpdateKernel<<<grid, kThreads, 0, s>>>(a);
+}
+void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
+    grid = ((grid + 7) / 8) * 8;
+    switch (a.mat.kind) {
+        case kFixedCorotated: G2P2GKernel<kFixedCorotated><<<grid, kThreads, 0, s>>>(a); break;
+        case kNeoHookean: G2P2GKernel<kNeoHookean><<<grid, kThreads, 0, s>>>(a); break;
+        case kDruckerPrager: G2P2GKernel<kDruckerPrager><<<grid, kThreads, 0, s>>>(a); break;
+        default: G2P2GKernel<kFluid><<<grid, kThreads, 0, s>>>(a); break;
+    }
+}
+void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
+                      const RebinModels& rm, uint32_t* occ, uint32_t cap, hipStream_t s) {
+    RebinCountKernel<<<Blocks(cap, kThreads), kThreads, 0, s>>>(d, keys, table, lc, rm, occ, cap);
+}
+void LaunchRebinCompact(const uint32_t* keys, const LayoutCounts* lc, const uint32_t* occ, const uint32_t* newidx,
+                        const RebinModels& rm, uint32_t* keys_next, uint32_t cap, hipStream_t s) {
+    RebinCompactKernel<<<Blocks(cap, kThreads), kThreads, 0, s>>>(keys, lc, occ, newidx, rm, keys_next, cap);
+}
+void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
+                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap, hipStream_t s) {
+    RebinBaseKernel<<<Blocks(cap, 8), kThreads, 0, s>>>(d, keys, table, lc, occ, newidx, rm, cap);
+}
+void LaunchRebinScatter(const LayoutCounts* lc, const uint32_t* off, const uint32_t* code, const uint32_t* base,
+                        uint32_t* perm_next, uint32_t n, uint32_t grid, hipStream_t s) {
+    RebinScatterKernel<<<grid, kThreads, 0, s>>>(lc, off, code, base, perm_next, n);
+}
+void LaunchSetCount(LayoutCounts* lc, int field, const uint32_t* total, uint32_t cap, Status* st, hipStream_t s) {
+    SetCountKernel<<<1, 64, 0, s>>>(lc, field, total, cap, st);
+}
+void LaunchFinishOffsets(uint32_t* off, const LayoutCounts* lc, uint32_t n, hipStream_t s) {
+    FinishOffsetsKernel<<<1, 64, 0, s>>>(off, lc, n);
+}
+
+}  // namespace ckmpm

@@ -1,0 +1,236 @@
+"""Python mirror of the reference's engine API over the C-ABI.
+
+Names follow the reference (include/mpm_engine.cuh, mpm_material.cuh, mpm_model.h,
+mpm_domain.h): ``MPMDomain``, ``MPMMaterial``, ``MPMModel``, ``MPMConfig`` and
+``MPMEngine`` with ``Initialize`` / ``InitializeParticle`` / ``InitialSetup`` /
+``Simulate``.  Unlike the reference there is no singleton (any number of engines) and
+no call exits the process: errors raise ``CkmpmError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import native
+from .native import CkmpmError, _f32, _ptr, _f32p, _u32p
+from .scenes import DRUCKER_PRAGER, FIXED_COROTATED, FLUID, NEO_HOOKEAN, Scene
+
+NATTR = {FIXED_COROTATED: 12, NEO_HOOKEAN: 12, DRUCKER_PRAGER: 13, FLUID: 4}
+
+
+@dataclasses.dataclass
+class MPMDomain:
+    """MPMDomain<MPMDomainRange<...>, MPMDomainOffset<...>> (mpm_domain.h:53-61), in blocks."""
+    dim: Sequence[int]
+    offset: Sequence[int] = (0, 0, 0)
+
+
+@dataclasses.dataclass
+class MPMConfig:
+    """MPMConfigBase getters (mpm_config.h:17-123)."""
+    dx: float
+    gravity: float = 0.0
+    mass_clamp: float = 0.0
+    cfl: float = 0.5
+    dt: float = 1e-4
+    fps: int = 48
+    max_active_block_count: int = 1 << 16
+
+
+@dataclasses.dataclass
+class MPMMaterial:
+    """MPMMaterial<Model>::parameter_ (mpm_material.cuh:62-175)."""
+    kind: int
+    lam: float = 0.0
+    mu: float = 0.0
+    bulk: float = 0.0
+    gamma: float = 0.0
+    viscosity: float = 0.0
+    J0: float = 1.0
+    friction_angle_deg: float = 30.0
+
+
+@dataclasses.dataclass
+class MPMModel:
+    """MPMModel (mpm_model.h:10-60): constant particle mass / volume, positions, 1 or n velocities."""
+    mass: float
+    volume: float
+    positions: np.ndarray
+    velocities: np.ndarray
+    material: MPMMaterial
+    lam_p: Optional[np.ndarray] = None
+    mu_p: Optional[np.ndarray] = None
+
+
+class MPMEngine:
+    def __init__(self):
+        self._lib = native.load()
+        self._h = ctypes.c_void_p()
+        self.models: List[MPMModel] = []
+
+    # -- lifecycle ------------------------------------------------------------------
+    def _check(self, rc: int, where: str):
+        if rc != native.OK:
+            msg = self._lib.ckmpm_last_error(self._h).decode() if self._h else ""
+            raise CkmpmError(rc, where, msg)
+
+    def Initialize(self, domain: MPMDomain, config: MPMConfig):
+        d = native.DomainDesc((ctypes.c_int32 * 3)(*domain.dim), (ctypes.c_int32 * 3)(*domain.offset))
+        s = native.SimDesc(config.dx, config.gravity, config.mass_clamp, config.cfl, config.dt, config.fps,
+                           config.max_active_block_count)
+        self.config = config
+        self.domain = domain
+        rc = self._lib.ckmpm_create(ctypes.byref(d), ctypes.byref(s), ctypes.byref(self._h))
+        self._check(rc, "ckmpm_create")
+
+    def InitializeParticle(self, models: Sequence[MPMModel]):
+        for m in models:
+            x = _f32(m.positions).reshape(-1, 3)
+            v = _f32(m.velocities).reshape(-1, 3)
+            mat = m.material
+            md = native.MaterialDesc(mat.kind, mat.lam, mat.mu, mat.bulk, mat.gamma, mat.viscosity, mat.J0,
+                                     mat.friction_angle_deg)
+            lp = None if m.lam_p is None else _f32(m.lam_p)
+            mp = None if m.mu_p is None else _f32(m.mu_p)
+            idx = ctypes.c_int32()
+            rc = self._lib.ckmpm_add_model(self._h, ctypes.byref(md), m.mass, m.volume, _ptr(x, _f32p), x.shape[0],
+                                           _ptr(v, _f32p), v.shape[0], _ptr(lp, _f32p), _ptr(mp, _f32p),
+                                           ctypes.byref(idx))
+            self._check(rc, "ckmpm_add_model")
+            self.models.append(m)
+
+    def InitialSetup(self):
+        self._check(self._lib.ckmpm_initial_setup(self._h), "ckmpm_initial_setup")
+
+    def Substep(self, dt: float, next_dt: float):
+        self._check(self._lib.ckmpm_substep(self._h, float(dt), float(next_dt)), "ckmpm_substep")
+
+    def Simulate(self, frames: int):
+        """Frame loop with the reference dt rules (mpm_engine.cuh:514-876)."""
+        self._check(self._lib.ckmpm_run_frames(self._h, int(frames)), "ckmpm_run_frames")
+
+    def Synchronize(self):
+        self._check(self._lib.ckmpm_synchronize(self._h), "ckmpm_synchronize")
+
+    def ClearFlags(self):
+        self._check(self._lib.ckmpm_clear_flags(self._h), "ckmpm_clear_flags")
+
+    def counters(self) -> native.Counters:
+        c = native.Counters()
+        self._check(self._lib.ckmpm_get_counters(self._h, ctypes.byref(c)), "ckmpm_get_counters")
+        return c
+
+    def set_profiling(self, enable: bool):
+        self._check(self._lib.ckmpm_set_profiling(self._h, int(enable)), "set_profiling")
+
+    def phase_times(self):
+        """(grid update, G2P2G, rebin) device milliseconds summed since the last call, substeps."""
+        ms = (ctypes.c_double * 3)()
+        n = ctypes.c_uint64()
+        self._check(self._lib.ckmpm_get_phase_times(self._h, ms, ctypes.byref(n)), "get_phase_times")
+        return tuple(ms), int(n.value)
+
+    def close(self):
+        if self._h:
+            self._lib.ckmpm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state access ---------------------------------------------------------------
+    def n(self, model: int) -> int:
+        n = ctypes.c_uint64()
+        na = ctypes.c_int32()
+        self._check(self._lib.ckmpm_model_info(self._h, model, ctypes.byref(n), ctypes.byref(na)), "model_info")
+        return int(n.value)
+
+    def nattr(self, model: int) -> int:
+        n = ctypes.c_uint64()
+        na = ctypes.c_int32()
+        self._check(self._lib.ckmpm_model_info(self._h, model, ctypes.byref(n), ctypes.byref(na)), "model_info")
+        return int(na.value)
+
+    def positions(self, model: int) -> np.ndarray:
+        n = self.n(model)
+        out = np.zeros((n, 3), np.float32)
+        got = ctypes.c_uint64()
+        self._check(self._lib.ckmpm_get_positions(self._h, model, _ptr(out, _f32p), n, ctypes.byref(got)),
+                    "get_positions")
+        return out
+
+    def layout(self, which: int = 0):
+        counts = np.zeros(3, np.uint32)
+        self._check(self._lib.ckmpm_export_layout(self._h, which, None, 0, _ptr(counts, _u32p)), "export_layout")
+        keys = np.zeros(int(counts[2]), np.uint32)
+        self._check(self._lib.ckmpm_export_layout(self._h, which, _ptr(keys, _u32p), keys.shape[0],
+                                                  _ptr(counts, _u32p)), "export_layout")
+        return keys, tuple(int(c) for c in counts)
+
+    def ordering(self, model: int, which: int = 0):
+        """(perm, offsets) of the current (0) or previous (1) ordering; perm = slot | move << 27."""
+        A = self.layout(which)[1][0]
+        perm = np.zeros(self.n(model), np.uint32)
+        off = np.zeros(A + 1, np.uint32)
+        self._check(self._lib.ckmpm_export_ordering(self._h, model, which, _ptr(perm, _u32p), _ptr(off, _u32p),
+                                                    A + 1), "export_ordering")
+        return perm, off
+
+    def global_state(self, model: int) -> np.ndarray:
+        """State in the reference's variables, FP64: global positions (exact origin + u) and,
+        for a fluid, J = 1 + q."""
+        st = self.state(model).astype(np.float64)
+        keys, (A, _, _) = self.layout(1)
+        _, off = self.ordering(model, 1)
+        dim = self.domain.dim
+        blk = np.repeat(np.arange(A), np.diff(off.astype(np.int64)))
+        k = keys[blk].astype(np.int64)
+        coords = np.stack([k // (dim[1] * dim[2]), (k // dim[2]) % dim[1], k % dim[2]])
+        st[:3] += 4.0 * coords * np.float64(np.float32(self.config.dx))
+        if self.models[model].material.kind == FLUID:
+            st[3] += 1.0
+        return st
+
+    def state(self, model: int) -> np.ndarray:
+        n, na = self.n(model), self.nattr(model)
+        out = np.zeros((na, n), np.float32)
+        self._check(self._lib.ckmpm_export_state(self._h, model, _ptr(out, _f32p)), "export_state")
+        return out
+
+    def set_state(self, model: int, soa: np.ndarray):
+        s = _f32(soa)
+        self._check(self._lib.ckmpm_import_state(self._h, model, _ptr(s, _f32p)), "import_state")
+
+    def grid_m(self) -> np.ndarray:
+        nb = self.layout(1)[1][1]
+        out = np.zeros((nb, 4, 128), np.float32)
+        self._check(self._lib.ckmpm_export_grid_m(self._h, _ptr(out, _f32p), nb), "export_grid_m")
+        return out
+
+    def grid_v(self) -> np.ndarray:
+        nb = self.layout(1)[1][1]
+        out = np.zeros((nb, 3, 128), np.float32)
+        self._check(self._lib.ckmpm_export_grid_v(self._h, _ptr(out, _f32p), nb), "export_grid_v")
+        return out
+
+
+def engine_from_scene(scene: Scene, capacity: Optional[int] = None) -> MPMEngine:
+    """Build and set up an engine for a synthetic scene."""
+    eng = MPMEngine()
+    eng.Initialize(MPMDomain(scene.dim, scene.offset),
+                   MPMConfig(dx=scene.dx, gravity=scene.gravity, mass_clamp=scene.mass_clamp, dt=scene.dt,
+                             max_active_block_count=capacity or scene.block_capacity()))
+    models = []
+    for m in scene.models:
+        mat = MPMMaterial(m.material.kind, m.material.lam, m.material.mu, m.material.bulk, m.material.gamma,
+                          m.material.viscosity, m.material.J0, m.material.friction_deg)
+        models.append(MPMModel(m.mass, m.volume, m.x, m.v, mat, m.lam_p, m.mu_p))
+    eng.InitializeParticle(models)
+    eng.InitialSetup()
+    return eng

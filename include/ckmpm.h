@@ -1,0 +1,165 @@
+/*
+ * ckmpm.h - C-ABI of the MI355X-native Compact-Kernel MPM substep (libckmpm.so).
+ *
+ * The reference exposes no C-ABI: its API is a header-only C++ template engine
+ * (include/mpm_engine.cuh) whose configuration is compile-time.  Each entry point
+ * below names the reference interface it replaces (file:line, relative to the
+ * reference tree).  All pointers are host pointers; sizes are element counts.
+ * Every call returns CKMPM_OK or an error code; ckmpm_last_error() gives the text.
+ * One engine is single-threaded (the caller serialises calls on it); any number of
+ * engines may coexist in a process (the reference engine is a singleton,
+ * mpm_engine.cuh:80-84).  No call terminates the process (the reference calls
+ * exit()/abort(), mpm_engine.cuh:545-549,739-744,908).
+ */
+#ifndef CKMPM_H
+#define CKMPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ckmpm_engine ckmpm_engine;
+
+/* status codes (the reference: cudaCheckError -> exit, cuda_util.cuh:9-22; abort on
+ * capacity, mpm_engine.cuh:250-254,307-312,739-744,809-816,858-865) */
+enum {
+    CKMPM_OK = 0,
+    CKMPM_ERR_INVALID = 1,        /* bad argument or call order */
+    CKMPM_ERR_HIP = 2,            /* HIP runtime error */
+    CKMPM_ERR_CAPACITY = 3,       /* block list exceeded block_capacity */
+    CKMPM_ERR_NONFINITE = 4,      /* NaN / inf grid velocity */
+    CKMPM_ERR_OUT_OF_WINDOW = 5,  /* a particle moved farther than its P2G window */
+    CKMPM_ERR_OUT_OF_DOMAIN = 6,  /* a particle block left the domain */
+    CKMPM_ERR_NO_DEVICE = 7       /* no gfx950 device / library built without HIP */
+};
+
+/* device error flag bits reported in ckmpm_counters.flags */
+#define CKMPM_FLAG_OUT_OF_WINDOW 0x1u
+#define CKMPM_FLAG_OUT_OF_DOMAIN 0x2u
+#define CKMPM_FLAG_NONFINITE 0x4u
+#define CKMPM_FLAG_CAPACITY 0x8u
+#define CKMPM_FLAG_MISSING_BLOCK 0x10u
+
+/* constitutive models: mpm_material.cuh:10-18 (FixedCorotated, DruckerPragerStvkhencky,
+ * Fluid) plus Neo-Hookean, which the reference lacks */
+enum {
+    CKMPM_FIXED_COROTATED = 0,
+    CKMPM_NEO_HOOKEAN = 1,
+    CKMPM_DRUCKER_PRAGER = 2,
+    CKMPM_FLUID = 3
+};
+
+/* MPMDomain<MPMDomainRange<Dx,Dy,Dz>, MPMDomainOffset<Ox,Oy,Oz>> (mpm_domain.h:9-61):
+ * extent and offset in 4^3-cell blocks */
+typedef struct {
+    int32_t dim[3];
+    int32_t offset[3];
+} ckmpm_domain_desc;
+
+/* MPMConfigBase getters (mpm_config.h:17-123) as runtime fields */
+typedef struct {
+    float dx;                /* GetDx */
+    float gravity;           /* GetGravity (applied along y) */
+    float mass_clamp;        /* GetMassClamp */
+    float cfl;               /* GetCfl */
+    float dt;                /* GetDt (default / maximum dt) */
+    uint32_t fps;            /* GetFps */
+    uint32_t block_capacity; /* GetMaxActiveBlockCount */
+} ckmpm_sim_desc;
+
+/* MPMMaterial<Model>::parameter_ (mpm_material.cuh:62-175) */
+typedef struct {
+    int32_t kind;             /* CKMPM_FIXED_COROTATED ... */
+    float lambda, mu;         /* FC, NH, DP */
+    float bulk, gamma;        /* fluid: pressure = bulk (J^-gamma - 1) */
+    float viscosity, J0;      /* fluid */
+    float friction_angle_deg; /* DP; the reference hard-codes 30 (mpm_material.cuh:369) */
+} ckmpm_material_desc;
+
+typedef struct {
+    uint32_t active_blocks;   /* particle blocks (activePartitionBlockCount_) */
+    uint32_t neighbor_blocks; /* grid blocks (activePartitionNeighborBlockCount_) */
+    uint32_t exterior_blocks; /* activePartitionExteriorBlockCount_ */
+    uint32_t flags;           /* CKMPM_FLAG_* accumulated since the last clear */
+    float max_speed;          /* sqrt(max |v|^2) of the last grid update */
+    uint32_t reserved;
+    uint64_t substeps;
+} ckmpm_counters;
+
+/* MPMEngine::GetInstance + Initialize (mpm_engine.cuh:80-84,170-191) */
+int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckmpm_engine** out);
+
+/* MPMEngine::InitializeParticle for one MPMModel (mpm_engine.cuh:91-168,
+ * mpm_model.h:19-33).  vel holds nvel = 1 or n velocities.  lambda_p / mu_p are
+ * optional per-particle Lame parameters of a solid (both NULL: the material's); they
+ * become two extra state attributes that move with the particles. */
+int ckmpm_add_model(ckmpm_engine* eng, const ckmpm_material_desc* material, float mass, float volume,
+                    const float* xyz, uint64_t n, const float* vel, uint64_t nvel, const float* lambda_p,
+                    const float* mu_p, int32_t* model_index);
+
+/* MPMEngine::InitialSetup (mpm_engine.cuh:215-383): binning, block activation,
+ * rasterize; then the first grid update with dt = 0. */
+int ckmpm_initial_setup(ckmpm_engine* eng);
+
+/* One iteration of the Simulate substep loop (mpm_engine.cuh:519-876) with
+ * caller-chosen dt (advection) and next_dt (stress deposit): grid update with dt,
+ * G2P2G, rebin.  Enqueued on the engine's stream; returns without a host sync.
+ * The reference's first iteration has dt = 0 (mpm_engine.cuh:479). */
+int ckmpm_substep(ckmpm_engine* eng, double dt, double next_dt);
+
+/* Simulate's frame loop with the reference dt rules (mpm_engine.cuh:514-876):
+ * next_dt = min(default dt, time to frame end, dx cfl / max|v|); stops on a
+ * non-finite velocity.  Runs `frames` frames. */
+int ckmpm_run_frames(ckmpm_engine* eng, uint32_t frames);
+
+/* Waits for the stream; returns the first error encoded in the device flags. */
+int ckmpm_synchronize(ckmpm_engine* eng);
+int ckmpm_get_counters(ckmpm_engine* eng, ckmpm_counters* out);
+int ckmpm_clear_flags(ckmpm_engine* eng);
+
+/* RetrieveParticleBuffer + ExportParticleToModel (mpm_algorithm.cuh:1977-2020,
+ * mpm_engine.cuh:385-454): global FP32 positions of one model, AoS xyz, storage order. */
+int ckmpm_get_positions(ckmpm_engine* eng, int32_t model, float* xyz, uint64_t capacity, uint64_t* n);
+
+/* --- state export / import, in the engine's storage layout.  Used by parity tests
+ * and by callers that checkpoint; no reference counterpart (its buffers are only
+ * reachable through the debug printers, mpm_particle.cuh:96-200). */
+int ckmpm_model_info(ckmpm_engine* eng, int32_t model, uint64_t* n, int32_t* nattr);
+/* which = 0: current layout, 1: previous layout (the one the grid mass/momentum and
+ * the stored particle state were produced in) */
+int ckmpm_export_layout(ckmpm_engine* eng, int32_t which, uint32_t* keys, uint32_t capacity, uint32_t counts[3]);
+/* Ordering of the current (which = 0) or previous (1) layout: perm[t] = slot | move << 27
+ * (slot of the stored state read by processing position t; move = (dx+1)*9+(dy+1)*3+(dz+1)
+ * of the particle's block since the state was written), offsets[j] = first position of
+ * block j (A + 1 entries). */
+int ckmpm_export_ordering(ckmpm_engine* eng, int32_t model, int32_t which, uint32_t* perm, uint32_t* offsets,
+                          uint32_t offsets_capacity);
+/* SoA [attr][n] float32, storage order: u (3, position relative to the origin 4 b dx of
+ * the block b that wrote it: block j of the previous ordering for slot t in its range),
+ * then F (9, row-major) or q = J - 1 (fluid), then logJp (DP) */
+int ckmpm_export_state(ckmpm_engine* eng, int32_t model, float* soa);
+int ckmpm_import_state(ckmpm_engine* eng, int32_t model, const float* soa);
+/* grid mass/momentum of the previous layout, [block][m,px,py,pz][node 128] */
+int ckmpm_export_grid_m(ckmpm_engine* eng, float* grid, uint64_t capacity_blocks);
+/* grid velocity of the current layout, [block][vx,vy,vz][node 128] */
+int ckmpm_export_grid_v(ckmpm_engine* eng, float* grid, uint64_t capacity_blocks);
+
+/* Per-phase device time, measured with hipEvents on the engine's stream, like the
+ * reference's MPMBenchmark registry (mpm_benchmark.h:12-79; phases "G2P2G (Cuda)",
+ * "Activate Neighbor", ... at mpm_engine.cuh:485-492).  ms[0] grid update, ms[1]
+ * G2P2G (all models), ms[2] rebin and block activation, summed over the substeps
+ * since the last call; the call waits for the stream and resets the sums. */
+int ckmpm_set_profiling(ckmpm_engine* eng, int32_t enable);
+int ckmpm_get_phase_times(ckmpm_engine* eng, double ms[3], uint64_t* substeps);
+
+/* MPMEngine destructor (the reference frees almost nothing) */
+int ckmpm_destroy(ckmpm_engine* eng);
+const char* ckmpm_last_error(const ckmpm_engine* eng);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CKMPM_H */

@@ -1,0 +1,1068 @@
+// FP64 host restatement of the CK-MPM substep (binning, block activation,
+// fused G2P/constitutive/P2G, grid update with sticky boundary, rebin).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP engine is compared
+// against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// it.  The product library (libckmpm.so) never links or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to the
+// reference tree).  The reference leaves three things unspecified or
+// nondeterministic (atomic block insertion order, atomic particle order within a
+// block, float atomic summation order); this restatement fixes them with the
+// canonical rules that the HIP engine implements as well:
+//   * block keys are computed from FP32 positions with FP32 arithmetic: the engine keeps
+//     each position in FP32 relative to the origin 4 b dx of the block b that processed
+//     it, so a particle's next block is b + floor((floor(u/dx + 1/4) - 2) / 4) on its
+//     FP32 local coordinate u (identical to the reference's rule on global coordinates
+//     whenever the global FP32 value is exact, and about 100x finer otherwise);
+//   * a block list is "previous order with empty blocks removed, then newly
+//     occupied blocks in list order", and the neighbour / exterior extensions are
+//     appended in ascending linear key;
+//   * particle order inside a block is the stable order of the previous slot
+//     (a stable counting sort by new block index); an ordering word is
+//     slot | move << 27, move = (dx+1)*9 + (dy+1)*3 + (dz+1) of the block change.
+// Physics (weights, G2P, F update, stress, P2G, grid update) runs in FP64.
+//
+// Parity pinning: see the header of oracle/pyoracle.py and tests/test_oracle_*.py.
+// Every function below is checked there against closed forms, invariants,
+// brute force or numpy (SVD); none is left unpinned.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kMoveShift = 27;
+constexpr uint32_t kSlotMask = (1u << kMoveShift) - 1u;
+constexpr uint32_t kNoMove = 13u << kMoveShift;
+
+enum Kind { kFixedCorotated = 0, kNeoHookean = 1, kDruckerPrager = 2, kFluid = 3 };
+
+enum Flags : uint32_t {
+    kFlagOutOfWindow = 1u << 0,   // particle left its 8^3-node P2G window
+    kFlagOutOfDomain = 1u << 1,   // particle block outside the domain
+    kFlagNonFinite = 1u << 2,     // NaN grid velocity (mapped to +inf)
+    kFlagCapacity = 1u << 3,      // block list exceeds capacity
+    kFlagMissingBlock = 1u << 4,  // a stencil node's block not in the neighbour set
+};
+
+struct Domain {
+    int dim[3];
+    int off[3];
+    uint64_t nblocks;
+};
+
+inline uint32_t KeyOf(const Domain& d, int bx, int by, int bz) {
+    // mpm_partition.cuh:154-166 (Index: GetFlattenedIndex<Dy,Dz>(x,y,z))
+    return static_cast<uint32_t>((static_cast<int64_t>(bx) * d.dim[1] + by) * d.dim[2] + bz);
+}
+inline void CoordsOf(const Domain& d, uint32_t key, int c[3]) {
+    c[2] = static_cast<int>(key % static_cast<uint32_t>(d.dim[2]));
+    const uint32_t r = key / static_cast<uint32_t>(d.dim[2]);
+    c[1] = static_cast<int>(r % static_cast<uint32_t>(d.dim[1]));
+    c[0] = static_cast<int>(r / static_cast<uint32_t>(d.dim[1]));
+}
+inline bool InDomain(const Domain& d, int bx, int by, int bz) {
+    return bx >= 0 && by >= 0 && bz >= 0 && bx < d.dim[0] && by < d.dim[1] && bz < d.dim[2];
+}
+inline int FloorDiv4(int v) { return v >= 0 ? v / 4 : -((-v + 3) / 4); }
+
+// ---------------------------------------------------------------- FP32 binning
+// mpm_algorithm.cuh:59-67 (GetCellIndex<-1>): cell = int(x*invDx + 0.25f) in FP32.
+// Built with -ffp-contract=off so the multiply and the add round separately, exactly
+// as the two FP32 operations of the reference.
+inline int Cell0F(float x, float inv_dx) {
+    volatile float t = x * inv_dx;
+    float u = t + 0.25f;
+    return static_cast<int>(std::floor(u));
+}
+
+// Particle block of a position: (cell-2)/4 per axis, mpm_algorithm.cuh:595-600 and
+// mpm_particle.cuh:337-340 (floor division; identical to the reference's C division
+// for every cell >= 2, which the sticky boundary band guarantees).
+inline void BlockOfF(const float x[3], float inv_dx, int b[3], int c[3]) {
+    for (int a = 0; a < 3; ++a) {
+        c[a] = Cell0F(x[a], inv_dx);
+        b[a] = FloorDiv4(c[a] - 2);
+    }
+}
+
+// ------------------------------------------------------------------- 3x3 math
+using M3 = double[9];
+
+inline void MatMul(const double* A, const double* B, double* C) {
+    double T[9];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0;
+            for (int j = 0; j < 3; ++j) s += A[i * 3 + j] * B[j * 3 + k];
+            T[i * 3 + k] = s;
+        }
+    std::memcpy(C, T, sizeof(T));
+}
+inline void Transpose(const double* A, double* T) {
+    double R[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i * 3 + j] = A[j * 3 + i];
+    std::memcpy(T, R, sizeof(R));
+}
+// data_type.cuh:402-414
+inline double Det(const double* a) {
+    return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+           a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+// data_type.cuh:369-387 (adjugate / determinant)
+inline void Inverse(const double* a, double* r) {
+    const double id = 1.0 / Det(a);
+    double t[9];
+    t[0] = id * (a[4] * a[8] - a[5] * a[7]);
+    t[1] = id * (a[2] * a[7] - a[1] * a[8]);
+    t[2] = id * (a[1] * a[5] - a[2] * a[4]);
+    t[3] = id * (a[5] * a[6] - a[3] * a[8]);
+    t[4] = id * (a[0] * a[8] - a[2] * a[6]);
+    t[5] = id * (a[2] * a[3] - a[0] * a[5]);
+    t[6] = id * (a[3] * a[7] - a[4] * a[6]);
+    t[7] = id * (a[1] * a[6] - a[0] * a[7]);
+    t[8] = id * (a[0] * a[4] - a[1] * a[3]);
+    std::memcpy(r, t, sizeof(t));
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi, run to convergence.
+// Columns of Q are eigenvectors, lam the eigenvalues.
+void SymEig(const double* S, double* Q, double* lam) {
+    double A[9];
+    std::memcpy(A, S, sizeof(A));
+    for (int i = 0; i < 9; ++i) Q[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        const double off = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
+        const double diag = A[0] * A[0] + A[4] * A[4] + A[8] * A[8];
+        if (off <= 1e-34 * diag || off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double apq = A[p * 3 + q];
+                if (apq == 0.0) continue;
+                const double app = A[p * 3 + p], aqq = A[q * 3 + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                // A <- J^T A J with J the rotation in the (p,q) plane
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = A[k * 3 + p], akq = A[k * 3 + q];
+                    A[k * 3 + p] = c * akp - s * akq;
+                    A[k * 3 + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = A[p * 3 + k], aqk = A[q * 3 + k];
+                    A[p * 3 + k] = c * apk - s * aqk;
+                    A[q * 3 + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double qkp = Q[k * 3 + p], qkq = Q[k * 3 + q];
+                    Q[k * 3 + p] = c * qkp - s * qkq;
+                    Q[k * 3 + q] = s * qkp + c * qkq;
+                }
+            }
+    }
+    lam[0] = A[0];
+    lam[1] = A[4];
+    lam[2] = A[8];
+}
+
+// Signed SVD F = U diag(s) V^T with the sign and order convention of
+// mpm_math.cuh:37-1072 (McAdams et al. 2011): U, V rotations (det +1),
+// s0 >= s1 >= |s2|, the sign of det F carried by s2.  The FP64 algorithm itself
+// (Jacobi on F^T F to convergence, then orthonormalisation of F V) is this
+// oracle's own; only the convention follows the reference.
+void SignedSvd(const double* F, double* U, double* s, double* V) {
+    double FtF[9], Ft[9];
+    Transpose(F, Ft);
+    MatMul(Ft, F, FtF);
+    double lam[3];
+    SymEig(FtF, V, lam);
+    // sort eigenpairs by eigenvalue descending
+    int idx[3] = {0, 1, 2};
+    std::sort(idx, idx + 3, [&](int a, int b) { return lam[a] > lam[b]; });
+    double Vs[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Vs[r * 3 + c] = V[r * 3 + idx[c]];
+    std::memcpy(V, Vs, sizeof(Vs));
+    if (Det(V) < 0)
+        for (int r = 0; r < 3; ++r) V[r * 3 + 2] = -V[r * 3 + 2];
+    double B[9];
+    MatMul(F, V, B);
+    // U columns: normalise b0, orthonormalise b1, u2 = u0 x u1
+    double u0[3] = {B[0], B[3], B[6]}, u1[3] = {B[1], B[4], B[7]};
+    double n0 = std::sqrt(u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+    if (n0 < 1e-300) {
+        u0[0] = 1;
+        u0[1] = 0;
+        u0[2] = 0;
+        n0 = 0;
+    } else
+        for (double& v : u0) v /= n0;
+    double d = u0[0] * u1[0] + u0[1] * u1[1] + u0[2] * u1[2];
+    for (int a = 0; a < 3; ++a) u1[a] -= d * u0[a];
+    double n1 = std::sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    if (n1 < 1e-150 * (n0 + 1e-300) || n1 < 1e-300) {
+        // pick any unit vector orthogonal to u0
+        double e[3] = {0, 0, 0};
+        int m = 0;
+        for (int a = 1; a < 3; ++a)
+            if (std::fabs(u0[a]) < std::fabs(u0[m])) m = a;
+        e[m] = 1;
+        d = u0[m];
+        for (int a = 0; a < 3; ++a) u1[a] = e[a] - d * u0[a];
+        const double nn = std::sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+        for (double& v : u1) v /= nn;
+    } else {
+        for (double& v : u1) v /= n1;
+    }
+    // true singular value of column 1 is |B1| (columns of F V are orthogonal)
+    const double s1 = B[1] * u1[0] + B[4] * u1[1] + B[7] * u1[2];
+    double u2[3] = {u0[1] * u1[2] - u0[2] * u1[1], u0[2] * u1[0] - u0[0] * u1[2], u0[0] * u1[1] - u0[1] * u1[0]};
+    const double s2 = B[2] * u2[0] + B[5] * u2[1] + B[8] * u2[2];
+    for (int r = 0; r < 3; ++r) {
+        U[r * 3 + 0] = u0[r];
+        U[r * 3 + 1] = u1[r];
+        U[r * 3 + 2] = u2[r];
+    }
+    s[0] = n0;
+    s[1] = s1;
+    s[2] = s2;
+}
+
+struct Material {
+    int kind = kFixedCorotated;
+    double lambda = 0, mu = 0, bulk = 0, gamma = 0, viscosity = 0, J0 = 1, sin_phi = 0.5;
+};
+
+// FC: mpm_material.cuh:318-357.  tau = U diag(2 mu s(s-1) + lambda J(J-1)) U^T.
+void StressFixedCorotated(double lambda, double mu, const double* F, double* tau) {
+    double U[9], s[3], V[9];
+    SignedSvd(F, U, s, V);
+    const double J = s[0] * s[1] * s[2];
+    double sc[3];
+    for (int a = 0; a < 3; ++a) sc[a] = 2.0 * mu * s[a] * (s[a] - 1.0) + lambda * J * (J - 1.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0;
+            for (int k = 0; k < 3; ++k) v += U[i * 3 + k] * sc[k] * U[j * 3 + k];
+            tau[i * 3 + j] = v;
+        }
+}
+
+// Neo-Hookean (absent from the reference, north-star addition; closed form):
+// tau = mu (F F^T - I) + lambda ln(J) I.
+void StressNeoHookean(double lambda, double mu, const double* F, double* tau) {
+    double Ft[9], B[9];
+    Transpose(F, Ft);
+    MatMul(F, Ft, B);
+    const double lnJ = std::log(Det(F));
+    for (int i = 0; i < 9; ++i) tau[i] = mu * (B[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+    for (int i = 0; i < 3; ++i) tau[i * 4] += lambda * lnJ;
+}
+
+// Drucker-Prager StVK-Hencky return map: mpm_material.cuh:360-464 (cohesion 0,
+// beta 1, volume correction on; friction angle a parameter, 30 degrees in the
+// reference's hard-coded constant at :369).  Projects F in place, updates logJp.
+void StressDruckerPrager(double lambda, double mu, double sin_phi, double* F, double& logJp, double* tau) {
+    const double alpha = std::sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi);
+    const double smu = 2.0 * mu;
+    double U[9], s[3], V[9];
+    SignedSvd(F, U, s, V);
+    double eps[3];
+    for (int a = 0; a < 3; ++a) eps[a] = std::log(std::max(std::fabs(s[a]), 1e-4));
+    const double eps_sum = eps[0] + eps[1] + eps[2];
+    const double eps_tr = eps_sum + logJp;
+    double eh[3];
+    for (int a = 0; a < 3; ++a) eh[a] = eps[a] - eps_tr / 3.0;
+    const double ehn = std::sqrt(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+    double ns[3];
+    if (eps_tr >= 0.0) {
+        ns[0] = ns[1] = ns[2] = 1.0;
+        logJp = eps_sum + logJp;
+    } else {
+        double H[3] = {eps[0], eps[1], eps[2]};
+        if (mu != 0.0) {
+            logJp = 0.0;
+            const double dg = ehn + (3.0 * lambda + smu) / smu * eps_tr * alpha;
+            if (dg > 0.0)
+                for (int a = 0; a < 3; ++a) H[a] = eps[a] - (dg / ehn) * eh[a];
+        }
+        for (int a = 0; a < 3; ++a) ns[a] = std::exp(H[a]);
+    }
+    // F = U diag(ns) V^T
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0;
+            for (int k = 0; k < 3; ++k) v += U[i * 3 + k] * ns[k] * V[j * 3 + k];
+            F[i * 3 + j] = v;
+        }
+    // tau = U diag(phat) V^T F^T with phat = (2 mu log ns + lambda tr log ns) / ns
+    double lg[3] = {std::log(ns[0]), std::log(ns[1]), std::log(ns[2])};
+    const double ltr = lg[0] + lg[1] + lg[2];
+    double ph[3];
+    for (int a = 0; a < 3; ++a) ph[a] = (smu * lg[a] + lambda * ltr) / ns[a];
+    double P[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0;
+            for (int k = 0; k < 3; ++k) v += U[i * 3 + k] * ph[k] * V[j * 3 + k];
+            P[i * 3 + j] = v;
+        }
+    double Ft[9];
+    Transpose(F, Ft);
+    MatMul(P, Ft, tau);
+}
+
+// Weakly compressible fluid: mpm_algorithm.cuh:527-569 and mpm_material.cuh:466-477.
+// J <- J (1 + dt tr C); p = K (J^-gamma - 1); tau = J (nu (C + C^T) - p I).
+void StressFluid(const Material& m, const double* C, double J, double* tau) {
+    const double p = m.bulk * (std::pow(J, -m.gamma) - 1.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) tau[i * 3 + j] = J * (m.viscosity * (C[i * 3 + j] + C[j * 3 + i]) - (i == j ? p : 0.0));
+}
+
+// Compact-kernel 1-D weight of node 0 at fractional offset t in [0,1):
+// mpm_kernel.cuh:91-114 (w0 = 1 - t + sin(2 pi t)/(2 pi); node 1 gets 1 - w0).
+inline double W0(double t) { return 1.0 - t + std::sin(2.0 * kPi * t) / (2.0 * kPi); }
+
+struct Layout {
+    std::vector<uint32_t> keys;
+    uint32_t A = 0, Nb = 0, Ex = 0;
+    std::vector<uint32_t> table;  // dense key -> list index (kEmpty if absent)
+};
+
+struct Model {
+    Material mat;
+    double mass = 0, volume = 0;
+    uint32_t n = 0;
+    int nattr = 0;                     // persisted attributes (x,y,z + F / J / logJp)
+    std::vector<double> st;            // SoA [attr][n] in storage (slot) order
+    std::vector<double> st_next;       // written by the physics step in processing order
+    std::vector<float> vel0;           // initial velocities (3 per particle, or 3 total)
+    uint32_t nvel = 0;
+    std::vector<float> x0;             // initial FP32 positions (3n, AoS)
+    std::vector<uint32_t> perm;        // processing position t -> slot | move << 27
+    std::vector<uint32_t> setup_order; // processing position at setup -> input index
+    std::vector<uint32_t> off;         // A+1 block offsets into processing order
+    int pp_attr = 0;                   // state attribute of per-particle (lambda, mu); 0: none
+};
+
+inline int AttrCount(int kind) {
+    switch (kind) {
+        case kFixedCorotated:
+        case kNeoHookean: return 12;
+        case kDruckerPrager: return 13;
+        default: return 4;
+    }
+}
+
+struct Engine {
+    Domain dom{};
+    float dx_f = 0, inv_dx_f = 0;
+    double dx = 0, inv_dx = 0;
+    double gravity = 0, mass_clamp = 0;
+    uint32_t capacity = 0;
+    std::vector<Model> models;
+    Layout cur, prev;             // prev: layout in which M is indexed
+    std::vector<double> M;        // prev.Nb * 512  [blk][ch 4][node 128]
+    std::vector<double> Mnext;    // cur.Nb * 512, produced by the physics step
+    std::vector<double> V;        // cur.Nb * 384  [blk][ch 3][node 128]
+    uint32_t flags = 0;
+    double max_v2 = 0;
+    bool physics_pending = false;
+};
+
+// ------------------------------------------------------------------ layouts
+// Neighbour (b + {0,1}^3) and exterior (b + {-1,0,1}^3) activation of the first A
+// keys, domain-clipped: mpm_algorithm.cuh:657-695; appended in ascending key.
+void BuildLayout(Engine& e, Layout& L, const std::vector<uint32_t>& akeys) {
+    const Domain& d = e.dom;
+    if (L.table.size() != d.nblocks) L.table.assign(d.nblocks, kEmpty);
+    for (uint32_t k : L.keys) L.table[k] = kEmpty;
+    L.keys = akeys;
+    L.A = static_cast<uint32_t>(akeys.size());
+    for (uint32_t i = 0; i < L.A; ++i) L.table[akeys[i]] = i;
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<uint32_t> cand;
+        for (uint32_t i = 0; i < L.A; ++i) {
+            int c[3];
+            CoordsOf(d, akeys[i], c);
+            const int lo = pass == 0 ? 0 : -1;
+            for (int ox = lo; ox <= 1; ++ox)
+                for (int oy = lo; oy <= 1; ++oy)
+                    for (int oz = lo; oz <= 1; ++oz) {
+                        const int bx = c[0] + ox, by = c[1] + oy, bz = c[2] + oz;
+                        if (!InDomain(d, bx, by, bz)) continue;
+                        const uint32_t k = KeyOf(d, bx, by, bz);
+                        if (L.table[k] == kEmpty) cand.push_back(k);
+                    }
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        for (uint32_t k : cand) {
+            L.table[k] = static_cast<uint32_t>(L.keys.size());
+            L.keys.push_back(k);
+        }
+        if (pass == 0) L.Nb = static_cast<uint32_t>(L.keys.size());
+    }
+    L.Ex = static_cast<uint32_t>(L.keys.size());
+    if (L.Ex > e.capacity) e.flags |= kFlagCapacity;
+}
+
+inline uint32_t NodeBlock(const Engine& e, const Layout& L, int cx, int cy, int cz, int& local) {
+    const int bx = FloorDiv4(cx), by = FloorDiv4(cy), bz = FloorDiv4(cz);
+    local = (cx - 4 * bx) * 16 + (cy - 4 * by) * 4 + (cz - 4 * bz);
+    if (!InDomain(e.dom, bx, by, bz)) return kEmpty;
+    const uint32_t idx = L.table[KeyOf(e.dom, bx, by, bz)];
+    if (idx == kEmpty || idx >= L.Nb) return kEmpty;
+    return idx;
+}
+
+// Stencil of one sub-grid at a position: base node c, fractional offsets t, 1-D
+// weights w[axis][d].  Grid g = 0 sits at offset -dx/4, g = 1 at +dx/4:
+// mpm_algorithm.cuh:1555-1606 (cell = int(x/dx - g*0.25), t = x/dx - cell - 0.25 g).
+struct Stencil {
+    int c[3];
+    double t[3];
+    double w[3][2];
+};
+inline void MakeStencil(const Engine& e, const double* x, int g, Stencil& s) {
+    const double sg = g == 0 ? -1.0 : 1.0;
+    for (int a = 0; a < 3; ++a) {
+        const double u = x[a] * e.inv_dx - 0.25 * sg;
+        s.c[a] = static_cast<int>(std::floor(u));
+        s.t[a] = u - s.c[a];
+        s.w[a][0] = W0(s.t[a]);
+        s.w[a][1] = 1.0 - s.w[a][0];
+    }
+}
+
+// ---------------------------------------------------------------- rasterize
+// mpm_algorithm.cuh:871-947: initial P2G of m and m v0 on both grids (no APIC, no
+// stress), into grid blocks indexed by the setup layout.
+void Rasterize(Engine& e) {
+    const Layout& L = e.cur;
+    e.M.assign(static_cast<size_t>(L.Nb) * 512, 0.0);
+    for (const Model& m : e.models) {
+        for (uint32_t p = 0; p < m.n; ++p) {
+            double x[3] = {m.x0[3 * p], m.x0[3 * p + 1], m.x0[3 * p + 2]};
+            const float* v = m.nvel == 1 ? &m.vel0[0] : &m.vel0[3 * p];
+            for (int g = 0; g < 2; ++g) {
+                Stencil s;
+                MakeStencil(e, x, g, s);
+                for (int o = 0; o < 8; ++o) {
+                    const int dx_ = (o >> 2) & 1, dy_ = (o >> 1) & 1, dz_ = o & 1;
+                    const double w = s.w[0][dx_] * s.w[1][dy_] * s.w[2][dz_];
+                    int local;
+                    const uint32_t bi = NodeBlock(e, L, s.c[0] + dx_, s.c[1] + dy_, s.c[2] + dz_, local);
+                    if (bi == kEmpty) {
+                        e.flags |= kFlagMissingBlock;
+                        continue;
+                    }
+                    double* blk = &e.M[static_cast<size_t>(bi) * 512];
+                    const int n = g * 64 + local;
+                    const double wm = w * m.mass;
+                    blk[n] += wm;
+                    blk[128 + n] += wm * v[0];
+                    blk[256 + n] += wm * v[1];
+                    blk[384 + n] += wm * v[2];
+                }
+            }
+        }
+    }
+    e.prev = e.cur;
+}
+
+// ---------------------------------------------------------------- grid update
+// mpm_algorithm.cuh:738-869 (UpdateGridVelocityAndQueryMax<true>): v = p/m when
+// m > massClamp, gravity on y, whole block zeroed within 2 blocks of the domain
+// edge, NaN -> inf, max |v|^2.  Restated as a read of the previous layout's M
+// through its key table (this also performs CopySelectedGridBlocks,
+// mpm_algorithm.cuh:1920-1945, which only moves blocks between layouts).
+void GridUpdate(Engine& e, double dt) {
+    const Layout& L = e.cur;
+    e.V.assign(static_cast<size_t>(L.Nb) * 384, 0.0);
+    double vmax = 0.0;
+    for (uint32_t i = 0; i < L.Nb; ++i) {
+        const uint32_t key = L.keys[i];
+        int b[3];
+        CoordsOf(e.dom, key, b);
+        bool oob = false;
+        for (int a = 0; a < 3; ++a) oob |= (b[a] + e.dom.off[a] < 2) || (b[a] + e.dom.off[a] + 2 >= e.dom.dim[a]);
+        const uint32_t io = e.prev.table.empty() ? kEmpty : e.prev.table[key];
+        const double* src = (io != kEmpty && io < e.prev.Nb) ? &e.M[static_cast<size_t>(io) * 512] : nullptr;
+        double* dst = &e.V[static_cast<size_t>(i) * 384];
+        for (int n = 0; n < 128; ++n) {
+            double v2 = 0.0;
+            const double mass = src ? src[n] : 0.0;
+            double v[3] = {0, 0, 0};
+            if (mass > e.mass_clamp) {
+                for (int a = 0; a < 3; ++a) v[a] = src[128 * (a + 1) + n] / mass;
+                v[1] += e.gravity * dt;
+                if (oob) v[0] = v[1] = v[2] = 0.0;
+                v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            }
+            for (int a = 0; a < 3; ++a) dst[128 * a + n] = v[a];
+            if (std::isnan(v2)) {
+                v2 = INFINITY;
+                e.flags |= kFlagNonFinite;
+            }
+            vmax = std::max(vmax, v2);
+        }
+    }
+    e.max_v2 = vmax;
+}
+
+inline void LoadState(const Model& m, uint32_t s, double* x, double* F, double& J, double& logJp) {
+    const size_t n = m.n;
+    for (int a = 0; a < 3; ++a) x[a] = m.st[a * n + s];
+    if (m.mat.kind == kFluid) {
+        J = m.st[3 * n + s];
+    } else {
+        for (int k = 0; k < 9; ++k) F[k] = m.st[(3 + k) * n + s];
+        if (m.mat.kind == kDruckerPrager) logJp = m.st[12 * n + s];
+    }
+}
+inline void StoreState(Model& m, uint32_t t, const double* x, const double* F, double J, double logJp) {
+    const size_t n = m.n;
+    for (int a = 0; a < 3; ++a) m.st_next[a * n + t] = x[a];
+    if (m.mat.kind == kFluid) {
+        m.st_next[3 * n + t] = J;
+    } else {
+        for (int k = 0; k < 9; ++k) m.st_next[(3 + k) * n + t] = F[k];
+        if (m.mat.kind == kDruckerPrager) m.st_next[12 * n + t] = logJp;
+    }
+}
+
+// ---------------------------------------------------------------- G2P2G
+// mpm_algorithm.cuh:1365-1845 per particle block, in list order, particles in
+// processing order (perm), FP64 throughout:
+//   G2P (:1555-1606): v = 0.5 sum w v_i, B = sum v_i (w (x_i - x_p))^T,
+//                     D = sum (x_i - x_p)(w (x_i - x_p))^T over both grids;
+//   C = B D^-1, x += dt v (:1636-1638);
+//   F / J / logJp update and Kirchhoff stress tau (UpdateForce :192-327, :527-569);
+//   window check on the new cell (:1669-1683) -> flag and skip P2G;
+//   D' at the new x, D^ = (0.5 D')^-1, K = m C - nextDt V D^ tau (:1687-1736);
+//   P2G m w, w (m v + K (x_i - x_p)) on both grids (:1739-1792).
+void Physics(Engine& e, double dt, double next_dt) {
+    const Layout& L = e.cur;
+    e.Mnext.assign(static_cast<size_t>(L.Nb) * 512, 0.0);
+    for (Model& m : e.models) {
+        m.st_next.assign(m.st.size(), 0.0);
+        for (uint32_t j = 0; j < L.A; ++j) {
+            int bb[3];
+            CoordsOf(e.dom, L.keys[j], bb);
+            for (uint32_t t = m.off[j]; t < m.off[j + 1]; ++t) {
+                const uint32_t s = m.perm[t] & kSlotMask;
+                double x[3], F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, J = 1.0, logJp = 0.0;
+                LoadState(m, s, x, F, J, logJp);
+                double v[3] = {0, 0, 0}, B[9] = {0}, D[9] = {0};
+                for (int g = 0; g < 2; ++g) {
+                    Stencil st;
+                    MakeStencil(e, x, g, st);
+                    for (int o = 0; o < 8; ++o) {
+                        const int d3[3] = {(o >> 2) & 1, (o >> 1) & 1, o & 1};
+                        const double w = st.w[0][d3[0]] * st.w[1][d3[1]] * st.w[2][d3[2]];
+                        int local;
+                        const uint32_t bi = NodeBlock(e, L, st.c[0] + d3[0], st.c[1] + d3[1], st.c[2] + d3[2], local);
+                        double vi[3] = {0, 0, 0};
+                        if (bi == kEmpty)
+                            e.flags |= kFlagMissingBlock;
+                        else
+                            for (int a = 0; a < 3; ++a) vi[a] = e.V[static_cast<size_t>(bi) * 384 + 128 * a + g * 64 + local];
+                        double xx[3];
+                        for (int a = 0; a < 3; ++a) xx[a] = (d3[a] - st.t[a]) * e.dx;
+                        for (int a = 0; a < 3; ++a) {
+                            v[a] += w * vi[a];
+                            for (int c = 0; c < 3; ++c) {
+                                B[a * 3 + c] += vi[a] * w * xx[c];
+                                D[a * 3 + c] += xx[a] * w * xx[c];
+                            }
+                        }
+                    }
+                }
+                double Dinv[9], C[9];
+                Inverse(D, Dinv);
+                MatMul(B, Dinv, C);
+                for (int a = 0; a < 3; ++a) {
+                    v[a] *= 0.5;
+                    x[a] += dt * v[a];
+                }
+                const size_t Nn = m.n;
+                const double lam = m.pp_attr ? m.st[m.pp_attr * Nn + s] : m.mat.lambda;
+                const double mu = m.pp_attr ? m.st[(m.pp_attr + 1) * Nn + s] : m.mat.mu;
+                if (m.pp_attr) {
+                    m.st_next[m.pp_attr * Nn + t] = lam;
+                    m.st_next[(m.pp_attr + 1) * Nn + t] = mu;
+                }
+                double tau[9];
+                if (m.mat.kind == kFluid) {
+                    J += (C[0] + C[4] + C[8]) * dt * J;
+                    StressFluid(m.mat, C, J, tau);
+                } else {
+                    double dW[9];
+                    for (int k = 0; k < 9; ++k) dW[k] = C[k] * dt + ((k % 4 == 0) ? 1.0 : 0.0);
+                    MatMul(dW, F, F);
+                    if (m.mat.kind == kFixedCorotated)
+                        StressFixedCorotated(lam, mu, F, tau);
+                    else if (m.mat.kind == kNeoHookean)
+                        StressNeoHookean(lam, mu, F, tau);
+                    else
+                        StressDruckerPrager(lam, mu, m.mat.sin_phi, F, logJp, tau);
+                }
+                StoreState(m, t, x, F, J, logJp);
+
+                // window check on the new grid-0 cell, relative to this block's origin
+                bool out = false;
+                for (int a = 0; a < 3; ++a) {
+                    const int c0 = static_cast<int>(std::floor(x[a] * e.inv_dx + 0.25));
+                    const int loc = c0 - 4 * bb[a];
+                    out |= (loc <= 0) || (loc >= 7);
+                }
+                if (out) {
+                    e.flags |= kFlagOutOfWindow;
+                    continue;
+                }
+                double Dn[9] = {0};
+                Stencil sn[2];
+                for (int g = 0; g < 2; ++g) {
+                    MakeStencil(e, x, g, sn[g]);
+                    for (int o = 0; o < 8; ++o) {
+                        const int d3[3] = {(o >> 2) & 1, (o >> 1) & 1, o & 1};
+                        const double w = sn[g].w[0][d3[0]] * sn[g].w[1][d3[1]] * sn[g].w[2][d3[2]];
+                        double xx[3];
+                        for (int a = 0; a < 3; ++a) xx[a] = (d3[a] - sn[g].t[a]) * e.dx;
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c) Dn[a * 3 + c] += xx[a] * w * xx[c];
+                    }
+                }
+                for (double& q : Dn) q *= 0.5;
+                double Dh[9], DT[9], K[9];
+                Inverse(Dn, Dh);
+                MatMul(Dh, tau, DT);
+                for (int k = 0; k < 9; ++k) K[k] = C[k] * m.mass - (next_dt * m.volume) * DT[k];
+                for (int g = 0; g < 2; ++g) {
+                    const Stencil& st = sn[g];
+                    for (int o = 0; o < 8; ++o) {
+                        const int d3[3] = {(o >> 2) & 1, (o >> 1) & 1, o & 1};
+                        const double w = st.w[0][d3[0]] * st.w[1][d3[1]] * st.w[2][d3[2]];
+                        int local;
+                        const uint32_t bi = NodeBlock(e, L, st.c[0] + d3[0], st.c[1] + d3[1], st.c[2] + d3[2], local);
+                        if (bi == kEmpty) {
+                            e.flags |= kFlagMissingBlock;
+                            continue;
+                        }
+                        double xx[3];
+                        for (int a = 0; a < 3; ++a) xx[a] = (d3[a] - st.t[a]) * e.dx;
+                        double* blk = &e.Mnext[static_cast<size_t>(bi) * 512];
+                        const int n = g * 64 + local;
+                        const double wm = w * m.mass;
+                        blk[n] += wm;
+                        for (int a = 0; a < 3; ++a) {
+                            const double km = K[a * 3 + 0] * xx[0] + K[a * 3 + 1] * xx[1] + K[a * 3 + 2] * xx[2];
+                            blk[128 * (a + 1) + n] += wm * v[a] + w * km;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    e.physics_pending = true;
+}
+
+// ---------------------------------------------------------------- rebin
+// Replaces AddAdvection / ParticleCellBucketToBlock / MarkActive* / ExclusiveScan /
+// UpdatePartition / UpdateBucket / ComputeBinCount (mpm_particle.cuh:331-371,
+// mpm_algorithm.cuh:572-649,1847-1918, mpm_engine.cuh:659-866) by the canonical
+// order: each particle's new block is its FP32 grid-0 block, clamped to the 27
+// neighbours of its current block and to the domain; the new particle-block list is
+// the current exterior list with unoccupied blocks removed; particles are stably
+// sorted by new block index.  xf[m] holds each model's FP32 positions relative to the
+// origin of the block that processed them, in processing order (3 per particle, AoS).
+void Rebin(Engine& e, const std::vector<const float*>& xf) {
+    const Layout& L = e.cur;
+    const Domain& d = e.dom;
+    std::vector<uint32_t> occ(L.Ex, 0);
+    std::vector<std::vector<uint32_t>> target(e.models.size()), move(e.models.size());
+    for (size_t mi = 0; mi < e.models.size(); ++mi) {
+        Model& m = e.models[mi];
+        target[mi].resize(m.n);
+        move[mi].resize(m.n);
+        for (uint32_t j = 0; j < L.A; ++j) {
+            int bb[3];
+            CoordsOf(d, L.keys[j], bb);
+            for (uint32_t t = m.off[j]; t < m.off[j + 1]; ++t) {
+                int b[3], c[3];
+                BlockOfF(&xf[mi][3 * static_cast<size_t>(t)], e.inv_dx_f, b, c);  // local block
+                int nb[3];
+                bool clamped = false;
+                for (int a = 0; a < 3; ++a) {
+                    int delta = b[a];
+                    if (delta < -1 || delta > 1) clamped = true;
+                    delta = std::max(-1, std::min(1, delta));
+                    nb[a] = bb[a] + delta;
+                }
+                if (clamped) e.flags |= kFlagOutOfWindow;
+                if (!InDomain(d, nb[0], nb[1], nb[2])) {
+                    e.flags |= kFlagOutOfDomain;
+                    nb[0] = bb[0];
+                    nb[1] = bb[1];
+                    nb[2] = bb[2];
+                }
+                const uint32_t ei = L.table[KeyOf(d, nb[0], nb[1], nb[2])];
+                target[mi][t] = ei;
+                move[mi][t] = static_cast<uint32_t>((nb[0] - bb[0] + 1) * 9 + (nb[1] - bb[1] + 1) * 3 + (nb[2] - bb[2] + 1));
+                occ[ei]++;
+            }
+        }
+    }
+    // new particle-block list: exterior list order, unoccupied removed
+    std::vector<uint32_t> newidx(L.Ex, kEmpty), akeys;
+    for (uint32_t i = 0; i < L.Ex; ++i)
+        if (occ[i]) {
+            newidx[i] = static_cast<uint32_t>(akeys.size());
+            akeys.push_back(L.keys[i]);
+        }
+    const uint32_t A = static_cast<uint32_t>(akeys.size());
+    for (size_t mi = 0; mi < e.models.size(); ++mi) {
+        Model& m = e.models[mi];
+        std::vector<uint32_t> cnt(A + 1, 0);
+        for (uint32_t t = 0; t < m.n; ++t) cnt[newidx[target[mi][t]]]++;
+        std::vector<uint32_t> off(A + 1, 0);
+        for (uint32_t j = 0; j < A; ++j) off[j + 1] = off[j] + cnt[j];
+        std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+        std::vector<uint32_t> perm(m.n);
+        for (uint32_t t = 0; t < m.n; ++t) perm[fill[newidx[target[mi][t]]]++] = t | (move[mi][t] << kMoveShift);
+        m.perm.swap(perm);
+        m.off.swap(off);
+    }
+    Layout next;
+    next.table.swap(e.prev.table);  // reuse the oldest table's storage
+    next.keys.swap(e.prev.keys);
+    BuildLayout(e, next, akeys);
+    e.prev = std::move(e.cur);
+    e.cur = std::move(next);
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+struct orc_engine;
+
+orc_engine* orc_create(const int dim[3], const int offset[3], float dx, double gravity, double mass_clamp,
+                       uint32_t capacity) {
+    Engine* e = new Engine();
+    for (int a = 0; a < 3; ++a) {
+        e->dom.dim[a] = dim[a];
+        e->dom.off[a] = offset ? offset[a] : 0;
+    }
+    e->dom.nblocks = static_cast<uint64_t>(dim[0]) * dim[1] * dim[2];
+    e->dx_f = dx;
+    e->inv_dx_f = static_cast<float>(1.0 / static_cast<double>(dx));  // mpm_config.h:22-25
+    e->dx = dx;
+    e->inv_dx = 1.0 / static_cast<double>(dx);
+    e->gravity = gravity;
+    e->mass_clamp = mass_clamp;
+    e->capacity = capacity;
+    return reinterpret_cast<orc_engine*>(e);
+}
+
+void orc_destroy(orc_engine* h) { delete reinterpret_cast<Engine*>(h); }
+
+// params: {lambda, mu, bulk, gamma, viscosity, J0, friction_angle_deg}
+int orc_add_model(orc_engine* h, int kind, const double* params, double mass, double volume, const float* xyz,
+                  uint32_t n, const float* vel, uint32_t nvel, const float* lam_p, const float* mu_p) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    Model m;
+    m.mat.kind = kind;
+    m.mat.lambda = params[0];
+    m.mat.mu = params[1];
+    m.mat.bulk = params[2];
+    m.mat.gamma = params[3];
+    m.mat.viscosity = params[4];
+    m.mat.J0 = params[5];
+    m.mat.sin_phi = std::sin(params[6] * kPi / 180.0);
+    m.mass = mass;
+    m.volume = volume;
+    m.n = n;
+    m.nattr = AttrCount(kind);
+    m.x0.assign(xyz, xyz + 3 * static_cast<size_t>(n));
+    m.nvel = nvel;
+    m.vel0.assign(vel, vel + 3 * static_cast<size_t>(nvel));
+    if (lam_p && mu_p) {
+        m.pp_attr = m.nattr;
+        m.nattr += 2;
+    }
+    // initial state in input order: x, F = I (FC/NH/DP), logJp = 0 (DP), J = J0 (fluid)
+    // (CopyParticleArrayToParticleBuffer, mpm_algorithm.cuh:973-1081)
+    m.st.assign(static_cast<size_t>(m.nattr) * n, 0.0);
+    for (uint32_t p = 0; p < n; ++p)
+        for (int a = 0; a < 3; ++a) m.st[a * static_cast<size_t>(n) + p] = xyz[3 * static_cast<size_t>(p) + a];
+    if (m.pp_attr)
+        for (uint32_t p = 0; p < n; ++p) {
+            m.st[m.pp_attr * static_cast<size_t>(n) + p] = lam_p[p];
+            m.st[(m.pp_attr + 1) * static_cast<size_t>(n) + p] = mu_p[p];
+        }
+    if (kind == kFluid) {
+        for (uint32_t p = 0; p < n; ++p) m.st[3 * static_cast<size_t>(n) + p] = m.mat.J0;
+    } else {
+        for (int k = 0; k < 9; k += 4)
+            for (uint32_t p = 0; p < n; ++p) m.st[(3 + k) * static_cast<size_t>(n) + p] = 1.0;
+    }
+    e->models.push_back(std::move(m));
+    return static_cast<int>(e->models.size()) - 1;
+}
+
+// InitialSetup (mpm_engine.cuh:215-383): particle blocks from FP32 positions,
+// ascending key; particles stably sorted by block; neighbour / exterior layout;
+// rasterize.  Returns the flags.
+uint32_t orc_initial_setup(orc_engine* h) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    std::vector<uint32_t> keys;
+    std::vector<std::vector<uint32_t>> pkey(e->models.size());
+    for (size_t mi = 0; mi < e->models.size(); ++mi) {
+        Model& m = e->models[mi];
+        pkey[mi].resize(m.n);
+        for (uint32_t p = 0; p < m.n; ++p) {
+            int b[3], c[3];
+            BlockOfF(&m.x0[3 * static_cast<size_t>(p)], e->inv_dx_f, b, c);
+            if (!InDomain(e->dom, b[0], b[1], b[2])) {
+                e->flags |= kFlagOutOfDomain;
+                for (int a = 0; a < 3; ++a) b[a] = std::max(0, std::min(e->dom.dim[a] - 1, b[a]));
+            }
+            pkey[mi][p] = KeyOf(e->dom, b[0], b[1], b[2]);
+            keys.push_back(pkey[mi][p]);
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    BuildLayout(*e, e->cur, keys);
+    const uint32_t A = e->cur.A;
+    for (size_t mi = 0; mi < e->models.size(); ++mi) {
+        Model& m = e->models[mi];
+        std::vector<uint32_t> cnt(A, 0);
+        for (uint32_t p = 0; p < m.n; ++p) cnt[e->cur.table[pkey[mi][p]]]++;
+        m.off.assign(A + 1, 0);
+        for (uint32_t j = 0; j < A; ++j) m.off[j + 1] = m.off[j] + cnt[j];
+        std::vector<uint32_t> fill(m.off.begin(), m.off.end() - 1);
+        m.setup_order.resize(m.n);
+        for (uint32_t p = 0; p < m.n; ++p) m.setup_order[fill[e->cur.table[pkey[mi][p]]]++] = p;
+        // state in processing order; the ordering becomes the identity with no move
+        // (CopyParticleArrayToParticleBuffer + InitializeAdvectionBucket,
+        // mpm_algorithm.cuh:949-1081)
+        std::vector<double> st(m.st.size());
+        for (int a = 0; a < m.nattr; ++a)
+            for (uint32_t t = 0; t < m.n; ++t) st[a * static_cast<size_t>(m.n) + t] = m.st[a * static_cast<size_t>(m.n) + m.setup_order[t]];
+        m.st.swap(st);
+        m.perm.resize(m.n);
+        for (uint32_t t = 0; t < m.n; ++t) m.perm[t] = t | kNoMove;
+    }
+    Rasterize(*e);
+    return e->flags;
+}
+
+// Grid update with dt, then G2P2G(dt, next_dt); the rebin is separate.
+uint32_t orc_physics(orc_engine* h, double dt, double next_dt) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    GridUpdate(*e, dt);
+    Physics(*e, dt, next_dt);
+    return e->flags;
+}
+
+// Rebin using the given FP32 positions (per model, AoS, processing order), or the
+// oracle's own FP64 positions rounded to FP32 when xyz_per_model is NULL.  Commits
+// the physics step's state and grid.
+uint32_t orc_rebin(orc_engine* h, const float* const* xyz_per_model) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    std::vector<std::vector<float>> own;
+    own.reserve(e->models.size());
+    std::vector<const float*> ptr(e->models.size());
+    for (size_t mi = 0; mi < e->models.size(); ++mi) {
+        Model& m = e->models[mi];
+        if (xyz_per_model && xyz_per_model[mi]) {
+            ptr[mi] = xyz_per_model[mi];
+        } else {
+            // FP32 position relative to the origin of the processing block
+            own.emplace_back(3 * static_cast<size_t>(m.n));
+            const Layout& L = e->cur;
+            for (uint32_t j = 0; j < L.A; ++j) {
+                int bb[3];
+                CoordsOf(e->dom, L.keys[j], bb);
+                for (uint32_t t = m.off[j]; t < m.off[j + 1]; ++t)
+                    for (int a = 0; a < 3; ++a)
+                        own.back()[3 * static_cast<size_t>(t) + a] =
+                            static_cast<float>(m.st_next[a * static_cast<size_t>(m.n) + t] - 4.0 * bb[a] * e->dx);
+            }
+            ptr[mi] = own.back().data();
+        }
+    }
+    if (e->physics_pending) {
+        for (Model& m : e->models) m.st.swap(m.st_next);
+        e->M.swap(e->Mnext);
+        e->physics_pending = false;
+    }
+    Rebin(*e, ptr);
+    return e->flags;
+}
+
+uint32_t orc_substep(orc_engine* h, double dt, double next_dt) {
+    orc_physics(h, dt, next_dt);
+    return orc_rebin(h, nullptr);
+}
+
+uint32_t orc_flags(orc_engine* h) { return reinterpret_cast<Engine*>(h)->flags; }
+void orc_clear_flags(orc_engine* h) { reinterpret_cast<Engine*>(h)->flags = 0; }
+double orc_max_v2(orc_engine* h) { return reinterpret_cast<Engine*>(h)->max_v2; }
+
+// which: 0 = current layout, 1 = previous layout (the one M is indexed by)
+void orc_layout_counts(orc_engine* h, int which, uint32_t out[3]) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    const Layout& L = which ? e->prev : e->cur;
+    out[0] = L.A;
+    out[1] = L.Nb;
+    out[2] = L.Ex;
+}
+void orc_layout_keys(orc_engine* h, int which, uint32_t* keys) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    const Layout& L = which ? e->prev : e->cur;
+    std::memcpy(keys, L.keys.data(), L.keys.size() * sizeof(uint32_t));
+}
+uint32_t orc_model_size(orc_engine* h, int mi) { return reinterpret_cast<Engine*>(h)->models[mi].n; }
+int orc_model_attrs(orc_engine* h, int mi) { return reinterpret_cast<Engine*>(h)->models[mi].nattr; }
+void orc_get_perm(orc_engine* h, int mi, uint32_t* out) {
+    const Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    std::memcpy(out, m.perm.data(), m.perm.size() * sizeof(uint32_t));
+}
+void orc_get_offsets(orc_engine* h, int mi, uint32_t* out) {
+    const Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    std::memcpy(out, m.off.data(), m.off.size() * sizeof(uint32_t));
+}
+// committed state (SoA [attr][n], storage order)
+void orc_get_state(orc_engine* h, int mi, double* out) {
+    const Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    std::memcpy(out, m.st.data(), m.st.size() * sizeof(double));
+}
+// state produced by the last physics step (processing order), before the rebin
+void orc_get_state_next(orc_engine* h, int mi, double* out) {
+    const Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    std::memcpy(out, m.st_next.data(), m.st_next.size() * sizeof(double));
+}
+// M: previous-layout grid (after a rebin) ; Mnext: grid produced by the physics step
+void orc_get_grid_m(orc_engine* h, double* out) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    std::memcpy(out, e->M.data(), e->M.size() * sizeof(double));
+}
+void orc_get_grid_mnext(orc_engine* h, double* out) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    std::memcpy(out, e->Mnext.data(), e->Mnext.size() * sizeof(double));
+}
+void orc_get_grid_v(orc_engine* h, double* out) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    std::memcpy(out, e->V.data(), e->V.size() * sizeof(double));
+}
+
+// ---- lockstep import: replace the engine's layouts, orderings, state and grid
+// with externally supplied ones (e.g. the GPU engine's), so that one physics step
+// and one rebin can be compared on identical inputs.
+void orc_set_layout(orc_engine* h, int which, const uint32_t* keys, uint32_t A, uint32_t Nb, uint32_t Ex) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    Layout& L = which ? e->prev : e->cur;
+    if (L.table.size() != e->dom.nblocks) L.table.assign(e->dom.nblocks, kEmpty);
+    for (uint32_t k : L.keys) L.table[k] = kEmpty;
+    L.keys.assign(keys, keys + Ex);
+    L.A = A;
+    L.Nb = Nb;
+    L.Ex = Ex;
+    for (uint32_t i = 0; i < Ex; ++i) L.table[keys[i]] = i;
+}
+void orc_set_ordering(orc_engine* h, int mi, const uint32_t* perm, const uint32_t* off) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    Model& m = e->models[mi];
+    m.perm.assign(perm, perm + m.n);
+    m.off.assign(off, off + e->cur.A + 1);
+}
+void orc_set_state_f32(orc_engine* h, int mi, const float* soa) {
+    Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    for (size_t i = 0; i < m.st.size(); ++i) m.st[i] = soa[i];
+}
+// global positions (FP64) and the remaining attributes, SoA, storage order
+void orc_set_state_f64(orc_engine* h, int mi, const double* soa) {
+    Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    std::memcpy(m.st.data(), soa, m.st.size() * sizeof(double));
+}
+void orc_get_setup_order(orc_engine* h, int mi, uint32_t* out) {
+    const Model& m = reinterpret_cast<Engine*>(h)->models[mi];
+    std::memcpy(out, m.setup_order.data(), m.setup_order.size() * sizeof(uint32_t));
+}
+void orc_set_grid_m_f32(orc_engine* h, const float* M) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    e->M.resize(static_cast<size_t>(e->prev.Nb) * 512);
+    for (size_t i = 0; i < e->M.size(); ++i) e->M[i] = M[i];
+}
+
+// ---- unit-level entry points used to pin the restatement in tests
+void orc_svd(const double* F, double* U, double* s, double* V) { SignedSvd(F, U, s, V); }
+void orc_stress(int kind, const double* params, const double* F_in, double logJp_in, const double* C, double J,
+                double* F_out, double* logJp_out, double* tau) {
+    Material m;
+    m.kind = kind;
+    m.lambda = params[0];
+    m.mu = params[1];
+    m.bulk = params[2];
+    m.gamma = params[3];
+    m.viscosity = params[4];
+    m.J0 = params[5];
+    m.sin_phi = std::sin(params[6] * kPi / 180.0);
+    double F[9];
+    std::memcpy(F, F_in, sizeof(F));
+    double lj = logJp_in;
+    if (kind == kFixedCorotated) StressFixedCorotated(m.lambda, m.mu, F, tau);
+    else if (kind == kNeoHookean) StressNeoHookean(m.lambda, m.mu, F, tau);
+    else if (kind == kDruckerPrager) StressDruckerPrager(m.lambda, m.mu, m.sin_phi, F, lj, tau);
+    else StressFluid(m, C, J, tau);
+    std::memcpy(F_out, F, sizeof(F));
+    *logJp_out = lj;
+}
+// 1-D weight, stencil of a position (grid g), binning of FP32 positions
+double orc_weight0(double t) { return W0(t); }
+void orc_stencil(orc_engine* h, const double* x, int g, int* c, double* t, double* w) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    Stencil s;
+    MakeStencil(*e, x, g, s);
+    for (int a = 0; a < 3; ++a) {
+        c[a] = s.c[a];
+        t[a] = s.t[a];
+        w[2 * a] = s.w[a][0];
+        w[2 * a + 1] = s.w[a][1];
+    }
+}
+void orc_bin_f32(orc_engine* h, const float* xyz, uint32_t n, int* block_xyz, int* cell_xyz) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    for (uint32_t p = 0; p < n; ++p) BlockOfF(&xyz[3 * static_cast<size_t>(p)], e->inv_dx_f, &block_xyz[3 * p], &cell_xyz[3 * p]);
+}
+// Neighbour / exterior extension of an ordered particle-block key list.
+uint32_t orc_build_layout(orc_engine* h, const uint32_t* akeys, uint32_t A, uint32_t* out_keys, uint32_t* out_counts) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    Layout L;
+    BuildLayout(*e, L, std::vector<uint32_t>(akeys, akeys + A));
+    std::memcpy(out_keys, L.keys.data(), L.keys.size() * sizeof(uint32_t));
+    out_counts[0] = L.A;
+    out_counts[1] = L.Nb;
+    out_counts[2] = L.Ex;
+    return L.Ex;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every entry point declared in include/ckmpm.h
+(no compute call: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from ckmpm_amd import native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "ckmpm.h")).read()
+    return sorted(set(re.findall(r"\b(ckmpm_[a-z_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(native.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(native.LIB_PATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_create_without_device_fails_cleanly():
+    """Without a gfx950 device ckmpm_create returns an error code, not a crash."""
+    import torch
+
+    if torch.cuda.is_available():
+        return
+    lib = native.load()
+    d = native.DomainDesc((ctypes.c_int32 * 3)(16, 16, 16), (ctypes.c_int32 * 3)(0, 0, 0))
+    s = native.SimDesc(1 / 64, 0, 0, 0.5, 1e-4, 48, 4096)
+    h = ctypes.c_void_p()
+    rc = lib.ckmpm_create(ctypes.byref(d), ctypes.byref(s), ctypes.byref(h))
+    assert rc in (native.ERR_NO_DEVICE, native.ERR_HIP)
+    assert not h

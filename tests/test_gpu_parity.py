@@ -1,0 +1,182 @@
+"""Parity of the HIP engine (through the C-ABI) with the FP64 oracle.
+
+Bars (written in each test):
+  * binning, block keys, block lists, particle order and offsets: bit-exact;
+  * grid mass / momentum: |M_gpu - M_oracle|_inf <= 1e-5 |M_oracle|_inf per channel;
+  * positions after 100 colliding-sphere substeps: <= 1e-4 dx.
+Lockstep tests import the GPU's state into the oracle before every substep, so each
+substep is compared on identical inputs and the oracle re-bins the GPU's FP32 positions.
+"""
+import numpy as np
+import pytest
+
+from ckmpm_amd import scenes
+from ckmpm_amd.engine import engine_from_scene
+from ckmpm_amd.native import SLOT_MASK
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GRID_TOL = 1e-5  # per substep, normwise
+
+
+def _grid_close(Mg, Mo, tol=GRID_TOL):
+    """Mass relative to max |m|; momentum relative to max |p| over its three components."""
+    Mg = Mg.astype(np.float64)
+    for name, sl in (("mass", slice(0, 1)), ("momentum", slice(1, 4))):
+        scale = np.abs(Mo[:, sl]).max()
+        err = np.abs(Mg[:, sl] - Mo[:, sl]).max()
+        print(f"  grid {name}: max err {err:.3e}, scale {scale:.3e}, rel {err / max(scale, 1e-300):.2e}")
+        assert err <= tol * scale, f"{name}: err {err:.3e} vs scale {scale:.3e}"
+
+
+def _compare_layout(eng, orc, which):
+    kg, cg = eng.layout(which)
+    ko = orc.keys(which)
+    assert cg == orc.counts(which), (cg, orc.counts(which))
+    np.testing.assert_array_equal(kg, ko)
+
+
+def _compare_ordering(eng, orc, nmodels):
+    for m in range(nmodels):
+        pg, og = eng.ordering(m)
+        np.testing.assert_array_equal(og, orc.offsets(m))
+        np.testing.assert_array_equal(pg, orc.perm(m))
+
+
+def _import(eng, orc, nmodels):
+    kc, (A, Nb, Ex) = eng.layout(0)
+    kp, (pA, pNb, pEx) = eng.layout(1)
+    orc.set_layout(0, kc, A, Nb, Ex)
+    orc.set_layout(1, kp, pA, pNb, pEx)
+    orc.set_grid_m(eng.grid_m())
+    for m in range(nmodels):
+        perm, off = eng.ordering(m)
+        orc.set_ordering(m, perm, off)
+        orc.set_state(m, eng.global_state(m))
+
+
+def _lockstep(scene, steps, dts, state_atol):
+    eng = engine_from_scene(scene)
+    orc = Oracle.from_scene(scene)
+    orc.setup()
+    nm = len(scene.models)
+    # setup: layout, ordering bit-exact; rasterized grid within tolerance
+    _compare_layout(eng, orc, 0)
+    _compare_ordering(eng, orc, nm)
+    _grid_close(eng.grid_m(), orc.grid_m())
+    for m in range(nm):  # setup state in processing order (the sort order)
+        np.testing.assert_allclose(eng.global_state(m), orc.state(m), rtol=0, atol=1e-7)
+    for k in range(steps):
+        dt, ndt = dts(k)
+        _import(eng, orc, nm)
+        orc.physics(dt, ndt)
+        eng.Substep(dt, ndt)
+        eng.Synchronize()
+        # physics on identical inputs
+        _grid_close(eng.grid_m(), orc.grid_mnext())
+        for m in range(nm):
+            sg, so = eng.global_state(m), orc.state_next(m)
+            np.testing.assert_allclose(sg[:3], so[:3], rtol=0, atol=state_atol[0])
+            if sg.shape[0] > 3:
+                np.testing.assert_allclose(sg[3:], so[3:], rtol=0, atol=state_atol[1])
+        # rebin of the GPU's FP32 positions: bit-exact
+        orc.rebin([eng.state(m)[:3].T.copy() for m in range(nm)])
+        _compare_layout(eng, orc, 0)
+        _compare_layout(eng, orc, 1)
+        _compare_ordering(eng, orc, nm)
+        assert orc.flags == 0
+    c = eng.counters()
+    assert c.flags == 0, hex(c.flags)
+    return eng, orc
+
+
+def test_colliding_spheres_lockstep():
+    sc = scenes.colliding_spheres()
+    _lockstep(sc, 8, lambda k: (0.0 if k == 0 else 1e-4, 1e-4), (2e-7, 1e-5))
+
+
+def _ids_after(eng_or_orc, ids, nm, is_gpu):
+    out = []
+    for m in range(nm):
+        perm = eng_or_orc.ordering(m)[0] if is_gpu else eng_or_orc.perm(m)
+        out.append(ids[m][perm & SLOT_MASK])
+    return out
+
+
+def test_colliding_spheres_trajectory_100_substeps():
+    """Independent FP32 GPU and FP64 oracle runs; positions within 1e-4 dx after 100 substeps."""
+    sc = scenes.colliding_spheres()
+    nm = len(sc.models)
+    eng = engine_from_scene(sc)
+    orc = Oracle.from_scene(sc)
+    orc.setup()
+    ids_o = [orc.setup_order(m).astype(np.int64) for m in range(nm)]
+    for m in range(nm):  # the GPU's setup order is the oracle's: same state in the same slots
+        np.testing.assert_allclose(eng.global_state(m)[:3], orc.state(m)[:3], rtol=0, atol=1e-7)
+    ids_g = [i.copy() for i in ids_o]
+    for k in range(100):
+        dt = 0.0 if k == 0 else sc.dt
+        ids_g = _ids_after(eng, ids_g, nm, True)
+        ids_o = _ids_after(orc, ids_o, nm, False)
+        eng.Substep(dt, sc.dt)
+        orc.substep(dt, sc.dt)
+    eng.Synchronize()
+    assert orc.flags == 0 and eng.counters().flags == 0
+    for m in range(nm):
+        xg = eng.global_state(m)[:3].T
+        xo = orc.state(m)[:3].T
+        pg = np.empty_like(xg)
+        po = np.empty_like(xo)
+        pg[ids_g[m]] = xg
+        po[ids_o[m]] = xo
+        err = np.abs(pg - po).max()
+        assert err <= 1e-4 * sc.dx, f"model {m}: max position error {err:.3e} > {1e-4 * sc.dx:.3e}"
+        # the spheres moved: 99 substeps at 1 m/s
+        disp = np.abs(po[:, 0] - sc.models[m].x[:, 0]).mean()
+        assert abs(disp - 99 * sc.dt) < 0.05 * 99 * sc.dt
+
+
+def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5):
+    """A small box of one material with random initial velocities (exercises C and stress)."""
+    dx = 1.0 / (4 * dim)
+    rng = scenes.Splitmix(seed)
+    lo, hi = int(0.3 * 4 * dim), int(0.6 * 4 * dim)
+    g = np.stack(np.meshgrid(*[np.arange(lo, hi)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = scenes._stratified_cells(rng, g.astype(np.float64), dx, ppc_side).astype(np.float32)
+    v = (speed * rng.normal(3 * x.shape[0]).reshape(-1, 3)).astype(np.float32)
+    vol = dx ** 3 / ppc_side ** 3
+    if kind == scenes.FLUID:
+        mat = scenes.Material(kind, bulk=1e4, gamma=7.0, viscosity=0.1)
+    else:
+        lam, mu = scenes.lame(2e5, 0.3)
+        mat = scenes.Material(kind, lam, mu, friction_deg=30.0)
+    m = scenes.ModelSpec(mat, 1000 * vol, vol, x, v)
+    return scenes.Scene(f"box{kind}", (dim, dim, dim), (0, 0, 0), dx, gravity, dt, 10, [m])
+
+
+@pytest.mark.parametrize("kind", [scenes.FIXED_COROTATED, scenes.NEO_HOOKEAN, scenes.DRUCKER_PRAGER, scenes.FLUID])
+def test_material_box_lockstep(kind):
+    sc = _box_scene(kind, seed=10 + kind)
+    _lockstep(sc, 6, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+
+
+def test_mixed_models_lockstep():
+    """Two models (fluid + sand) sharing one grid, as in the 8M mixed workload."""
+    sc = scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8)
+    _lockstep(sc, 4, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+
+
+def test_out_of_window_is_reported_not_faulting():
+    """A particle teleported 3 blocks away: the substep flags it and returns an error code."""
+    from ckmpm_amd.native import CkmpmError, ERR_OUT_OF_WINDOW
+
+    sc = scenes.colliding_spheres()
+    eng = engine_from_scene(sc)
+    st = eng.state(0)
+    st[0, 5] += 12 * sc.dx
+    eng.set_state(0, st)
+    eng.Substep(0.0, sc.dt)
+    with pytest.raises(CkmpmError) as ei:
+        eng.Synchronize()
+    assert ei.value.code == ERR_OUT_OF_WINDOW
