@@ -76,11 +76,13 @@ def main():
     print(f"[bench] scene {sc.name}: {sc.n_particles} particles, {time.perf_counter() - t:.1f} s", file=sys.stderr,
           flush=True)
     eng = engine_from_scene(sc)
+    print(f"[bench] engine set up, {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     dt = sc.dt
     for k in range(args.warmup):
         eng.Substep(0.0 if k == 0 else dt, dt)
     eng.Synchronize()
     torch.cuda.synchronize()
+    print(f"[bench] {args.warmup} warmup substeps done, timing {args.steps}", file=sys.stderr, flush=True)
     c0 = eng.counters()
     eng.set_profiling(True)
     torch.cuda.synchronize()

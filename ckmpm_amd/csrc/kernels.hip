@@ -3,7 +3,8 @@
 // Data layout in HBM (all FP32 / u32, structure of arrays):
 //   particle state   st[attr][n]      u (3: position relative to the origin 4 b dx of the
 //                                     block b that processed the particle last), then
-//                                     F (9, row-major) | q = J - 1 (fluid), then logJp (DP)
+//                                     E = F - I (9, row-major) | q = J - 1 (fluid), then
+//                                     logJp (DP)
 //   ordering         perm[t] = slot | move << 27: slot of the previous state array and
 //                    the particle's block move since then; off[j] block ranges
 //   grid mass/mom.   M[block][ch 4][node 128]   node = g*64 + lx*16 + ly*4 + lz
@@ -231,7 +232,8 @@ __global__ void SetupOffsetsKernel(const uint32_t* sorted, uint32_t n, const uin
 // Initial state in processing order (CopyParticleArrayToParticleBuffer,
 // mpm_algorithm.cuh:973-1081): slot t holds input particle order[t], its position
 // relative to its block origin 4 b dx (exact for power-of-two dx: the product is exact
-// and the difference of two nearby numbers is exact); F = I, logJp = 0, J = J0.  The
+// and the difference of two nearby numbers is exact); F = I (E = 0), logJp = 0,
+// J = J0 (q = J0 - 1).  The
 // ordering becomes the identity with move 13 (no move), like InitializeAdvectionBucket
 // (mpm_algorithm.cuh:949-964).
 __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const float* lam_p, const float* mu_p,
@@ -254,7 +256,7 @@ __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const f
             st[3 * N + p] = J0 - 1.0f;  // the fluid state is q = J - 1
         } else {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) st[(3 + k) * N + p] = (k % 4 == 0) ? 1.0f : 0.0f;
+            for (int k = 0; k < 9; ++k) st[(3 + k) * N + p] = 0.0f;  // E = F - I = 0
             if (kind == kDruckerPrager) st[12 * N + p] = 0.0f;
         }
     }
@@ -506,6 +508,12 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
     const uint32_t hi = static_cast<uint32_t>((static_cast<uint64_t>(A) * (xcd + 1)) / nx);
 
     for (uint32_t j = lo + blockIdx.x / nx; j < hi; j += per) {
+        const uint32_t beg = a.off[j], end = a.off[j + 1];
+        if (beg == end) {
+            // no particle of this model here: only the (zero) move counts are written
+            if (tid < 27) a.cnt[static_cast<size_t>(j) * kDirections + tid] = 0u;
+            continue;
+        }
         int bb[3];
         CoordsOf(a.dom, a.keys[j], bb[0], bb[1], bb[2]);
         if (tid < 8) {
@@ -535,7 +543,6 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
         }
         __syncthreads();
 
-        const uint32_t beg = a.off[j], end = a.off[j + 1];
         for (uint32_t c0 = beg; c0 < end; c0 += kThreads) {
             const uint32_t t = c0 + tid;
             const bool active = t < end;
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
                 // position relative to this block's origin: the stored value is relative
                 // to the block that wrote it, which sits at this block - move
                 const int mov[3] = {mv / 9 - 1, (mv / 3) % 3 - 1, mv % 3 - 1};
-                float x[3], F[9], q = 0.0f, logJp = 0.0f;  // q = J - 1 (fluid)
+                float x[3], F[9], q = 0.0f, logJp = 0.0f;  // F holds E = F - I; q = J - 1 (fluid)
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
                     x[k] = __fsub_rn(a.st_in[k * N + s], __fmul_rn(static_cast<float>(4 * mov[k]), dx));
@@ -636,10 +643,11 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
                 if (KIND == kFluid) {
                     StressFluid(a.mat, C, a.dt, q, tau);
                 } else {
-                    float dW[9];
+                    // F' = (I + dt C) F  <=>  E' = E + dt C (I + E), E = F - I
+                    float CE[9];
+                    MatMul3(C, F, CE);
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) dW[k] = C[k] * a.dt + ((k % 4 == 0) ? 1.0f : 0.0f);
-                    MatMul3(dW, F, F);
+                    for (int k = 0; k < 9; ++k) F[k] += a.dt * (C[k] + CE[k]);
                     if (KIND == kFixedCorotated) StressFixedCorotated(lam, mu, F, tau);
                     else if (KIND == kNeoHookean) StressNeoHookean(lam, mu, F, tau);
                     else StressDruckerPrager(lam, mu, a.mat.sin_phi, F, logJp, tau);

@@ -37,51 +37,68 @@ __device__ __forceinline__ void InverseSym3(const float (&a)[9], float (&r)[9]) 
     r[8] = id * c22;
 }
 
+// The solid state holds E = F - I (the reference stores F): an FP32 F near the identity
+// keeps only ~6e-8 absolute resolution, a large relative error of small strains; E keeps
+// full relative precision and the stresses below are evaluated from it without forming
+// differences of numbers near 1.
+
+// det(I + E) - 1 = tr E + ((tr E)^2 - tr(E^2)) / 2 + det E
+__device__ __forceinline__ float DetIPlusEMinus1(const float (&E)[9]) {
+    const float tr = E[0] + E[4] + E[8];
+    const float tr2 = E[0] * E[0] + E[4] * E[4] + E[8] * E[8] + 2.0f * (E[1] * E[3] + E[2] * E[6] + E[5] * E[7]);
+    return tr + 0.5f * (tr * tr - tr2) + Det3(E);
+}
+
 // Fixed corotated, mpm_material.cuh:318-357: tau = U diag(2 mu s (s - 1) + lambda J (J - 1)) U^T.
-__device__ __forceinline__ void StressFixedCorotated(float lambda, float mu, const float (&F)[9], float (&tau)[9]) {
-    Svd3 d;
-    SignedSvd3(F, d);
-    const float J = d.s[0] * d.s[1] * d.s[2];
-    const float vol = lambda * J * (J - 1.0f);
+__device__ __forceinline__ void StressFixedCorotated(float lambda, float mu, const float (&E)[9], float (&tau)[9]) {
+    Svd3 sv;
+    float d[3];
+    SignedSvdStrain(E, sv, d);
+    const float jm1 = d[0] + d[1] + d[2] + d[0] * d[1] + d[0] * d[2] + d[1] * d[2] + d[0] * d[1] * d[2];
+    const float vol = lambda * (1.0f + jm1) * jm1;
     float k[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) k[a] = 2.0f * mu * d.s[a] * (d.s[a] - 1.0f) + vol;
+    for (int a = 0; a < 3; ++a) k[a] = 2.0f * mu * sv.s[a] * d[a] + vol;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = i; j < 3; ++j) {
-            const float v = d.U[i * 3 + 0] * k[0] * d.U[j * 3 + 0] + d.U[i * 3 + 1] * k[1] * d.U[j * 3 + 1] +
-                            d.U[i * 3 + 2] * k[2] * d.U[j * 3 + 2];
+            const float v = sv.U[i * 3 + 0] * k[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * k[1] * sv.U[j * 3 + 1] +
+                            sv.U[i * 3 + 2] * k[2] * sv.U[j * 3 + 2];
             tau[i * 3 + j] = v;
             tau[j * 3 + i] = v;
         }
 }
 
-// Neo-Hookean (closed form): tau = mu (F F^T - I) + lambda ln(J) I.
-__device__ __forceinline__ void StressNeoHookean(float lambda, float mu, const float (&F)[9], float (&tau)[9]) {
-    const float lnJ = logf(Det3(F));
+// Neo-Hookean (closed form): tau = mu (F F^T - I) + lambda ln(J) I,
+// F F^T - I = E + E^T + E E^T.
+__device__ __forceinline__ void StressNeoHookean(float lambda, float mu, const float (&E)[9], float (&tau)[9]) {
+    const float lnJ = log1pf(DetIPlusEMinus1(E));
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = i; j < 3; ++j) {
-            const float b = F[i * 3 + 0] * F[j * 3 + 0] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2];
-            const float v = mu * (b - (i == j ? 1.0f : 0.0f)) + (i == j ? lambda * lnJ : 0.0f);
+            const float b = E[i * 3 + j] + E[j * 3 + i] + E[i * 3 + 0] * E[j * 3 + 0] + E[i * 3 + 1] * E[j * 3 + 1] +
+                            E[i * 3 + 2] * E[j * 3 + 2];
+            const float v = mu * b + (i == j ? lambda * lnJ : 0.0f);
             tau[i * 3 + j] = v;
             tau[j * 3 + i] = v;
         }
 }
 
 // Drucker-Prager StVK-Hencky return map, mpm_material.cuh:360-464 (cohesion 0, beta 1,
-// volume correction).  Projects F in place and updates logJp.
-__device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, float sin_phi, float (&F)[9], float& logJp,
+// volume correction).  Projects E = F - I in place and updates logJp.  In the elastic
+// branch the reference rebuilds F = U diag(s) V^T, which equals F; E is kept as is.
+__device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, float sin_phi, float (&E)[9], float& logJp,
                                                     float (&tau)[9]) {
     const float alpha = 0.816496580927726f * 2.0f * sin_phi / (3.0f - sin_phi);  // sqrt(2/3) 2 sin / (3 - sin)
     const float smu = 2.0f * mu;
-    Svd3 d;
-    SignedSvd3(F, d);
+    Svd3 sv;
+    float d[3];
+    SignedSvdStrain(E, sv, d);
     float eps[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) eps[a] = logf(fmaxf(fabsf(d.s[a]), 1e-4f));
+    for (int a = 0; a < 3; ++a) eps[a] = sv.s[a] >= 1e-4f ? log1pf(d[a]) : logf(fmaxf(fabsf(sv.s[a]), 1e-4f));
     const float eps_sum = eps[0] + eps[1] + eps[2];
     const float eps_tr = eps_sum + logJp;
     float eh[3];
@@ -89,8 +106,8 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
     for (int a = 0; a < 3; ++a) eh[a] = eps[a] - eps_tr * (1.0f / 3.0f);
     const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
     float H[3] = {eps[0], eps[1], eps[2]};
-    bool tension = eps_tr >= 0.0f;
-    if (tension) {
+    bool rebuild = true;
+    if (eps_tr >= 0.0f) {
         H[0] = H[1] = H[2] = 0.0f;
         logJp = eps_sum + logJp;
     } else if (mu != 0.0f) {
@@ -100,31 +117,35 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
             const float r = dg / ehn;
 #pragma unroll
             for (int a = 0; a < 3; ++a) H[a] = eps[a] - r * eh[a];
+        } else {
+            rebuild = false;
         }
+    } else {
+        rebuild = false;
     }
-    float ns[3], ph[3];
     const float htr = H[0] + H[1] + H[2];
+    float k[3], ns[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         ns[a] = expf(H[a]);
-        ph[a] = (smu * H[a] + lambda * htr) / ns[a];  // log(ns) = H
+        k[a] = smu * H[a] + lambda * htr;  // (2 mu log ns + lambda tr log ns) / ns * ns
     }
-    // F = U diag(ns) V^T ; tau = U diag(ph) V^T F^T = U diag(ph * ns) U^T
+    if (rebuild) {
+        // E = U diag(ns) V^T - I
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            F[i * 3 + j] = d.U[i * 3 + 0] * ns[0] * d.V[j * 3 + 0] + d.U[i * 3 + 1] * ns[1] * d.V[j * 3 + 1] +
-                           d.U[i * 3 + 2] * ns[2] * d.V[j * 3 + 2];
-    float k[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) k[a] = ph[a] * ns[a];
+            for (int j = 0; j < 3; ++j)
+                E[i * 3 + j] = sv.U[i * 3 + 0] * ns[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * ns[1] * sv.V[j * 3 + 1] +
+                               sv.U[i * 3 + 2] * ns[2] * sv.V[j * 3 + 2] - (i == j ? 1.0f : 0.0f);
+    }
+    // tau = U diag(phat) V^T F^T = U diag(phat * ns) U^T
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = i; j < 3; ++j) {
-            const float v = d.U[i * 3 + 0] * k[0] * d.U[j * 3 + 0] + d.U[i * 3 + 1] * k[1] * d.U[j * 3 + 1] +
-                            d.U[i * 3 + 2] * k[2] * d.U[j * 3 + 2];
+            const float v = sv.U[i * 3 + 0] * k[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * k[1] * sv.U[j * 3 + 1] +
+                            sv.U[i * 3 + 2] * k[2] * sv.U[j * 3 + 2];
             tau[i * 3 + j] = v;
             tau[j * 3 + i] = v;
         }

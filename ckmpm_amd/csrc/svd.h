@@ -152,4 +152,55 @@ __device__ __forceinline__ void SignedSvd3(const float (&A)[9], Svd3& out) {
     out.s[2] = B[8];
 }
 
+// Signed SVD of F = I + E that also returns d_i = s_i - 1 to full relative precision.
+// The Jacobi stage diagonalises M = F^T F - I = E + E^T + E^T E, whose entries are of
+// strain size, so its eigenvalues m_i (s_i^2 = 1 + m_i) keep the strain's relative
+// precision; d_i = m_i / (1 + sqrt(1 + m_i)) for a positive s_i and -(1 + sqrt(1 + m_i))
+// for the negative s_2 of an inverted F.  U comes from the Givens QR of F V as above.
+__device__ __forceinline__ void SignedSvdStrain(const float (&E)[9], Svd3& out, float (&d)[3]) {
+    using namespace svd_detail;
+    float S[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            const float v = E[i * 3 + j] + E[j * 3 + i] + E[0 * 3 + i] * E[0 * 3 + j] + E[1 * 3 + i] * E[1 * 3 + j] +
+                            E[2 * 3 + i] * E[2 * 3 + j];
+            S[i * 3 + j] = v;
+            S[j * 3 + i] = v;
+        }
+    float V[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+#pragma unroll
+    for (int sweep = 0; sweep < 4; ++sweep) {
+        JacobiConjugate<0, 1, 2>(S, V);
+        JacobiConjugate<0, 2, 1>(S, V);
+        JacobiConjugate<1, 2, 0>(S, V);
+    }
+    float B[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            B[i * 3 + j] = V[i * 3 + j] + E[i * 3 + 0] * V[0 * 3 + j] + E[i * 3 + 1] * V[1 * 3 + j] + E[i * 3 + 2] * V[2 * 3 + j];
+    float m[3] = {S[0], S[4], S[8]};
+    SortPair<0, 1>(B, V, m);
+    SortPair<0, 2>(B, V, m);
+    SortPair<1, 2>(B, V, m);
+    float U[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    QrGivens<0, 1, 0>(B, U);
+    QrGivens<0, 2, 0>(B, U);
+    QrGivens<1, 2, 1>(B, U);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        out.U[k] = U[k];
+        out.V[k] = V[k];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float root = sqrtf(fmaxf(1.0f + m[i], 0.0f));
+        d[i] = B[i * 4] >= 0.0f ? m[i] / (1.0f + root) : -(1.0f + root);
+        out.s[i] = 1.0f + d[i];
+    }
+}
+
 }  // namespace ckmpm

@@ -183,8 +183,8 @@ class MPMEngine:
         return perm, off
 
     def global_state(self, model: int) -> np.ndarray:
-        """State in the reference's variables, FP64: global positions (exact origin + u) and,
-        for a fluid, J = 1 + q."""
+        """State in the reference's variables, FP64: global positions (exact origin + u),
+        F = I + E for a solid, J = 1 + q for a fluid."""
         st = self.state(model).astype(np.float64)
         keys, (A, _, _) = self.layout(1)
         _, off = self.ordering(model, 1)
@@ -194,7 +194,9 @@ class MPMEngine:
         coords = np.stack([k // (dim[1] * dim[2]), (k // dim[2]) % dim[1], k % dim[2]])
         st[:3] += 4.0 * coords * np.float64(np.float32(self.config.dx))
         if self.models[model].material.kind == FLUID:
-            st[3] += 1.0
+            st[3] += 1.0                     # J = 1 + q
+        else:
+            st[[3, 7, 11]] += 1.0            # F = I + E
         return st
 
     def state(self, model: int) -> np.ndarray:

@@ -138,7 +138,8 @@ int ckmpm_export_ordering(ckmpm_engine* eng, int32_t model, int32_t which, uint3
                           uint32_t offsets_capacity);
 /* SoA [attr][n] float32, storage order: u (3, position relative to the origin 4 b dx of
  * the block b that wrote it: block j of the previous ordering for slot t in its range),
- * then F (9, row-major) or q = J - 1 (fluid), then logJp (DP) */
+ * then E = F - I (9, row-major) or q = J - 1 (fluid), then logJp (DP); strains are
+ * stored as deviations from the identity so that FP32 keeps their relative precision */
 int ckmpm_export_state(ckmpm_engine* eng, int32_t model, float* soa);
 int ckmpm_import_state(ckmpm_engine* eng, int32_t model, const float* soa);
 /* grid mass/momentum of the previous layout, [block][m,px,py,pz][node 128] */

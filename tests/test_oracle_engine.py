@@ -1,0 +1,207 @@
+"""Pins the FP64 oracle's full substep with invariants the mathematics fixes.
+
+* mass and momentum: P2G deposits exactly 2 sum m_p on the two grids and, without
+  gravity or boundary contact, conserves 2 sum m_p v_p (the two grids' first moments
+  cancel, so the affine and stress terms carry no net momentum);
+* uniform translation: C = 0, x += dt u exactly, F stays I (a dropped 1/2 on the
+  two-grid average or a wrong node offset breaks it);
+* free fall: v_k = k g dt, x_k = x_0 + g dt^2 k (k + 1) / 2 (symplectic Euler);
+* mirror symmetry of the colliding spheres about x = 1/2 (a swapped sub-grid or a
+  transposed matrix breaks it);
+* stress direction: a compressed block pushes its grid momentum outward (a sign error
+  in the stress term pulls it inward);
+* rebin: an independent numpy restatement of the canonical ordering.
+"""
+import numpy as np
+import pytest
+
+from ckmpm_amd import scenes
+from oracle.pyoracle import Oracle
+
+SLOT = (1 << 27) - 1
+
+
+def _box(kind, lo_cell, n_cells, dx, v, gravity=0.0, dim=16, seed=4):
+    rng = scenes.Splitmix(seed)
+    g = np.stack(np.meshgrid(*[np.arange(lo_cell[a], lo_cell[a] + n_cells) for a in range(3)], indexing="ij"), -1)
+    x = scenes._stratified_cells(rng, g.reshape(-1, 3).astype(np.float64), dx).astype(np.float32)
+    lam, mu = scenes.lame(1e4, 0.3)
+    mat = scenes.Material(kind, lam, mu, bulk=1e4, gamma=7.0, viscosity=0.0)
+    vol = dx ** 3 / 8
+    m = scenes.ModelSpec(mat, 1000 * vol, vol, x, np.array([v], np.float32))
+    return scenes.Scene("box", (dim, dim, dim), (0, 0, 0), dx, gravity, 1e-4, 10, [m])
+
+
+def _totals(o):
+    M = o.grid_m()
+    return M[:, 0].sum(), M[:, 1:].sum(axis=(0, 2))
+
+
+def test_mass_and_momentum_conservation():
+    sc = scenes.colliding_spheres(speed=0.7)
+    o = Oracle.from_scene(sc)
+    o.setup()
+    m_tot = sum(m.mass * m.x.shape[0] for m in sc.models)
+    p_tot = sum(m.mass * m.x.shape[0] * m.v[0].astype(np.float64) for m in sc.models)
+    mass, mom = _totals(o)
+    assert abs(mass - 2 * m_tot) < 1e-12 * m_tot
+    np.testing.assert_allclose(mom, 2 * p_tot, atol=1e-12 * m_tot)
+    for k in range(6):
+        o.substep(0.0 if k == 0 else sc.dt, sc.dt)
+        mass, mom = _totals(o)
+        assert abs(mass - 2 * m_tot) < 1e-12 * m_tot
+        np.testing.assert_allclose(mom, 2 * p_tot, atol=1e-11 * m_tot)
+    assert o.flags == 0
+
+
+@pytest.mark.parametrize("kind", [scenes.FIXED_COROTATED, scenes.NEO_HOOKEAN, scenes.DRUCKER_PRAGER, scenes.FLUID])
+def test_uniform_translation(kind):
+    u = np.array([0.3, -0.2, 0.1])
+    sc = _box(kind, (24, 24, 24), 6, 1 / 64, u)
+    o = Oracle.from_scene(sc)
+    o.setup()
+    x0 = o.state(0)[:3].copy()
+    steps = 8
+    for k in range(steps):
+        o.substep(0.0 if k == 0 else sc.dt, sc.dt)
+    # state is in processing order; follow identities through the orderings
+    st = o.state(0)
+    ids = np.arange(x0.shape[1])
+    o2 = Oracle.from_scene(sc)
+    o2.setup()
+    for k in range(steps):
+        ids = ids[o2.perm(0) & SLOT]
+        o2.substep(0.0 if k == 0 else sc.dt, sc.dt)
+    disp = st[:3] - x0[:, ids]
+    np.testing.assert_allclose(disp, (steps - 1) * sc.dt * u[:, None] * np.ones_like(disp), atol=1e-13)
+    if kind == scenes.FLUID:
+        np.testing.assert_allclose(st[3], 1.0, atol=1e-12)
+    else:
+        np.testing.assert_allclose(st[3:12], np.eye(3).reshape(9, 1) * np.ones((9, st.shape[1])), atol=1e-12)
+
+
+def test_free_fall_is_symplectic_euler():
+    g = -9.8
+    sc = _box(scenes.NEO_HOOKEAN, (24, 30, 24), 6, 1 / 64, (0.0, 0.0, 0.0), gravity=g)
+    o = Oracle.from_scene(sc)
+    o.setup()
+    y0 = o.state(0)[1].copy()
+    ids = np.arange(y0.shape[0])
+    K = 12
+    for k in range(K + 1):   # one priming substep (dt = 0), then K substeps
+        ids = ids[o.perm(0) & SLOT]
+        o.substep(0.0 if k == 0 else sc.dt, sc.dt)
+    y = o.state(0)[1]
+    np.testing.assert_allclose(y - y0[ids], g * sc.dt ** 2 * K * (K + 1) / 2, rtol=1e-10, atol=1e-15)
+    np.testing.assert_allclose(o.max_v2, (g * sc.dt * K) ** 2, rtol=1e-10)
+
+
+def test_colliding_spheres_mirror_symmetry():
+    sc = scenes.colliding_spheres(speed=1.0)
+    # make sphere 2 the exact mirror image of sphere 1 about x = 1/2
+    m0 = sc.models[0]
+    m0.x = (np.round(m0.x.astype(np.float64) * 2 ** 23) / 2 ** 23).astype(np.float32)  # 1 - x exact
+    x1 = m0.x.astype(np.float64).copy()
+    x1[:, 0] = 1.0 - x1[:, 0]
+    sc.models[1].x = x1.astype(np.float32)
+    assert np.array_equal(sc.models[1].x[:, 0].astype(np.float64), 1.0 - m0.x[:, 0].astype(np.float64))
+    o = Oracle.from_scene(sc)
+    o.setup()
+    orders = [o.setup_order(m).astype(np.int64) for m in range(2)]
+    for k in range(20):
+        for m in range(2):
+            orders[m] = orders[m][o.perm(m) & SLOT]
+        o.substep(0.0 if k == 0 else sc.dt, sc.dt)
+    xs = []
+    for m in range(2):
+        st = o.state(m)
+        x = np.empty((3, st.shape[1]))
+        x[:, orders[m]] = st[:3]
+        xs.append(x)
+    np.testing.assert_allclose(xs[1][0], 1.0 - xs[0][0], atol=1e-12)
+    np.testing.assert_allclose(xs[1][1:], xs[0][1:], atol=1e-12)
+    _, mom = _totals(o)
+    assert abs(mom[0]) < 1e-12
+
+
+def test_compressed_block_pushes_outward():
+    sc = _box(scenes.FIXED_COROTATED, (24, 24, 24), 8, 1 / 64, (0.0, 0.0, 0.0))
+    o = Oracle.from_scene(sc)
+    o.setup()
+    st = o.state(0).copy()
+    st[3:12] = (0.98 * np.eye(3)).reshape(9, 1)
+    o.set_state(0, st)
+    o.physics(0.0, sc.dt)
+    M = o.grid_mnext()
+    keys = o.keys(0)
+    dim = sc.dim
+    # node positions of the grid-0 / grid-1 nodes of each block
+    radial = 0.0
+    centre = st[:3].mean(axis=1)
+    for bi in range(M.shape[0]):
+        k = int(keys[bi])
+        b = np.array([k // (dim[1] * dim[2]), (k // dim[2]) % dim[1], k % dim[2]])
+        for n in range(128):
+            g, l = divmod(n, 64)
+            c = 4 * b + np.array([l // 16, (l // 4) % 4, l % 4])
+            xn = (c + (0.25 if g else -0.25)) * sc.dx
+            radial += M[bi, 1:, n] @ (xn - centre)
+    assert radial > 0.0
+
+
+def _expected_rebin(o, nmodels, dim, dx):
+    """Canonical rebin restated in numpy from the oracle's pre-rebin state."""
+    keys = o.keys(0)
+    A, Nb, Ex = o.counts(0)
+    index = {int(k): i for i, k in enumerate(keys)}
+    coords = lambda k: np.array([k // (dim[1] * dim[2]), (k // dim[2]) % dim[1], k % dim[2]])
+    targets, moves = [], []
+    occupied = np.zeros(Ex, bool)
+    for m in range(nmodels):
+        st = o.state_next(m)
+        off = o.offsets(m)
+        tg = np.zeros(st.shape[1], np.int64)
+        mv = np.zeros(st.shape[1], np.int64)
+        for j in range(A):
+            b = coords(int(keys[j]))
+            for t in range(off[j], off[j + 1]):
+                u = (st[:3, t] - 4 * b * dx).astype(np.float32)
+                cell = np.floor((u * np.float32(1 / dx)).astype(np.float32) + np.float32(0.25)).astype(np.int64)
+                d = np.clip(np.floor_divide(cell - 2, 4), -1, 1)
+                nb = b + d
+                tg[t] = index[int((nb[0] * dim[1] + nb[1]) * dim[2] + nb[2])]
+                mv[t] = (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1)
+        occupied[tg] = True
+        targets.append(tg)
+        moves.append(mv)
+    newidx = np.cumsum(occupied) - 1
+    out = []
+    for m in range(nmodels):
+        ni = newidx[targets[m]]
+        order = np.argsort(ni, kind="stable")
+        perm = order | (moves[m][order] << 27)
+        counts = np.bincount(ni, minlength=int(occupied.sum()))
+        off = np.concatenate([[0], np.cumsum(counts)])
+        out.append((perm.astype(np.uint32), off.astype(np.uint32)))
+    return keys[occupied], out
+
+
+def test_rebin_matches_numpy_restatement():
+    sc = scenes.colliding_spheres(speed=3.0)   # fast, so particles cross blocks
+    o = Oracle.from_scene(sc)
+    o.setup()
+    crossed = 0
+    for k in range(12):
+        o.physics(0.0 if k == 0 else 4 * sc.dt, sc.dt)
+        akeys, exp = _expected_rebin(o, 2, sc.dim, sc.dx)
+        o.rebin()
+        A = o.counts(0)[0]
+        np.testing.assert_array_equal(o.keys(0)[:A], akeys)
+        for m in range(2):
+            perm, off = o.perm(m), o.offsets(m)
+            np.testing.assert_array_equal(perm, exp[m][0])
+            np.testing.assert_array_equal(off, exp[m][1])
+            assert np.array_equal(np.sort(perm & SLOT), np.arange(perm.shape[0]))
+            crossed += int(((perm >> 27) != 13).sum())
+    assert crossed > 100
+    assert o.flags == 0
