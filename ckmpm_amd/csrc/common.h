@@ -10,6 +10,7 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kDirections = 27;      // new block - old block in {-1,0,1}^3
 constexpr int kMoveShift = 27;       // ordering word: slot (27 bits) | move << 27
 constexpr uint32_t kSlotMask = (1u << kMoveShift) - 1u;
+constexpr uint32_t kSetupRankClamp = (1u << 26) - 1u;  // within-cell rank in the setup sort key
 constexpr int kNodesPerBlock = 128;  // 2 sub-grids x 4^3 nodes
 constexpr int kMChannels = 4;        // m, px, py, pz
 constexpr int kVChannels = 3;        // vx, vy, vz

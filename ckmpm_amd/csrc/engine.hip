@@ -261,7 +261,7 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
         if (domain->dim[a] <= 0 || domain->dim[a] > 1024) return CKMPM_ERR_INVALID;
     if (!(sim->dx > 0.0f) || sim->block_capacity == 0) return CKMPM_ERR_INVALID;
     const uint64_t nblocks = static_cast<uint64_t>(domain->dim[0]) * domain->dim[1] * domain->dim[2];
-    if (nblocks >= (1ull << 31)) return CKMPM_ERR_INVALID;
+    if (nblocks > (1ull << 26)) return CKMPM_ERR_INVALID;  // block key << 6 | cell fits 32 bits
     ckmpm_engine* e = new ckmpm_engine();
     for (int a = 0; a < 3; ++a) {
         e->dom.dim[a] = domain->dim[a];
@@ -375,28 +375,29 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
     uint32_t nmax = 0;
     for (ModelBuf& m : e->models) nmax = std::max(nmax, m.n);
     float *xyz = nullptr, *vel = nullptr, *lam = nullptr, *mu = nullptr;
-    uint32_t *keys = nullptr, *vals = nullptr, *keys_alt = nullptr, *vals_alt = nullptr, *rtmp = nullptr;
+    uint32_t* keys = nullptr;
     std::vector<uint32_t*> sorted(e->models.size(), nullptr);
     const size_t mark = e->allocs.size();
     CK_TRY(Alloc(e, &xyz, 3 * static_cast<size_t>(nmax)));
     CK_TRY(Alloc(e, &vel, 3 * static_cast<size_t>(nmax)));
     CK_TRY(Alloc(e, &lam, nmax));
     CK_TRY(Alloc(e, &mu, nmax));
-    CK_TRY(Alloc(e, &vals_alt, nmax));
-    CK_TRY(Alloc(e, &keys_alt, nmax));
-    CK_TRY(Alloc(e, &rtmp, RadixTempWords(nmax)));
+    SetupTemps tmp;
+    for (uint32_t*& t : tmp.t) CK_TRY(Alloc(e, &t, nmax));
+    CK_TRY(Alloc(e, &tmp.keys_alt, nmax));
+    CK_TRY(Alloc(e, &tmp.vals_alt, nmax));
+    CK_TRY(Alloc(e, &tmp.radix, std::max(RadixTempWords(nmax), ScanTempWords(nmax))));
     int key_bits = 1;
     while ((1ull << key_bits) < e->dom.nblocks) ++key_bits;
     LayoutBuf& L = e->L[0];
-    // particle blocks from FP32 positions; per-model stable sort by block key
+    // particle blocks from FP32 positions; per-model canonical setup ordering
     for (size_t i = 0; i < e->models.size(); ++i) {
         ModelBuf& m = e->models[i];
         CK_TRY(Alloc(e, &keys, m.n));
         sorted[i] = keys;
-        vals = m.perm[0];
+        uint32_t* vals = m.perm[0];
         CK_HIP(e, hipMemcpyAsync(xyz, m.x0.data(), 3 * sizeof(float) * m.n, hipMemcpyHostToDevice, s));
-        LaunchSetupKeys(e->dom, xyz, m.n, keys, vals, e->bitmap, e->status, s);
-        RadixSortPairs(keys, vals, keys_alt, vals_alt, m.n, key_bits, rtmp, s);
+        SetupOrder(e->dom, xyz, m.n, key_bits, keys, vals, tmp, e->bitmap, e->status, s);
         if (m.pp_attr) {
             CK_HIP(e, hipMemcpyAsync(lam, m.lam0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
             CK_HIP(e, hipMemcpyAsync(mu, m.mu0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));

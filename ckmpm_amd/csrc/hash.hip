@@ -1,0 +1,417 @@
+// Deterministic three-choice bucketed hash of block keys, and its C-ABI.
+//
+// Build (all on the table's stream, no host sync):
+//   1. sort the (key, value) pairs by key;
+//   2. per round r: bucket[i] = h_r(key_i) % B for unplaced pairs, B (sorts last) for
+//      placed ones; stable sort of pair indices by bucket; rank inside each bucket's
+//      run; pair with rank < free slots of its bucket -> slot fill + rank; then each
+//      run's last element raises the bucket's fill;
+//   3. unplaced pairs, in key order, -> stash.
+// Find: wave-cooperative, a 16-lane quarter wavefront per query and probe (one 8-byte
+// slot per lane, one 128-byte line per bucket), as CooperativeFind
+// (gpu_hash_table.cuh:463-513) does with its 16-thread tiles.
+#include <algorithm>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/ckmpm.h"
+#include "hash.h"
+#include "kernels.h"
+
+using namespace ckmpm;
+
+namespace {
+
+constexpr uint32_t kHashThreads = 256;
+
+__global__ void HashPackKernel(const uint32_t* keys, const uint32_t* vals, uint32_t n, uint32_t* sk, uint32_t* sv,
+                               uint32_t* placed, uint32_t nblocks, uint32_t* bad) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = keys[i];
+        if (k >= nblocks) atomicOr(bad, 1u);
+        sk[i] = k;
+        sv[i] = vals[i];
+        placed[i] = 0u;
+    }
+}
+
+__global__ void HashClearKernel(uint64_t* slots, uint32_t nslots, uint32_t* fill, uint32_t buckets) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) {
+        slots[i] = kHashEmpty;
+        if (i < buckets) fill[i] = 0u;
+    }
+}
+
+// duplicate keys are an error (they would make Find ambiguous)
+__global__ void HashDupKernel(const uint32_t* sk, uint32_t n, uint32_t* bad) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x + 1; i < n; i += gridDim.x * blockDim.x)
+        if (sk[i] == sk[i - 1]) atomicOr(bad, 2u);
+}
+
+__global__ void HashBucketKernel(HashView h, int round, const uint32_t* sk, const uint32_t* placed, uint32_t n,
+                                 uint32_t* bucket, uint32_t* idx) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        bucket[i] = placed[i] ? h.buckets : h.Bucket(sk[i], round);
+        idx[i] = i;
+    }
+}
+
+__global__ void HashPlaceKernel(const uint32_t* bucket, const uint32_t* idx, const uint32_t* rank, uint32_t n,
+                                uint32_t buckets, const uint32_t* sk, const uint32_t* sv, const uint32_t* fill,
+                                uint64_t* slots, uint32_t* placed) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t b = bucket[i];
+        if (b >= buckets) continue;
+        const uint32_t f = fill[b];
+        if (rank[i] < kHashBucket - f) {
+            const uint32_t p = idx[i];
+            slots[static_cast<size_t>(b) * kHashBucket + f + rank[i]] =
+                (static_cast<uint64_t>(sk[p]) << 32) | static_cast<uint64_t>(sv[p]);
+            placed[p] = 1u;
+        }
+    }
+}
+
+__global__ void HashFillKernel(const uint32_t* bucket, const uint32_t* rank, uint32_t n, uint32_t buckets,
+                               uint32_t* fill) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t b = bucket[i];
+        if (b >= buckets) continue;
+        if (i + 1 < n && bucket[i + 1] == b) continue;  // not the run's last element
+        fill[b] = min(static_cast<uint32_t>(kHashBucket), fill[b] + rank[i] + 1u);
+    }
+}
+
+__global__ void HashUnplacedKernel(const uint32_t* placed, uint32_t n, uint32_t* flag) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flag[i] = placed[i] ? 0u : 1u;
+}
+
+__global__ void HashStashKernel(const uint32_t* flag, const uint32_t* pos, uint32_t n, const uint32_t* sk,
+                                const uint32_t* sv, uint64_t* stash, uint32_t stash_cap, const uint32_t* total,
+                                uint32_t* nstash, uint32_t* bad) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (flag[i] && pos[i] < stash_cap)
+            stash[pos[i]] = (static_cast<uint64_t>(sk[i]) << 32) | static_cast<uint64_t>(sv[i]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t t = *total;
+        *nstash = min(t, stash_cap);
+        if (t > stash_cap) atomicOr(bad, 4u);
+    }
+}
+
+// A 16-lane group serves the queries of its own 16 lanes in turn; lane l reads slot l.
+__global__ __launch_bounds__(kHashThreads) void HashFindKernel(HashView h, const uint32_t* keys, uint32_t n,
+                                                               uint32_t* out) {
+    const uint32_t lane = threadIdx.x & 63u, sub = lane & 15u, gbase = lane & ~15u;
+    const uint64_t gmask = 0xFFFFull << gbase;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t iters = (n + stride - 1) / stride;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool have = i < n;
+        const uint32_t mykey = have ? keys[i] : 0u;
+        uint32_t result = kEmpty;
+        for (uint32_t q = 0; q < 16u; ++q) {
+            const uint32_t owner = gbase + q;
+            const bool owner_has = (__ballot(have) >> owner) & 1ull;
+            if (!owner_has) continue;
+            const uint32_t key = __shfl(mykey, static_cast<int>(owner), 64);
+            uint32_t found = kEmpty;
+            bool done = false;
+            for (int r = 0; r < kHashRounds && !done; ++r) {
+                const uint64_t w = h.slots[static_cast<size_t>(h.Bucket(key, r)) * kHashBucket + sub];
+                const uint64_t hit = __ballot(static_cast<uint32_t>(w >> 32) == key) & gmask;
+                const uint64_t empty = __ballot(w == kHashEmpty) & gmask;
+                if (hit) {
+                    const int src = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+                    found = __shfl(static_cast<uint32_t>(w), src, 64);
+                    done = true;
+                } else if (empty) {
+                    done = true;
+                }
+            }
+            if (!done) {
+                // every probed bucket was full: scan the stash, 16 words at a time
+                const uint32_t ns = *h.nstash;
+                for (uint32_t b0 = 0; b0 < ns && !done; b0 += 16u) {
+                    const uint32_t j = b0 + sub;
+                    const uint64_t w = j < ns ? h.stash[j] : kHashEmpty;
+                    const uint64_t hit = __ballot(static_cast<uint32_t>(w >> 32) == key) & gmask;
+                    if (hit) {
+                        const int src = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+                        found = __shfl(static_cast<uint32_t>(w), src, 64);
+                        done = true;
+                    }
+                }
+            }
+            if (sub == q) result = found;
+        }
+        if (have) out[i] = result;
+    }
+}
+
+// One key per thread (HashView::Find), for comparison with the cooperative form.
+__global__ __launch_bounds__(kHashThreads) void HashFindScalarKernel(HashView h, const uint32_t* keys, uint32_t n,
+                                                                     uint32_t* out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = h.Find(keys[i]);
+}
+
+__global__ void HashHistKernel(const uint64_t* slots, uint32_t buckets, uint32_t* hist) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets; b += gridDim.x * blockDim.x) {
+        uint32_t f = 0;
+        for (int i = 0; i < kHashBucket; ++i) f += slots[static_cast<size_t>(b) * kHashBucket + i] != kHashEmpty;
+        atomicAdd(&hist[f], 1u);
+    }
+}
+
+int KeyBits(uint32_t nblocks) {
+    int b = 1;
+    while ((1ull << b) < nblocks) ++b;
+    return b;
+}
+
+}  // namespace
+
+struct ckmpm_hash {
+    hipStream_t stream = nullptr;
+    int dim[3] = {0, 0, 0};
+    uint32_t nblocks = 0;
+    uint32_t max_keys = 0;
+    uint32_t buckets = 0;
+    uint32_t stash_cap = 0;
+    uint32_t seed[kHashRounds] = {0, 0, 0};
+    uint64_t* slots = nullptr;
+    uint64_t* stash = nullptr;
+    uint32_t* nstash = nullptr;
+    uint32_t* bad = nullptr;
+    uint32_t* fill = nullptr;
+    uint32_t* tmp = nullptr;  // 10 arrays of max_keys words + scan/radix temporaries
+    uint32_t* dev_io = nullptr;  // 2 * max_keys words for the host-pointer entry points
+    uint32_t n = 0;
+    std::string err;
+};
+
+namespace {
+
+int HashFail(ckmpm_hash* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HASH_HIP(h, call)                                                                     \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) return HashFail((h), CKMPM_ERR_HIP, std::string(#call) + ": " + \
+                                                           hipGetErrorString(_e));            \
+    } while (0)
+
+HashView View(const ckmpm_hash* h) {
+    HashView v{};
+    v.slots = h->slots;
+    v.stash = h->stash;
+    v.nstash = h->nstash;
+    v.buckets = h->buckets;
+    for (int r = 0; r < kHashRounds; ++r) v.seed[r] = h->seed[r];
+    for (int a = 0; a < 3; ++a) v.dim[a] = h->dim[a];
+    return v;
+}
+
+size_t TmpWords(uint32_t m) {
+    return 10ull * m + std::max(RadixTempWords(m), ScanTempWords(m)) + 64;
+}
+
+// Enqueue the build from device arrays; no host sync.
+void EnqueueBuild(ckmpm_hash* h, const uint32_t* keys, const uint32_t* vals, uint32_t n) {
+    hipStream_t s = h->stream;
+    const uint32_t m = h->max_keys;
+    uint32_t* t = h->tmp;
+    uint32_t *sk = t, *sv = t + m, *placed = t + 2 * m, *bucket = t + 3 * m, *idx = t + 4 * m;
+    uint32_t *rank = t + 5 * m, *t0 = t + 6 * m, *t1 = t + 7 * m, *t2 = t + 8 * m, *alt = t + 9 * m;
+    uint32_t* radix = t + 10 * static_cast<size_t>(m);
+    uint32_t* total = radix + std::max(RadixTempWords(m), ScanTempWords(m));
+    const uint32_t nslots = h->buckets * kHashBucket;
+    (void)hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s);
+    HashClearKernel<<<LaunchBlocks(nslots), kHashThreads, 0, s>>>(h->slots, nslots, h->fill, h->buckets);
+    h->n = n;
+    if (n == 0) {
+        (void)hipMemsetAsync(h->nstash, 0, sizeof(uint32_t), s);
+        return;
+    }
+    const uint32_t g = LaunchBlocks(n);
+    HashPackKernel<<<g, kHashThreads, 0, s>>>(keys, vals, n, sk, sv, placed, h->nblocks, h->bad);
+    RadixSortPairs(sk, sv, t0, t1, n, KeyBits(h->nblocks), radix, s);
+    HashDupKernel<<<g, kHashThreads, 0, s>>>(sk, n, h->bad);
+    const HashView v = View(h);
+    const int bbits = KeyBits(h->buckets + 1);
+    for (int r = 0; r < kHashRounds; ++r) {
+        HashBucketKernel<<<g, kHashThreads, 0, s>>>(v, r, sk, placed, n, bucket, idx);
+        RadixSortPairs(bucket, idx, t0, alt, n, bbits, radix, s);
+        SegmentRanks(bucket, n, rank, t0, t1, t2, radix, s);
+        HashPlaceKernel<<<g, kHashThreads, 0, s>>>(bucket, idx, rank, n, h->buckets, sk, sv, h->fill, h->slots,
+                                                   placed);
+        HashFillKernel<<<g, kHashThreads, 0, s>>>(bucket, rank, n, h->buckets, h->fill);
+    }
+    HashUnplacedKernel<<<g, kHashThreads, 0, s>>>(placed, n, t0);
+    ExclusiveScan(t0, t1, nullptr, n, n, total, false, radix, s);
+    HashStashKernel<<<g, kHashThreads, 0, s>>>(t0, t1, n, sk, sv, h->stash, h->stash_cap, total, h->nstash, h->bad);
+}
+
+int CheckBuild(ckmpm_hash* h) {
+    uint32_t bad = 0;
+    HASH_HIP(h, hipMemcpyAsync(&bad, h->bad, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HASH_HIP(h, hipStreamSynchronize(h->stream));
+    if (bad & 1u) return HashFail(h, CKMPM_ERR_INVALID, "key outside the domain");
+    if (bad & 2u) return HashFail(h, CKMPM_ERR_INVALID, "duplicate key");
+    if (bad & 4u) return HashFail(h, CKMPM_ERR_CAPACITY, "hash stash overflow (too many keys for the table)");
+    return CKMPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ckmpm_hash_create(const ckmpm_domain_desc* domain, uint32_t max_keys, ckmpm_hash** out) {
+    if (!domain || !out || max_keys == 0) return CKMPM_ERR_INVALID;
+    *out = nullptr;
+    const uint64_t nb = static_cast<uint64_t>(domain->dim[0]) * domain->dim[1] * domain->dim[2];
+    if (domain->dim[0] <= 0 || domain->dim[1] <= 0 || domain->dim[2] <= 0 || nb >= 0xFFFFFFFFull)
+        return CKMPM_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CKMPM_ERR_NO_DEVICE;
+    ckmpm_hash* h = new ckmpm_hash();
+    for (int a = 0; a < 3; ++a) h->dim[a] = domain->dim[a];
+    h->nblocks = static_cast<uint32_t>(nb);
+    h->max_keys = max_keys;
+    // load factor at most 0.7: three rounds then leave almost nothing for the stash
+    h->buckets = static_cast<uint32_t>((static_cast<uint64_t>(max_keys) * 10 + 7 * kHashBucket - 1) / (7 * kHashBucket));
+    h->stash_cap = 256 + max_keys / 64;
+    std::mt19937 rng(2);  // the reference's seeds (gpu_hash_table.cuh:303-309)
+    for (int r = 0; r < kHashRounds; ++r) h->seed[r] = static_cast<uint32_t>(rng());
+    bool ok = hipStreamCreate(&h->stream) == hipSuccess;
+    ok = ok && hipMalloc(&h->slots, sizeof(uint64_t) * h->buckets * kHashBucket) == hipSuccess;
+    ok = ok && hipMalloc(&h->stash, sizeof(uint64_t) * h->stash_cap) == hipSuccess;
+    ok = ok && hipMalloc(&h->nstash, sizeof(uint32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&h->bad, sizeof(uint32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&h->fill, sizeof(uint32_t) * h->buckets) == hipSuccess;
+    ok = ok && hipMalloc(&h->tmp, sizeof(uint32_t) * TmpWords(max_keys)) == hipSuccess;
+    ok = ok && hipMalloc(&h->dev_io, sizeof(uint32_t) * 2 * static_cast<size_t>(max_keys)) == hipSuccess;
+    if (!ok) {
+        ckmpm_hash_destroy(h);
+        return CKMPM_ERR_HIP;
+    }
+    EnqueueBuild(h, nullptr, nullptr, 0);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        ckmpm_hash_destroy(h);
+        return CKMPM_ERR_HIP;
+    }
+    *out = h;
+    return CKMPM_OK;
+}
+
+int ckmpm_hash_build_device(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n) {
+    if (!h || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
+    if (n > h->max_keys) return HashFail(h, CKMPM_ERR_CAPACITY, "more keys than the table was created for");
+    EnqueueBuild(h, keys, values, n);
+    return CheckBuild(h);
+}
+
+int ckmpm_hash_build(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n) {
+    if (!h || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
+    if (n > h->max_keys) return HashFail(h, CKMPM_ERR_CAPACITY, "more keys than the table was created for");
+    uint32_t* dk = h->dev_io;
+    uint32_t* dv = h->dev_io + h->max_keys;
+    if (n) {
+        HASH_HIP(h, hipMemcpyAsync(dk, keys, sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
+        HASH_HIP(h, hipMemcpyAsync(dv, values, sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
+    }
+    EnqueueBuild(h, dk, dv, n);
+    return CheckBuild(h);
+}
+
+int ckmpm_hash_find_device(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values, int32_t cooperative) {
+    if (!h || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
+    if (n == 0) return CKMPM_OK;
+    const HashView v = View(h);
+    if (cooperative) HashFindKernel<<<LaunchBlocks(n), kHashThreads, 0, h->stream>>>(v, keys, n, values);
+    else HashFindScalarKernel<<<LaunchBlocks(n), kHashThreads, 0, h->stream>>>(v, keys, n, values);
+    HASH_HIP(h, hipGetLastError());
+    return CKMPM_OK;
+}
+
+int ckmpm_hash_find(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values) {
+    if (!h || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
+    for (uint32_t done = 0; done < n;) {
+        const uint32_t m = std::min(n - done, h->max_keys);
+        uint32_t* dk = h->dev_io;
+        uint32_t* dv = h->dev_io + h->max_keys;
+        HASH_HIP(h, hipMemcpyAsync(dk, keys + done, sizeof(uint32_t) * m, hipMemcpyHostToDevice, h->stream));
+        const int rc = ckmpm_hash_find_device(h, dk, m, dv, 1);
+        if (rc != CKMPM_OK) return rc;
+        HASH_HIP(h, hipMemcpyAsync(values + done, dv, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, h->stream));
+        HASH_HIP(h, hipStreamSynchronize(h->stream));
+        done += m;
+    }
+    return CKMPM_OK;
+}
+
+int ckmpm_hash_synchronize(ckmpm_hash* h) {
+    if (!h) return CKMPM_ERR_INVALID;
+    HASH_HIP(h, hipStreamSynchronize(h->stream));
+    return CKMPM_OK;
+}
+
+int ckmpm_hash_info(ckmpm_hash* h, ckmpm_hash_info_t* info) {
+    if (!h || !info) return CKMPM_ERR_INVALID;
+    std::memset(info, 0, sizeof(*info));
+    info->buckets = h->buckets;
+    info->slots_per_bucket = kHashBucket;
+    info->stash_capacity = h->stash_cap;
+    info->keys = h->n;
+    for (int r = 0; r < kHashRounds; ++r) info->seeds[r] = h->seed[r];
+    uint32_t* hist = h->tmp;  // 17 words of scratch
+    HASH_HIP(h, hipMemsetAsync(hist, 0, sizeof(uint32_t) * (kHashBucket + 1), h->stream));
+    HashHistKernel<<<LaunchBlocks(h->buckets), kHashThreads, 0, h->stream>>>(h->slots, h->buckets, hist);
+    HASH_HIP(h, hipMemcpyAsync(info->fill_histogram, hist, sizeof(uint32_t) * (kHashBucket + 1),
+                               hipMemcpyDeviceToHost, h->stream));
+    HASH_HIP(h, hipMemcpyAsync(&info->stash, h->nstash, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HASH_HIP(h, hipStreamSynchronize(h->stream));
+    for (int f = 0; f <= kHashBucket; ++f) info->occupied_slots += f * info->fill_histogram[f];
+    return CKMPM_OK;
+}
+
+int ckmpm_hash_export(ckmpm_hash* h, uint64_t* slots, uint64_t slots_capacity, uint64_t* stash,
+                      uint32_t stash_capacity, uint32_t* nstash) {
+    if (!h || !slots || !nstash) return CKMPM_ERR_INVALID;
+    const uint64_t ns = static_cast<uint64_t>(h->buckets) * kHashBucket;
+    if (slots_capacity < ns) return HashFail(h, CKMPM_ERR_CAPACITY, "slot buffer too small");
+    HASH_HIP(h, hipMemcpyAsync(slots, h->slots, sizeof(uint64_t) * ns, hipMemcpyDeviceToHost, h->stream));
+    uint32_t cnt = 0;
+    HASH_HIP(h, hipMemcpyAsync(&cnt, h->nstash, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HASH_HIP(h, hipStreamSynchronize(h->stream));
+    *nstash = cnt;
+    if (cnt) {
+        if (!stash || stash_capacity < cnt) return HashFail(h, CKMPM_ERR_CAPACITY, "stash buffer too small");
+        HASH_HIP(h, hipMemcpy(stash, h->stash, sizeof(uint64_t) * cnt, hipMemcpyDeviceToHost));
+    }
+    return CKMPM_OK;
+}
+
+int ckmpm_hash_destroy(ckmpm_hash* h) {
+    if (!h) return CKMPM_ERR_INVALID;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : {static_cast<void*>(h->slots), static_cast<void*>(h->stash), static_cast<void*>(h->nstash),
+                    static_cast<void*>(h->bad), static_cast<void*>(h->fill), static_cast<void*>(h->tmp),
+                    static_cast<void*>(h->dev_io)})
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CKMPM_OK;
+}
+
+const char* ckmpm_hash_last_error(const ckmpm_hash* h) { return h ? h->err.c_str() : "null table"; }
+
+}  // extern "C"

@@ -16,14 +16,30 @@ size_t ScanTempWords(size_t n_cap);
 void ExclusiveScan(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_imm, uint32_t n_cap,
                    uint32_t* total, bool popcount, uint32_t* tmp, hipStream_t s);
 
+// rank[i] = i - (first index of the run of equal values containing i) for a sorted
+// array; t0..t2 are n-word temporaries, scan_tmp holds ScanTempWords(n) words.
+void SegmentRanks(const uint32_t* sorted, uint32_t n, uint32_t* rank, uint32_t* t0, uint32_t* t1, uint32_t* t2,
+                  uint32_t* scan_tmp, hipStream_t s);
+uint32_t LaunchBlocks(uint64_t n);  // grid size for a grid-stride launch of n items
+
 // ---------------------------------------------------------------- setup
 // Stable LSD radix sort of (key, value) pairs on the low key_bits bits.
 size_t RadixTempWords(size_t n);
 void RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int key_bits,
                     uint32_t* tmp, hipStream_t s);
 
-void LaunchSetupKeys(const DomainDev& d, const float* xyz, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* bitmap,
-                     Status* st, hipStream_t s);
+// Setup ordering of one model: particles sorted by (block key, rank among the earlier
+// input particles of the same binning cell, cell within the block).  Writes the sorted
+// block keys and the input index of each processing position; marks occupied blocks
+// in the bitmap.  key_bits + 6 <= 32.
+struct SetupTemps {
+    uint32_t* t[4];       // n words each
+    uint32_t* keys_alt;   // n
+    uint32_t* vals_alt;   // n
+    uint32_t* radix;      // max(RadixTempWords(n), ScanTempWords(n))
+};
+void SetupOrder(const DomainDev& d, const float* xyz, uint32_t n, int key_bits, uint32_t* keys, uint32_t* order,
+                const SetupTemps& t, uint32_t* bitmap, Status* st, hipStream_t s);
 void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t* akeys, const LayoutCounts* lc,
                         uint32_t a_cap, uint32_t* off, hipStream_t s);
 // order: in = setup sort permutation (processing position -> input index),

@@ -204,9 +204,47 @@ __global__ void SetupKeysKernel(DomainDev d, const float* xyz, uint32_t n, uint3
         }
         if (out) SetFlag(st, kFlagOutOfDomain);
         const uint32_t key = KeyOf(d, b[0], b[1], b[2]);
-        keys[p] = key;
+        // binning cell inside the block, from the unclamped cell: ((c - 2) mod 4) per axis
+        uint32_t cb = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            cb = cb * 4u + static_cast<uint32_t>((Cell0(xyz[3 * static_cast<size_t>(p) + a], d.inv_dx) - 2) & 3);
+        keys[p] = (key << 6) | cb;
         vals[p] = p;
         atomicOr(&bitmap[key >> 5], 1u << (key & 31u));
+    }
+}
+
+// Setup ordering inside a block: (rank of the particle among the earlier input
+// particles of its binning cell, cell).  A block's particles then cycle through its
+// cells, so the lanes of a wavefront deposit into distinct grid nodes in P2G.
+// Computed from the cell-key-sorted list: heads of equal-key runs, their positions,
+// then the rank = position - run start.
+__global__ void CellHeadsKernel(const uint32_t* ck, uint32_t n, uint32_t* head) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        head[i] = (i == 0 || ck[i] != ck[i - 1]) ? 1u : 0u;
+}
+__global__ void CellStartsKernel(const uint32_t* head, const uint32_t* ex, uint32_t n, uint32_t* start) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (head[i]) start[ex[i]] = i;
+}
+// in place: head -> within-block sort key (rank << 6 | cell), ex -> block key
+__global__ void CellRankKernel(const uint32_t* ck, uint32_t* head_k2, uint32_t* ex_bk, const uint32_t* start,
+                               uint32_t n, uint32_t* idx) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t seg = ex_bk[i] + head_k2[i] - 1u;
+        const uint32_t rank = min(i - start[seg], kSetupRankClamp);
+        head_k2[i] = (rank << 6) | (ck[i] & 63u);
+        ex_bk[i] = ck[i] >> 6;
+        idx[i] = i;
+    }
+}
+__global__ void SetupGatherKernel(const uint32_t* idx, const uint32_t* bk, const uint32_t* orig, uint32_t n,
+                                  uint32_t* keys, uint32_t* order) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = idx[j];
+        keys[j] = bk[i];
+        order[j] = orig[i];
     }
 }
 
@@ -280,6 +318,26 @@ __device__ __forceinline__ void MakeStencil(const float (&xs)[3], Stencil32& s) 
         s.c[a] = c;
         s.t[a] = t;
         s.w0[a] = 1.0f - t + sinpif(2.0f * t) * kInv2Pi;
+    }
+}
+// Both sub-grid stencils from one sine per axis.  With x/dx < 2^22 every step is exact
+// in FP32: grid 1 has t1 = t0 - 1/2 (same base cell) when t0 >= 1/2, else t1 = t0 + 1/2
+// (base cell one lower), which equals MakeStencil<1>'s c and t bit for bit; and
+// sin(2 pi t1) = -sin(2 pi t0).
+__device__ __forceinline__ void MakeStencilPair(const float (&xs)[3], Stencil32& s0, Stencil32& s1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int c = static_cast<int>(floorf(__fadd_rn(xs[a], 0.25f)));
+        const float t = __fadd_rn(xs[a] - static_cast<float>(c), 0.25f);
+        const float sn = sinpif(2.0f * t) * kInv2Pi;
+        const bool hi = t >= 0.5f;
+        s0.c[a] = c;
+        s0.t[a] = t;
+        s0.w0[a] = 1.0f - t + sn;
+        const float t1 = hi ? t - 0.5f : t + 0.5f;
+        s1.c[a] = hi ? c : c - 1;
+        s1.t[a] = t1;
+        s1.w0[a] = 1.0f - t1 - sn;
     }
 }
 __device__ __forceinline__ float NodeWeight(const Stencil32& s, int dx, int dy, int dz) {
@@ -475,7 +533,15 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-    if (LaneId() == 0 && vmax > 0.0f) atomicMax(&a.st->vmax2_bits, __float_as_uint(vmax));
+    // one atomic per workgroup: the per-wave atomics on one word serialise
+    __shared__ float sMax[kThreads / 64];
+    if (LaneId() == 0) sMax[threadIdx.x >> 6] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < kThreads / 64; ++w) m = fmaxf(m, sMax[w]);
+        if (m > 0.0f) atomicMax(&a.st->vmax2_bits, __float_as_uint(m));
+    }
 }
 
 // ============================================================== G2P2G
@@ -487,10 +553,17 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
 // particles with the same move, for the counting-sort rebin.
 constexpr int kWin = 512;  // 8^3 nodes of one sub-grid window
 
+// P2G accumulator in LDS.  gfx950 executes ds_add_f32 lane by lane (about 190 cycles
+// per wavefront instruction whatever the addresses) but ds_add_f64 at about 20, so
+// the deposit accumulates in FP64 (which also makes the LDS sum nearly independent
+// of the order of the atomics) and the flush rounds each node once to FP32.
+using LdsAcc = double;
+__device__ __forceinline__ void LdsAdd(LdsAcc* p, float v) { atomicAdd(p, static_cast<LdsAcc>(v)); }
+
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
     __shared__ float sV[kVChannels * 2 * kWin];
-    __shared__ float sM[kMChannels * 2 * kWin];
+    __shared__ LdsAcc sM[kMChannels * 2 * kWin];
     __shared__ uint32_t sHalo[8];
     __shared__ uint32_t sWave[kThreads / 64][32];
     __shared__ uint32_t sRun[32];
@@ -501,13 +574,10 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
     const float dx = a.dom.dx, inv_dx = a.dom.inv_dx;
     const size_t N = a.n;
 
-    // XCD-aware mapping: workgroup w runs on XCD w % 8; each XCD walks a contiguous
-    // eighth of the block list so that neighbouring blocks share its L2.
-    const uint32_t nx = 8u, xcd = blockIdx.x % nx, per = gridDim.x / nx;
-    const uint32_t lo = static_cast<uint32_t>((static_cast<uint64_t>(A) * xcd) / nx);
-    const uint32_t hi = static_cast<uint32_t>((static_cast<uint64_t>(A) * (xcd + 1)) / nx);
-
-    for (uint32_t j = lo + blockIdx.x / nx; j < hi; j += per) {
+    // Round-robin over the block list: particle counts per block vary by orders of
+    // magnitude along the list (a model occupies only part of it), so contiguous
+    // per-XCD ranges leave most XCDs idle; interleaving balances them.
+    for (uint32_t j = blockIdx.x; j < A; j += gridDim.x) {
         const uint32_t beg = a.off[j], end = a.off[j + 1];
         if (beg == end) {
             // no particle of this model here: only the (zero) move counts are written
@@ -524,7 +594,7 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
             sHalo[tid] = bi;
         }
         if (tid < 32) sRun[tid] = 0;
-        for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) sM[i] = 0.0f;
+        for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) sM[i] = 0;
         __syncthreads();
         // velocity halo: 8 blocks x 3 channels x 128 nodes
         for (int i = tid; i < 8 * kVFloats; i += kThreads) {
@@ -582,11 +652,11 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
                 float v[3] = {0.f, 0.f, 0.f}, B[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 bool bad = false;
+                Stencil32 sp[2];
+                MakeStencilPair(xs, sp[0], sp[1]);
 #pragma unroll
                 for (int g = 0; g < 2; ++g) {
-                    Stencil32 sg;
-                    if (g == 0) MakeStencil<0>(xs, sg);
-                    else MakeStencil<1>(xs, sg);
+                    const Stencil32& sg = sp[g];
                     int l[3];
                     float m1[3];
 #pragma unroll
@@ -603,20 +673,33 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
                     D[1] += m1[0] * m1[1];
                     D[2] += m1[0] * m1[2];
                     D[5] += m1[1] * m1[2];
+                    // B = sum_i v_i (x_i - x_p)^T w_i with x_i - x_p = (o - t) dx per axis:
+                    // B_rc = dx (S_rc - vg_r t_c), S_rc = sum over the nodes with o_c = 1 of w v_r
                     const float* base = sV + g * kWin + l[0] * 64 + l[1] * 8 + l[2];
+                    float vg[3] = {0.f, 0.f, 0.f}, S[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    float wxy[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        wxy[q] = ((q >> 1) ? 1.0f - sg.w0[0] : sg.w0[0]) * ((q & 1) ? 1.0f - sg.w0[1] : sg.w0[1]);
 #pragma unroll
                     for (int o = 0; o < 8; ++o) {
                         const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
-                        const float w = NodeWeight(sg, ox, oy, oz);
+                        const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
                         const int off = ox * 64 + oy * 8 + oz;
-                        const float vi[3] = {base[off], base[2 * kWin + off], base[4 * kWin + off]};
-                        const float wx[3] = {w * (ox - sg.t[0]) * dx, w * (oy - sg.t[1]) * dx, w * (oz - sg.t[2]) * dx};
+                        const float wv[3] = {w * base[off], w * base[2 * kWin + off], w * base[4 * kWin + off]};
 #pragma unroll
                         for (int r = 0; r < 3; ++r) {
-                            v[r] += w * vi[r];
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) B[r * 3 + c] += vi[r] * wx[c];
+                            vg[r] += wv[r];
+                            if (ox) S[r * 3 + 0] += wv[r];
+                            if (oy) S[r * 3 + 1] += wv[r];
+                            if (oz) S[r * 3 + 2] += wv[r];
                         }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        v[r] += vg[r];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) B[r * 3 + c] += dx * (S[r * 3 + c] - vg[r] * sg.t[c]);
                     }
                 }
                 if (bad) SetFlag(a.st, kFlagOutOfWindow);
@@ -684,8 +767,7 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
                 Stencil32 sn[2];
-                MakeStencil<0>(xs, sn[0]);
-                MakeStencil<1>(xs, sn[1]);
+                MakeStencilPair(xs, sn[0], sn[1]);
                 bool inwin = true;
                 float Dn[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -724,21 +806,38 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
 #pragma unroll
                     for (int k = 0; k < 9; ++k) K[k] = C[k] * a.mat.mass - sv * DT[k];
                     const float mv[3] = {a.mat.mass * v[0], a.mat.mass * v[1], a.mat.mass * v[2]};
+                    // node momentum m v + K (x_i - x_p), x_i - x_p = (o - t) dx:
+                    // = a + sum_c o_c kc_c with a = m v - dx K t and kc_c = dx K e_c
+                    float kc[3][3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) kc[c][r] = dx * K[r * 3 + c];
 #pragma unroll
                     for (int g = 0; g < 2; ++g) {
                         const Stencil32& sg = sn[g];
-                        float* base = sM + g * kWin + sg.c[0] * 64 + sg.c[1] * 8 + sg.c[2];
+                        LdsAcc* base = sM + g * kWin + sg.c[0] * 64 + sg.c[1] * 8 + sg.c[2];
+                        float pa[3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+                            pa[r] = mv[r] - (kc[0][r] * sg.t[0] + kc[1][r] * sg.t[1] + kc[2][r] * sg.t[2]);
+                        float wxy[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            wxy[q] = ((q >> 1) ? 1.0f - sg.w0[0] : sg.w0[0]) * ((q & 1) ? 1.0f - sg.w0[1] : sg.w0[1]);
 #pragma unroll
                         for (int o = 0; o < 8; ++o) {
                             const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
-                            const float w = NodeWeight(sg, ox, oy, oz);
-                            const float xx[3] = {(ox - sg.t[0]) * dx, (oy - sg.t[1]) * dx, (oz - sg.t[2]) * dx};
+                            const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
                             const int off = ox * 64 + oy * 8 + oz;
-                            atomicAdd(base + off, w * a.mat.mass);
+                            LdsAdd(base + off, w * a.mat.mass);
 #pragma unroll
                             for (int r = 0; r < 3; ++r) {
-                                const float kx = K[r * 3 + 0] * xx[0] + K[r * 3 + 1] * xx[1] + K[r * 3 + 2] * xx[2];
-                                atomicAdd(base + (2 * (r + 1)) * kWin + off, w * (mv[r] + kx));
+                                float pr = pa[r];
+                                if (ox) pr += kc[0][r];
+                                if (oy) pr += kc[1][r];
+                                if (oz) pr += kc[2][r];
+                                LdsAdd(base + (2 * (r + 1)) * kWin + off, w * pr);
                             }
                         }
                     }
@@ -766,7 +865,7 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
         }
         // ---------------- flush: one global atomic per non-zero node and channel
         for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) {
-            const float val = sM[i];
+            const float val = static_cast<float>(sM[i]);
             if (val == 0.0f) continue;
             const int ch = i / (2 * kWin), g = (i / kWin) & 1, l = i & (kWin - 1);
             const int lx = l >> 6, ly = (l >> 3) & 7, lz = l & 7;
@@ -888,6 +987,13 @@ __global__ void FinishOffsetsKernel(uint32_t* off, const LayoutCounts* lc, uint3
     if (threadIdx.x == 0 && blockIdx.x == 0) off[lc->A] = n;
 }
 
+// rank of each element inside its run of equal keys (from run heads and their positions)
+__global__ void RunRankKernel(const uint32_t* head, const uint32_t* ex, const uint32_t* start, uint32_t n,
+                              uint32_t* rank) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        rank[i] = i - start[ex[i] + head[i] - 1u];
+}
+
 uint32_t Blocks(uint64_t n, uint32_t threads, uint32_t maxb = 4096) {
     uint64_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -898,6 +1004,7 @@ uint32_t Blocks(uint64_t n, uint32_t threads, uint32_t maxb = 4096) {
 }  // namespace
 
 // ============================================================== launchers
+uint32_t LaunchBlocks(uint64_t n) { return Blocks(n, kThreads); }
 size_t ScanTempWords(size_t n_cap) { return (n_cap + kScanTile - 1) / kScanTile + 1; }
 
 void ExclusiveScan(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_imm, uint32_t n_cap,
@@ -939,9 +1046,32 @@ void RadixSortPairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t
     }
 }
 
-void LaunchSetupKeys(const DomainDev& d, const float* xyz, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* bitmap,
-                     Status* st, hipStream_t s) {
-    SetupKeysKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, xyz, n, keys, vals, bitmap, st);
+void SegmentRanks(const uint32_t* sorted, uint32_t n, uint32_t* rank, uint32_t* t0, uint32_t* t1, uint32_t* t2,
+                  uint32_t* scan_tmp, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t g = Blocks(n, kThreads);
+    CellHeadsKernel<<<g, kThreads, 0, s>>>(sorted, n, t0);
+    ExclusiveScan(t0, t1, nullptr, n, n, nullptr, false, scan_tmp, s);
+    CellStartsKernel<<<g, kThreads, 0, s>>>(t0, t1, n, t2);
+    RunRankKernel<<<g, kThreads, 0, s>>>(t0, t1, t2, n, rank);
+}
+
+void SetupOrder(const DomainDev& d, const float* xyz, uint32_t n, int key_bits, uint32_t* keys, uint32_t* order,
+                const SetupTemps& t, uint32_t* bitmap, Status* st, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t g = Blocks(n, kThreads);
+    // cell keys (block << 6 | cell) in input order, stably sorted: runs of one cell
+    SetupKeysKernel<<<g, kThreads, 0, s>>>(d, xyz, n, keys, t.t[0], bitmap, st);
+    RadixSortPairs(keys, t.t[0], t.keys_alt, t.vals_alt, n, key_bits + 6, t.radix, s);
+    CellHeadsKernel<<<g, kThreads, 0, s>>>(keys, n, t.t[1]);
+    ExclusiveScan(t.t[1], t.t[2], nullptr, n, n, nullptr, false, t.radix, s);
+    CellStartsKernel<<<g, kThreads, 0, s>>>(t.t[1], t.t[2], n, t.t[3]);
+    CellRankKernel<<<g, kThreads, 0, s>>>(keys, t.t[1], t.t[2], t.t[3], n, order);
+    // LSD: by (rank, cell), then stably by block
+    RadixSortPairs(t.t[1], order, t.keys_alt, t.vals_alt, n, 32, t.radix, s);
+    SetupGatherKernel<<<g, kThreads, 0, s>>>(order, t.t[2], t.t[0], n, keys, t.t[3]);
+    RadixSortPairs(keys, t.t[3], t.keys_alt, t.vals_alt, n, key_bits, t.radix, s);
+    (void)hipMemcpyAsync(order, t.t[3], sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
 }
 void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t* akeys, const LayoutCounts* lc,
                         uint32_t a_cap, uint32_t* off, hipStream_t s) {

@@ -29,6 +29,9 @@ EXPORTS = [
     "ckmpm_export_layout", "ckmpm_export_ordering", "ckmpm_export_state", "ckmpm_import_state",
     "ckmpm_export_grid_m", "ckmpm_export_grid_v", "ckmpm_set_profiling", "ckmpm_get_phase_times",
     "ckmpm_destroy", "ckmpm_last_error",
+    "ckmpm_hash_create", "ckmpm_hash_build", "ckmpm_hash_build_device", "ckmpm_hash_find",
+    "ckmpm_hash_find_device", "ckmpm_hash_synchronize", "ckmpm_hash_info", "ckmpm_hash_export",
+    "ckmpm_hash_destroy", "ckmpm_hash_last_error",
 ]
 
 
@@ -46,6 +49,12 @@ class MaterialDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("lam", ctypes.c_float), ("mu", ctypes.c_float), ("bulk", ctypes.c_float),
                 ("gamma", ctypes.c_float), ("viscosity", ctypes.c_float), ("J0", ctypes.c_float),
                 ("friction_angle_deg", ctypes.c_float)]
+
+
+class HashInfo(ctypes.Structure):
+    _fields_ = [("buckets", ctypes.c_uint32), ("slots_per_bucket", ctypes.c_uint32),
+                ("stash_capacity", ctypes.c_uint32), ("keys", ctypes.c_uint32), ("occupied_slots", ctypes.c_uint32),
+                ("stash", ctypes.c_uint32), ("seeds", ctypes.c_uint32 * 3), ("fill_histogram", ctypes.c_uint32 * 17)]
 
 
 class Counters(ctypes.Structure):
@@ -67,6 +76,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libckmpm.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    # One HIP runtime per process: torch bundles its own libamdhip64.so.7 (same soname
+    # as /opt/rocm's).  Whichever is loaded first serves both; torch only initialises
+    # with its own, so let it load first when it is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     L.ckmpm_create.argtypes = [ctypes.POINTER(DomainDesc), ctypes.POINTER(SimDesc), ctypes.POINTER(_vp)]
     L.ckmpm_add_model.argtypes = [_vp, ctypes.POINTER(MaterialDesc), ctypes.c_float, ctypes.c_float, _f32p,
@@ -90,8 +106,20 @@ def load():
     L.ckmpm_destroy.argtypes = [_vp]
     L.ckmpm_last_error.argtypes = [_vp]
     L.ckmpm_last_error.restype = ctypes.c_char_p
+    _u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.ckmpm_hash_create.argtypes = [ctypes.POINTER(DomainDesc), ctypes.c_uint32, ctypes.POINTER(_vp)]
+    L.ckmpm_hash_build.argtypes = [_vp, _u32p, _u32p, ctypes.c_uint32]
+    L.ckmpm_hash_build_device.argtypes = [_vp, _vp, _vp, ctypes.c_uint32]
+    L.ckmpm_hash_find.argtypes = [_vp, _u32p, ctypes.c_uint32, _u32p]
+    L.ckmpm_hash_find_device.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_int32]
+    L.ckmpm_hash_synchronize.argtypes = [_vp]
+    L.ckmpm_hash_info.argtypes = [_vp, ctypes.POINTER(HashInfo)]
+    L.ckmpm_hash_export.argtypes = [_vp, _u64p, ctypes.c_uint64, _u64p, ctypes.c_uint32, _u32p]
+    L.ckmpm_hash_destroy.argtypes = [_vp]
+    L.ckmpm_hash_last_error.argtypes = [_vp]
+    L.ckmpm_hash_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name != "ckmpm_last_error":
+        if not name.endswith("last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -159,6 +159,43 @@ int ckmpm_get_phase_times(ckmpm_engine* eng, double ms[3], uint64_t* substeps);
 int ckmpm_destroy(ckmpm_engine* eng);
 const char* ckmpm_last_error(const ckmpm_engine* eng);
 
+/* ---------------------------------------------------------------- sparse block table
+ * Replaces BucketedCuckooHashTable<Pair, 16, MurmurHash3_32> (gpu_hash_table.cuh:274-552):
+ * linear block key -> uint32 value.  Same hash (MurmurHash3_x86_32 of the int3 block
+ * coordinate, :39-118), same seeds (std::mt19937(2), :303-309), same bucket choice
+ * (hash % buckets) and Find protocol (:463-513).  The build is deterministic: the table
+ * contents are a function of the (key, value) set alone (the reference's atomic cuckoo
+ * insert, :378-461, is not).  Keys must be unique and inside the domain. */
+typedef struct ckmpm_hash ckmpm_hash;
+typedef struct {
+    uint32_t buckets;            /* ceil(max_keys / (16 * 0.7)) */
+    uint32_t slots_per_bucket;   /* 16 */
+    uint32_t stash_capacity;
+    uint32_t keys;               /* pairs of the last build */
+    uint32_t occupied_slots;     /* pairs in buckets */
+    uint32_t stash;              /* pairs in the stash */
+    uint32_t seeds[3];
+    uint32_t fill_histogram[17]; /* buckets holding 0..16 pairs */
+} ckmpm_hash_info_t;
+/* Allocate (gpu_hash_table.cuh:343-354) for at most max_keys pairs of the domain's keys */
+int ckmpm_hash_create(const ckmpm_domain_desc* domain, uint32_t max_keys, ckmpm_hash** out);
+/* Build from host arrays (replaces a pass of CooperativeInsert, gpu_hash_table.cuh:378-461) */
+int ckmpm_hash_build(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n);
+/* Same from device arrays; the table's stream is synchronised before return */
+int ckmpm_hash_build_device(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n);
+/* Find (CooperativeFind, gpu_hash_table.cuh:463-513) of host keys; missing -> 0xFFFFFFFF */
+int ckmpm_hash_find(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values);
+/* Same on device arrays, enqueued on the table's stream (no sync).  cooperative = 1: a
+ * 16-lane quarter wavefront per probe (the reference's tile of 16); 0: one thread per key */
+int ckmpm_hash_find_device(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values, int32_t cooperative);
+int ckmpm_hash_synchronize(ckmpm_hash* h);
+int ckmpm_hash_info(ckmpm_hash* h, ckmpm_hash_info_t* info);
+/* Raw table: buckets*16 slot words (key << 32 | value, empty = ~0) and the stash */
+int ckmpm_hash_export(ckmpm_hash* h, uint64_t* slots, uint64_t slots_capacity, uint64_t* stash,
+                      uint32_t stash_capacity, uint32_t* nstash);
+int ckmpm_hash_destroy(ckmpm_hash* h);
+const char* ckmpm_hash_last_error(const ckmpm_hash* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <random>
+#include <unordered_map>
 #include <vector>
 
 #ifdef _OPENMP
@@ -832,10 +834,11 @@ int orc_add_model(orc_engine* h, int kind, const double* params, double mass, do
 uint32_t orc_initial_setup(orc_engine* h) {
     Engine* e = reinterpret_cast<Engine*>(h);
     std::vector<uint32_t> keys;
-    std::vector<std::vector<uint32_t>> pkey(e->models.size());
+    std::vector<std::vector<uint32_t>> pkey(e->models.size()), pcell(e->models.size());
     for (size_t mi = 0; mi < e->models.size(); ++mi) {
         Model& m = e->models[mi];
         pkey[mi].resize(m.n);
+        pcell[mi].resize(m.n);
         for (uint32_t p = 0; p < m.n; ++p) {
             int b[3], c[3];
             BlockOfF(&m.x0[3 * static_cast<size_t>(p)], e->inv_dx_f, b, c);
@@ -844,6 +847,8 @@ uint32_t orc_initial_setup(orc_engine* h) {
                 for (int a = 0; a < 3; ++a) b[a] = std::max(0, std::min(e->dom.dim[a] - 1, b[a]));
             }
             pkey[mi][p] = KeyOf(e->dom, b[0], b[1], b[2]);
+            // binning cell inside the block from the unclamped cell, (c - 2) mod 4 per axis
+            pcell[mi][p] = static_cast<uint32_t>(((c[0] - 2) & 3) * 16 + ((c[1] - 2) & 3) * 4 + ((c[2] - 2) & 3));
             keys.push_back(pkey[mi][p]);
         }
     }
@@ -857,9 +862,21 @@ uint32_t orc_initial_setup(orc_engine* h) {
         for (uint32_t p = 0; p < m.n; ++p) cnt[e->cur.table[pkey[mi][p]]]++;
         m.off.assign(A + 1, 0);
         for (uint32_t j = 0; j < A; ++j) m.off[j + 1] = m.off[j] + cnt[j];
-        std::vector<uint32_t> fill(m.off.begin(), m.off.end() - 1);
+        // Setup order (this engine's canonical choice; the reference's is atomic): by
+        // block, then by the particle's rank among the earlier input particles of its
+        // binning cell (clamped to 2^26 - 1), then by the cell; ties keep input order.
+        std::unordered_map<uint64_t, uint32_t> seen;
+        std::vector<uint64_t> sk(m.n);
+        for (uint32_t p = 0; p < m.n; ++p) {
+            const uint64_t ck = (static_cast<uint64_t>(pkey[mi][p]) << 6) | pcell[mi][p];
+            const uint32_t rank = std::min<uint32_t>(seen[ck]++, (1u << 26) - 1u);
+            sk[p] = (static_cast<uint64_t>(e->cur.table[pkey[mi][p]]) << 32) | (static_cast<uint64_t>(rank) << 6) |
+                    pcell[mi][p];
+        }
         m.setup_order.resize(m.n);
-        for (uint32_t p = 0; p < m.n; ++p) m.setup_order[fill[e->cur.table[pkey[mi][p]]]++] = p;
+        for (uint32_t p = 0; p < m.n; ++p) m.setup_order[p] = p;
+        std::stable_sort(m.setup_order.begin(), m.setup_order.end(),
+                         [&](uint32_t a, uint32_t b) { return sk[a] < sk[b]; });
         // state in processing order; the ordering becomes the identity with no move
         // (CopyParticleArrayToParticleBuffer + InitializeAdvectionBucket,
         // mpm_algorithm.cuh:949-1081)
@@ -1063,6 +1080,124 @@ uint32_t orc_build_layout(orc_engine* h, const uint32_t* akeys, uint32_t A, uint
     out_counts[1] = L.Nb;
     out_counts[2] = L.Ex;
     return L.Ex;
+}
+
+// ------------------------------------------------------------ sparse block table
+// MurmurHash3_x86_32 of a byte string (Appleby's published algorithm; the reference
+// hashes its 12-byte int3 keys with it, gpu_hash_table.cuh:39-118).
+uint32_t orc_murmur3_32(const uint8_t* data, uint32_t len, uint32_t seed) {
+    auto rotl = [](uint32_t x, int r) { return (x << r) | (x >> (32 - r)); };
+    const uint32_t c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+    uint32_t h = seed;
+    const uint32_t nblocks = len / 4;
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        uint32_t k = static_cast<uint32_t>(data[4 * i]) | static_cast<uint32_t>(data[4 * i + 1]) << 8 |
+                     static_cast<uint32_t>(data[4 * i + 2]) << 16 | static_cast<uint32_t>(data[4 * i + 3]) << 24;
+        k *= c1;
+        k = rotl(k, 15);
+        k *= c2;
+        h ^= k;
+        h = rotl(h, 13);
+        h = h * 5u + 0xe6546b64u;
+    }
+    const uint8_t* tail = data + 4 * nblocks;
+    uint32_t k = 0;
+    switch (len & 3u) {
+        case 3: k ^= static_cast<uint32_t>(tail[2]) << 16; [[fallthrough]];
+        case 2: k ^= static_cast<uint32_t>(tail[1]) << 8; [[fallthrough]];
+        case 1:
+            k ^= tail[0];
+            k *= c1;
+            k = rotl(k, 15);
+            k *= c2;
+            h ^= k;
+    }
+    h ^= len;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// The reference's three seeds: first draws of std::mt19937(2) (gpu_hash_table.cuh:303-309).
+void orc_hash_seeds(uint32_t out[3]) {
+    std::mt19937 rng(2);
+    for (int r = 0; r < 3; ++r) out[r] = static_cast<uint32_t>(rng());
+}
+
+// Bucket count for at most max_keys pairs: load factor <= 0.7 with 16-slot buckets.
+uint32_t orc_hash_buckets(uint32_t max_keys) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(max_keys) * 10 + 7 * 16 - 1) / (7 * 16));
+}
+
+static uint32_t HashBucketOf(const int dim[3], uint32_t buckets, uint32_t seed, uint32_t key) {
+    const int32_t c[3] = {static_cast<int32_t>(key / (static_cast<uint32_t>(dim[1]) * dim[2])),
+                          static_cast<int32_t>((key / dim[2]) % dim[1]), static_cast<int32_t>(key % dim[2])};
+    uint8_t b[12];
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < 4; ++i) b[4 * a + i] = static_cast<uint8_t>(static_cast<uint32_t>(c[a]) >> (8 * i));
+    return orc_murmur3_32(b, 12, seed) % buckets;
+}
+
+// Deterministic build: pairs in ascending key order; in round r every unplaced pair
+// asks for bucket h_r; the pairs asking for one bucket take its free slots in key
+// order; leftovers after three rounds go to the stash in key order.  Returns the
+// number of stash pairs (also when it exceeds stash_cap; only stash_cap are written).
+uint32_t orc_hash_build(const int dim[3], uint32_t buckets, const uint32_t seeds[3], const uint32_t* keys,
+                        const uint32_t* vals, uint32_t n, uint64_t* slots, uint64_t* stash, uint32_t stash_cap) {
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    for (uint64_t i = 0; i < static_cast<uint64_t>(buckets) * 16; ++i) slots[i] = ~0ull;
+    std::vector<uint32_t> fill(buckets, 0);
+    std::vector<uint32_t> rest = order;
+    for (int r = 0; r < 3; ++r) {
+        std::vector<uint32_t> asked(buckets, 0), next;
+        for (uint32_t p : rest) {  // ascending key order
+            const uint32_t b = HashBucketOf(dim, buckets, seeds[r], keys[p]);
+            const uint32_t rank = asked[b]++;
+            if (rank < 16 - fill[b]) slots[static_cast<uint64_t>(b) * 16 + fill[b] + rank] =
+                (static_cast<uint64_t>(keys[p]) << 32) | vals[p];
+            else next.push_back(p);
+        }
+        for (uint32_t b = 0; b < buckets; ++b) fill[b] = std::min<uint32_t>(16, fill[b] + asked[b]);
+        rest.swap(next);
+    }
+    for (uint32_t i = 0; i < rest.size() && i < stash_cap; ++i)
+        stash[i] = (static_cast<uint64_t>(keys[rest[i]]) << 32) | vals[rest[i]];
+    return static_cast<uint32_t>(rest.size());
+}
+
+// Find with the reference's protocol: probe h0, h1, h2; stop at a bucket that is not
+// full; then the stash.  Missing keys give 0xFFFFFFFF.
+void orc_hash_find(const int dim[3], uint32_t buckets, const uint32_t seeds[3], const uint64_t* slots,
+                   const uint64_t* stash, uint32_t nstash, const uint32_t* q, uint32_t nq, uint32_t* out) {
+    for (uint32_t i = 0; i < nq; ++i) {
+        uint32_t res = 0xFFFFFFFFu;
+        bool done = false;
+        for (int r = 0; r < 3 && !done; ++r) {
+            const uint64_t* b = slots + static_cast<uint64_t>(HashBucketOf(dim, buckets, seeds[r], q[i])) * 16;
+            bool full = true;
+            for (int j = 0; j < 16; ++j) {
+                if (b[j] == ~0ull) full = false;
+                else if (static_cast<uint32_t>(b[j] >> 32) == q[i]) {
+                    res = static_cast<uint32_t>(b[j]);
+                    done = true;
+                    break;
+                }
+            }
+            if (!full) done = true;
+        }
+        if (!done)
+            for (uint32_t j = 0; j < nstash; ++j)
+                if (static_cast<uint32_t>(stash[j] >> 32) == q[i]) {
+                    res = static_cast<uint32_t>(stash[j]);
+                    break;
+                }
+        out[i] = res;
+    }
 }
 
 }  // extern "C"

@@ -97,6 +97,18 @@ def lib():
         L.orc_bin_f32.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_uint32, _i32p, _i32p]
         L.orc_build_layout.restype = ctypes.c_uint32
         L.orc_build_layout.argtypes = [ctypes.c_void_p, _u32p, ctypes.c_uint32, _u32p, _u32p]
+        _u8p = ctypes.POINTER(ctypes.c_uint8)
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.orc_murmur3_32.restype = ctypes.c_uint32
+        L.orc_murmur3_32.argtypes = [_u8p, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_hash_seeds.argtypes = [_u32p]
+        L.orc_hash_buckets.restype = ctypes.c_uint32
+        L.orc_hash_buckets.argtypes = [ctypes.c_uint32]
+        L.orc_hash_build.restype = ctypes.c_uint32
+        L.orc_hash_build.argtypes = [_i32p, ctypes.c_uint32, _u32p, _u32p, _u32p, ctypes.c_uint32, _u64p, _u64p,
+                                     ctypes.c_uint32]
+        L.orc_hash_find.argtypes = [_i32p, ctypes.c_uint32, _u32p, _u64p, _u64p, ctypes.c_uint32, _u32p,
+                                    ctypes.c_uint32, _u32p]
         _lib = L
     return _lib
 
@@ -285,3 +297,47 @@ class Oracle:
         cnt = np.zeros(3, np.uint32)
         ex = lib().orc_build_layout(self.h, _p(a, _u32p), a.shape[0], _p(out, _u32p), _p(cnt, _u32p))
         return out[:ex].copy(), tuple(int(v) for v in cnt)
+
+
+# ----------------------------------------------------------------- sparse block table
+def murmur3_32(data: bytes, seed: int) -> int:
+    """MurmurHash3_x86_32 of a byte string."""
+    b = np.frombuffer(bytes(data), np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+    return int(lib().orc_murmur3_32(_p(b, ctypes.POINTER(ctypes.c_uint8)), len(data), seed & 0xFFFFFFFF))
+
+
+def hash_seeds() -> np.ndarray:
+    out = np.zeros(3, np.uint32)
+    lib().orc_hash_seeds(_p(out, _u32p))
+    return out
+
+
+class HashOracle:
+    """Deterministic three-choice bucketed hash (the oracle side of ckmpm_hash_*)."""
+
+    def __init__(self, dim, max_keys: int):
+        self.dim = np.ascontiguousarray(dim, np.int32)
+        self.buckets = int(lib().orc_hash_buckets(max_keys))
+        self.seeds = hash_seeds()
+        self.stash_cap = 256 + max_keys // 64
+        self.slots = np.full(self.buckets * 16, np.iinfo(np.uint64).max, np.uint64)
+        self.stash = np.zeros(self.stash_cap, np.uint64)
+        self.nstash = 0
+
+    def build(self, keys, values) -> int:
+        k = np.ascontiguousarray(keys, np.uint32)
+        v = np.ascontiguousarray(values, np.uint32)
+        u64 = ctypes.POINTER(ctypes.c_uint64)
+        ns = lib().orc_hash_build(_p(self.dim, _i32p), self.buckets, _p(self.seeds, _u32p), _p(k, _u32p),
+                                  _p(v, _u32p), k.shape[0], _p(self.slots, u64), _p(self.stash, u64), self.stash_cap)
+        self.nstash = int(ns)
+        return self.nstash
+
+    def find(self, keys) -> np.ndarray:
+        q = np.ascontiguousarray(keys, np.uint32)
+        out = np.zeros(q.shape[0], np.uint32)
+        u64 = ctypes.POINTER(ctypes.c_uint64)
+        lib().orc_hash_find(_p(self.dim, _i32p), self.buckets, _p(self.seeds, _u32p), _p(self.slots, u64),
+                            _p(self.stash, u64), min(self.nstash, self.stash_cap), _p(q, _u32p), q.shape[0],
+                            _p(out, _u32p))
+        return out

@@ -10,7 +10,7 @@
   transposed matrix breaks it);
 * stress direction: a compressed block pushes its grid momentum outward (a sign error
   in the stress term pulls it inward);
-* rebin: an independent numpy restatement of the canonical ordering.
+* rebin and setup: independent numpy restatements of the canonical orderings.
 """
 import numpy as np
 import pytest
@@ -205,3 +205,34 @@ def test_rebin_matches_numpy_restatement():
             crossed += int(((perm >> 27) != 13).sum())
     assert crossed > 100
     assert o.flags == 0
+
+
+def test_setup_order_matches_numpy_restatement():
+    """Setup order = sort by (block, rank of the particle among the earlier input
+    particles of its binning cell, cell within the block)."""
+    sc = scenes.mixed_dam_sand(n_fluid=9000, n_sand=5000, scale=8)
+    o = Oracle.from_scene(sc)
+    o.setup()
+    dim = sc.dim
+    inv_dx = np.float32(1.0 / sc.dx)
+    for mi, m in enumerate(sc.models):
+        x = m.x.astype(np.float32)
+        cell = np.floor((x * inv_dx).astype(np.float32) + np.float32(0.25)).astype(np.int64)
+        b = np.floor_divide(cell - 2, 4)
+        assert (b >= 0).all() and (b < np.array(dim)).all()
+        key = (b[:, 0] * dim[1] + b[:, 1]) * dim[2] + b[:, 2]
+        cb = ((cell[:, 0] - 2) % 4) * 16 + ((cell[:, 1] - 2) % 4) * 4 + (cell[:, 2] - 2) % 4
+        ck = key * 64 + cb
+        # rank among earlier particles with the same cell key
+        o1 = np.argsort(ck, kind="stable")
+        sk = ck[o1]
+        start = np.r_[0, np.flatnonzero(np.diff(sk)) + 1]
+        run = np.repeat(start, np.diff(np.r_[start, sk.size]))
+        rank = np.empty_like(ck)
+        rank[o1] = np.arange(sk.size) - run
+        expected = np.lexsort((cb, rank, key))
+        got = o.setup_order(mi)
+        np.testing.assert_array_equal(got, expected)
+        # consecutive particles of a block sit in distinct cells while the block has
+        # unused cells left for that rank
+        assert (np.diff(ck[got]) != 0).mean() > 0.95
