@@ -52,6 +52,17 @@ def cpu_baseline(workload: str, seconds: float = 12.0):
             "sample": f"{sample}; {n} substeps of the FP64 oracle in {el:.1f} s on one host core"}
 
 
+def _measured_traffic(path: str, workload: str):
+    """HBM bytes per G2P2G launch pair for this workload, as measured with rocprofv3
+    FETCH_SIZE + WRITE_SIZE and scaled by a plain copy's reading (None if absent)."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get(workload)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +70,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="mixed_dam_sand")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB three times (HBM counter calibration)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="measured HBM bytes per G2P2G launch (from rocprofv3 FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU sharding of the "
@@ -70,6 +85,15 @@ def main():
 
     from ckmpm_amd import scenes
     from ckmpm_amd.engine import engine_from_scene
+
+    if args.copy_calibration_mib > 0:
+        src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 3 x {args.copy_calibration_mib} MiB", file=sys.stderr, flush=True)
+        del src, dst
 
     t = time.perf_counter()
     sc = scenes.SCENES[args.workload]()
@@ -126,6 +150,10 @@ def main():
                                  "rebin": ms_rebin / max(nsub, 1)},
         "blocks": {"active": c1.active_blocks, "neighbor": c1.neighbor_blocks, "exterior": c1.exterior_blocks},
     }
+    traffic = _measured_traffic(args.traffic, sc.name)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch_pair"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
