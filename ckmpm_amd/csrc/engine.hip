@@ -650,3 +650,271 @@ int ckmpm_destroy(ckmpm_engine* e) {
 const char* ckmpm_last_error(const ckmpm_engine* e) { return e ? e->err.c_str() : "null engine"; }
 
 }  // extern "C"
+
+// ============================================================== batched scenes
+// Many independent engines advanced together.  Each engine keeps its own stream; the
+// graph mode captures two substeps of every engine (the buffer roles ping-pong every
+// substep, so two substeps return each engine to the roles it was captured in) as one
+// hipGraph forked over the engine streams, and replays it: one launch per two
+// substeps of the whole batch instead of ~30 launches per engine per substep.
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> roles;         // engine buffer roles at capture
+    std::vector<uint32_t> hints;    // block-count hints the launch sizes were chosen for
+};
+
+struct ckmpm_batch {
+    std::vector<ckmpm_engine*> eng;
+    hipStream_t cs = nullptr;  // capture and replay stream
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> join;
+    // the engines step together, so their buffer roles take one of two configurations
+    // (even / odd substep count): one graph each
+    BatchGraph g[2];
+    // mode 3: one graph per engine and parity, replayed on the engine's own stream
+    std::vector<BatchGraph> pe[2];
+    uint64_t captures = 0;
+    std::string err;
+};
+
+namespace {
+
+int BatchFail(ckmpm_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BATCH_HIP(b, call)                                                                                   \
+    do {                                                                                                     \
+        hipError_t _r = (call);                                                                              \
+        if (_r != hipSuccess) return BatchFail((b), CKMPM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_r)); \
+    } while (0)
+
+int EngineRoles(const ckmpm_engine* e) { return e->cur | (e->sc << 1) | (e->mprev << 2); }
+
+void EagerSubstep(ckmpm_engine* e) {
+    const double dt = e->substeps == 0 ? 0.0 : e->dt_default;
+    PhaseGrid(e, dt);
+    PhaseParticles(e, dt, e->dt_default);
+}
+
+void ReleaseGraph(BatchGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
+
+int Parity(const ckmpm_batch* b) { return static_cast<int>(b->eng[0]->substeps & 1u); }
+
+bool GraphCurrent(const ckmpm_batch* b, const BatchGraph& g) {
+    if (!g.exec) return false;
+    for (size_t i = 0; i < b->eng.size(); ++i) {
+        const ckmpm_engine* e = b->eng[i];
+        if (EngineRoles(e) != g.roles[i]) return false;
+        // launch sizes follow the block count; recapture once it drifts by more than 1/4
+        const uint32_t h = g.hints[i], a = e->a_hint;
+        if (a > h + h / 4 + 8 || a + a / 4 + 8 < h) return false;
+    }
+    return true;
+}
+
+int Capture(ckmpm_batch* b, BatchGraph& g) {
+    ReleaseGraph(g);
+    g.roles.assign(b->eng.size(), -1);
+    g.hints.assign(b->eng.size(), 0);
+    std::vector<bool> prof(b->eng.size());
+    BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
+    (void)hipEventRecord(b->fork, b->cs);
+    for (size_t i = 0; i < b->eng.size(); ++i) {
+        ckmpm_engine* e = b->eng[i];
+        prof[i] = e->profile;
+        e->profile = false;
+        (void)hipStreamWaitEvent(e->stream, b->fork, 0);
+        g.roles[i] = EngineRoles(e);
+        g.hints[i] = e->a_hint;
+        for (int k = 0; k < 2; ++k) {
+            PhaseGrid(e, e->dt_default);
+            PhaseParticles(e, e->dt_default, e->dt_default);
+        }
+        e->substeps -= 2;  // recorded, not run
+        (void)hipEventRecord(b->join[i], e->stream);
+        (void)hipStreamWaitEvent(b->cs, b->join[i], 0);
+    }
+    const hipError_t rc = hipStreamEndCapture(b->cs, &g.graph);
+    for (size_t i = 0; i < b->eng.size(); ++i) b->eng[i]->profile = prof[i];
+    if (rc != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(rc));
+    BATCH_HIP(b, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    ++b->captures;
+    return CKMPM_OK;
+}
+
+// Two substeps of one engine captured on its own stream.
+int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
+    ReleaseGraph(g);
+    g.roles.assign(1, EngineRoles(e));
+    g.hints.assign(1, e->a_hint);
+    const bool prof = e->profile;
+    e->profile = false;
+    BATCH_HIP(b, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    for (int k = 0; k < 2; ++k) {
+        PhaseGrid(e, e->dt_default);
+        PhaseParticles(e, e->dt_default, e->dt_default);
+    }
+    e->substeps -= 2;
+    const hipError_t rc = hipStreamEndCapture(e->stream, &g.graph);
+    e->profile = prof;
+    if (rc != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(rc));
+    BATCH_HIP(b, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    ++b->captures;
+    return CKMPM_OK;
+}
+
+bool EngineGraphCurrent(const ckmpm_engine* e, const BatchGraph& g) {
+    if (!g.exec || EngineRoles(e) != g.roles[0]) return false;
+    const uint32_t h = g.hints[0], a = e->a_hint;
+    return !(a > h + h / 4 + 8 || a + a / 4 + 8 < h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch** out) {
+    if (!engines || !out || count == 0) return CKMPM_ERR_INVALID;
+    *out = nullptr;
+    for (uint32_t i = 0; i < count; ++i)
+        if (!engines[i] || !engines[i]->setup_done) return CKMPM_ERR_INVALID;
+    ckmpm_batch* b = new ckmpm_batch();
+    b->eng.assign(engines, engines + count);
+    b->join.resize(count, nullptr);
+    bool ok = hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    for (uint32_t i = 0; ok && i < count; ++i) ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        ckmpm_batch_destroy(b);
+        return CKMPM_ERR_HIP;
+    }
+    *out = b;
+    return CKMPM_OK;
+}
+
+int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode) {
+    if (!b || mode < 0 || mode > 3) return CKMPM_ERR_INVALID;
+    if (mode == 3) {
+        uint32_t left = substeps;
+        bool first = false;
+        for (ckmpm_engine* e : b->eng) first |= e->substeps == 0;
+        if (first && left) {
+            for (ckmpm_engine* e : b->eng) EagerSubstep(e);
+            --left;
+        }
+        const uint32_t pairs = left / 2;
+        if (pairs) {
+            const int par = Parity(b);
+            b->pe[par].resize(b->eng.size());
+            for (size_t i = 0; i < b->eng.size(); ++i) {
+                ckmpm_engine* e = b->eng[i];
+                if (!EngineGraphCurrent(e, b->pe[par][i])) {
+                    BATCH_HIP(b, hipStreamSynchronize(e->stream));
+                    const int rc = CaptureEngine(b, e, b->pe[par][i]);
+                    if (rc != CKMPM_OK) return rc;
+                }
+            }
+            for (uint32_t p = 0; p < pairs; ++p)
+                for (size_t i = 0; i < b->eng.size(); ++i)
+                    BATCH_HIP(b, hipGraphLaunch(b->pe[par][i].exec, b->eng[i]->stream));
+            for (ckmpm_engine* e : b->eng) e->substeps += 2ull * pairs;
+        }
+        if (left & 1u)
+            for (ckmpm_engine* e : b->eng) EagerSubstep(e);
+        const hipError_t r = hipGetLastError();
+        if (r != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
+        return CKMPM_OK;
+    }
+    if (mode == 0) {
+        // one scene after the other, each with plain per-substep launches
+        for (ckmpm_engine* e : b->eng) {
+            for (uint32_t k = 0; k < substeps; ++k) EagerSubstep(e);
+            const int rc = ckmpm_synchronize(e);
+            if (rc != CKMPM_OK) return BatchFail(b, rc, "engine: " + e->err);
+        }
+        return CKMPM_OK;
+    }
+    if (mode == 1) {
+        for (uint32_t k = 0; k < substeps; ++k)
+            for (ckmpm_engine* e : b->eng) EagerSubstep(e);
+        return CKMPM_OK;
+    }
+    // graph mode: a pending first substep (dt = 0) and an odd remainder run eagerly
+    uint32_t left = substeps;
+    bool first = false;
+    for (ckmpm_engine* e : b->eng) first |= e->substeps == 0;
+    if (first && left) {
+        // every engine advances one substep (dt = 0 for those that had not started)
+        for (ckmpm_engine* e : b->eng) EagerSubstep(e);
+        --left;
+    }
+    const uint32_t pairs = left / 2;
+    if (pairs) {
+        BatchGraph& g = b->g[Parity(b)];
+        if (!GraphCurrent(b, g)) {
+            for (ckmpm_engine* e : b->eng) BATCH_HIP(b, hipStreamSynchronize(e->stream));
+            const int rc = Capture(b, g);
+            if (rc != CKMPM_OK) return rc;
+        }
+        // the replay stream runs after everything already queued on the engine streams
+        for (size_t i = 0; i < b->eng.size(); ++i) {
+            BATCH_HIP(b, hipEventRecord(b->join[i], b->eng[i]->stream));
+            BATCH_HIP(b, hipStreamWaitEvent(b->cs, b->join[i], 0));
+        }
+        for (uint32_t p = 0; p < pairs; ++p) BATCH_HIP(b, hipGraphLaunch(g.exec, b->cs));
+        for (ckmpm_engine* e : b->eng) e->substeps += 2ull * pairs;
+        // later work on an engine stream waits for the replays
+        BATCH_HIP(b, hipEventRecord(b->done, b->cs));
+        for (ckmpm_engine* e : b->eng) BATCH_HIP(b, hipStreamWaitEvent(e->stream, b->done, 0));
+    }
+    if (left & 1u)
+        for (ckmpm_engine* e : b->eng) EagerSubstep(e);
+    const hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
+    return CKMPM_OK;
+}
+
+int ckmpm_batch_synchronize(ckmpm_batch* b) {
+    if (!b) return CKMPM_ERR_INVALID;
+    BATCH_HIP(b, hipStreamSynchronize(b->cs));
+    for (ckmpm_engine* e : b->eng) {
+        const int rc = ckmpm_synchronize(e);
+        if (rc != CKMPM_OK) return BatchFail(b, rc, "engine: " + e->err);
+    }
+    return CKMPM_OK;
+}
+
+int ckmpm_batch_captures(const ckmpm_batch* b, uint64_t* captures) {
+    if (!b || !captures) return CKMPM_ERR_INVALID;
+    *captures = b->captures;
+    return CKMPM_OK;
+}
+
+int ckmpm_batch_destroy(ckmpm_batch* b) {
+    if (!b) return CKMPM_ERR_INVALID;
+    if (b->cs) (void)hipStreamSynchronize(b->cs);
+    ReleaseGraph(b->g[0]);
+    ReleaseGraph(b->g[1]);
+    for (int p = 0; p < 2; ++p)
+        for (BatchGraph& g : b->pe[p]) ReleaseGraph(g);
+    for (hipEvent_t ev : b->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->cs) (void)hipStreamDestroy(b->cs);
+    delete b;
+    return CKMPM_OK;
+}
+
+const char* ckmpm_batch_last_error(const ckmpm_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+}  // extern "C"

@@ -222,6 +222,57 @@ class MPMEngine:
         return out
 
 
+class MPMBatch:
+    """Independent engines advanced together (ckmpm_batch_*).
+
+    mode 0 runs the scenes one after another with plain launches (the sequential
+    baseline), mode 1 interleaves them on their own streams, mode 2 replays a hipGraph
+    of two substeps of every engine.  The engines stay owned by the caller."""
+
+    SEQUENTIAL, STREAMS, GRAPH, ENGINE_GRAPHS = 0, 1, 2, 3
+
+    def __init__(self, engines: Sequence[MPMEngine]):
+        self._lib = native.load()
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        self._h = ctypes.c_void_p()
+        rc = self._lib.ckmpm_batch_create(arr, len(self.engines), ctypes.byref(self._h))
+        if rc != native.OK:
+            raise CkmpmError(rc, "ckmpm_batch_create", "")
+
+    def _check(self, rc: int, where: str):
+        if rc != native.OK:
+            msg = self._lib.ckmpm_batch_last_error(self._h)
+            raise CkmpmError(rc, where, msg.decode() if msg else "")
+
+    def Run(self, substeps: int, mode: int = 2):
+        self._check(self._lib.ckmpm_batch_run(self._h, int(substeps), int(mode)), "ckmpm_batch_run")
+
+    def Synchronize(self):
+        self._check(self._lib.ckmpm_batch_synchronize(self._h), "ckmpm_batch_synchronize")
+
+    @property
+    def captures(self) -> int:
+        c = ctypes.c_uint64()
+        self._check(self._lib.ckmpm_batch_captures(self._h, ctypes.byref(c)), "ckmpm_batch_captures")
+        return int(c.value)
+
+    @property
+    def n_particles(self) -> int:
+        return int(sum(e.n(m) for e in self.engines for m in range(len(e.models))))
+
+    def close(self):
+        if self._h:
+            self._lib.ckmpm_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def engine_from_scene(scene: Scene, capacity: Optional[int] = None) -> MPMEngine:
     """Build and set up an engine for a synthetic scene."""
     eng = MPMEngine()

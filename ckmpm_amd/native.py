@@ -32,6 +32,8 @@ EXPORTS = [
     "ckmpm_hash_create", "ckmpm_hash_build", "ckmpm_hash_build_device", "ckmpm_hash_find",
     "ckmpm_hash_find_device", "ckmpm_hash_synchronize", "ckmpm_hash_info", "ckmpm_hash_export",
     "ckmpm_hash_destroy", "ckmpm_hash_last_error",
+    "ckmpm_batch_create", "ckmpm_batch_run", "ckmpm_batch_synchronize", "ckmpm_batch_captures",
+    "ckmpm_batch_destroy", "ckmpm_batch_last_error",
 ]
 
 
@@ -116,6 +118,13 @@ def load():
     L.ckmpm_hash_info.argtypes = [_vp, ctypes.POINTER(HashInfo)]
     L.ckmpm_hash_export.argtypes = [_vp, _u64p, ctypes.c_uint64, _u64p, ctypes.c_uint32, _u32p]
     L.ckmpm_hash_destroy.argtypes = [_vp]
+    L.ckmpm_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(_vp)]
+    L.ckmpm_batch_run.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32]
+    L.ckmpm_batch_synchronize.argtypes = [_vp]
+    L.ckmpm_batch_captures.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.ckmpm_batch_destroy.argtypes = [_vp]
+    L.ckmpm_batch_last_error.argtypes = [_vp]
+    L.ckmpm_batch_last_error.restype = ctypes.c_char_p
     L.ckmpm_hash_last_error.argtypes = [_vp]
     L.ckmpm_hash_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

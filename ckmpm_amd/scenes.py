@@ -16,6 +16,9 @@ Workloads (sizes as stated by the benchmark configurations):
   x-extent is 0.25 of the domain plus a 3,000,000-particle Drucker-Prager sand cone,
   dx = 1/512, 128^3 blocks, per-cell counts log-uniform in [4, 16].
 * ``elastic_bar``        - fixed-corotated bar 0.6 x 0.1 x 0.1, dx = 1/128, 8 ppc.
+* ``batch_scenes``       - 64 independent scenes: grids 64^3 / 128^3 / 256^3 cells, particle
+  counts log-uniform in [20k, 500k], material uniform over {FC, NH, DP, fluid}, a box or a
+  sphere at 8 ppc, one velocity per scene ~ N(0, 0.5 m/s) per component.
 """
 from __future__ import annotations
 
@@ -226,6 +229,53 @@ def elastic_bar(seed: int = 5) -> Scene:
     v = np.zeros((1, 3), np.float32)
     return Scene("elastic_bar", (32, 32, 32), (0, 0, 0), dx, 0.0, 1e-4, 300,
                  [ModelSpec(Material(FIXED_COROTATED, lam, mu), rho * vol, vol, x, v)])
+
+
+def _body_cells(shape: str, n_target: int, g: int) -> np.ndarray:
+    """Cells of a centred box or sphere holding about n_target / 8 cells, kept inside
+    the middle 70% of a g^3 grid (clear of the 8-cell sticky band)."""
+    cells = n_target / 8.0
+    limit = int(0.7 * g)
+    c = g / 2.0
+    if shape == "box":
+        side = min(max(2, int(round(cells ** (1.0 / 3.0)))), limit)
+        lo = int(c - side / 2)
+        r = np.arange(lo, lo + side)
+        return np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    rad = min(max(1.5, (3.0 * cells / (4.0 * math.pi)) ** (1.0 / 3.0)), limit / 2.0)
+    r = np.arange(int(c - rad) - 1, int(c + rad) + 2)
+    gx = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    keep = np.linalg.norm(gx + 0.5 - c, axis=1) <= rad
+    return gx[keep]
+
+
+def batch_scenes(count: int = 64, seed: int = 4, dt: float = 5e-5) -> List[Scene]:
+    rho = 1000.0
+    out = []
+    for i in range(count):
+        rng = Splitmix(seed * 1_000_003 + i)
+        u = rng.uniform(5)
+        g = (64, 128, 256)[min(2, int(u[0] * 3))]
+        n_target = int(round(math.exp(math.log(20_000) + u[1] * (math.log(500_000) - math.log(20_000)))))
+        kind = min(3, int(u[2] * 4))
+        shape = "box" if u[3] < 0.5 else "sphere"
+        dx = 1.0 / g
+        cells = _body_cells(shape, n_target, g)
+        x = _stratified_cells(rng, cells.astype(np.float64), dx).astype(np.float32)
+        v = (0.5 * rng.normal(3)).astype(np.float32).reshape(1, 3)
+        vol = dx ** 3 / 8
+        if kind == FLUID:
+            mat = Material(FLUID, bulk=1e5, gamma=7.0, viscosity=0.1, J0=1.0)
+        elif kind == DRUCKER_PRAGER:
+            lam, mu = lame(3.5e5, 0.3)
+            mat = Material(DRUCKER_PRAGER, lam, mu, friction_deg=30.0)
+        else:
+            lam, mu = lame(5e4, 0.3)
+            mat = Material(kind, lam, mu)
+        nb = g // 4
+        out.append(Scene(f"batch_{i:02d}", (nb, nb, nb), (0, 0, 0), dx, -9.8, dt, 200,
+                         [ModelSpec(mat, rho * vol, vol, x, v)]))
+    return out
 
 
 SCENES = {

@@ -159,6 +159,26 @@ int ckmpm_get_phase_times(ckmpm_engine* eng, double ms[3], uint64_t* substeps);
 int ckmpm_destroy(ckmpm_engine* eng);
 const char* ckmpm_last_error(const ckmpm_engine* eng);
 
+/* ---------------------------------------------------------------- batched scenes
+ * No reference counterpart (the reference engine is a singleton, mpm_engine.cuh:80-84):
+ * many independent engines advanced together.  The batch does not own the engines;
+ * each must have run ckmpm_initial_setup.  Every engine uses its own sim.dt (an engine
+ * that has not run a substep yet starts with dt = 0, as mpm_engine.cuh:479 does). */
+typedef struct ckmpm_batch ckmpm_batch;
+int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch** out);
+/* mode 0: one engine after the other, plain launches, synchronised per engine (the
+ *         sequential single-scene baseline);
+ * mode 1: substep k of every engine enqueued on the engines' own streams, no sync;
+ * mode 2: as 1, but pairs of substeps replay one hipGraph captured over all engine
+ *         streams (recaptured when an engine's block count drifts by more than 1/4);
+ * mode 3: pairs of substeps replay one hipGraph per engine on the engine's own stream. */
+int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode);
+/* waits for every engine; returns the first engine error */
+int ckmpm_batch_synchronize(ckmpm_batch* b);
+int ckmpm_batch_captures(const ckmpm_batch* b, uint64_t* captures);
+int ckmpm_batch_destroy(ckmpm_batch* b);
+const char* ckmpm_batch_last_error(const ckmpm_batch* b);
+
 /* ---------------------------------------------------------------- sparse block table
  * Replaces BucketedCuckooHashTable<Pair, 16, MurmurHash3_32> (gpu_hash_table.cuh:274-552):
  * linear block key -> uint32 value.  Same hash (MurmurHash3_x86_32 of the int3 block

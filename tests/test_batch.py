@@ -1,0 +1,91 @@
+"""Batched scenes (ckmpm_batch_*).
+
+The graph replay must give each engine the same result as stepping it alone with
+ckmpm_substep: same block lists and orderings (integer, bit-exact), same particle
+state up to the order of the floating-point atomics (the only nondeterminism).
+"""
+import numpy as np
+import pytest
+
+from ckmpm_amd import scenes
+
+
+def _small_batch():
+    # four scenes, one per material, at 64^3 and 128^3
+    out = []
+    for i, sc in enumerate(scenes.batch_scenes(count=24, seed=7)):
+        if sc.models[0].material.kind == len(out) % 4 and sc.n_particles < 120_000:
+            out.append(sc)
+        if len(out) == 4:
+            break
+    assert len(out) == 4
+    return out
+
+
+def test_batch_scenes_are_seeded_and_within_spec():
+    a = scenes.batch_scenes(count=8, seed=4)
+    b = scenes.batch_scenes(count=8, seed=4)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.models[0].x, y.models[0].x)
+    for sc in scenes.batch_scenes():
+        g = sc.dim[0] * 4
+        assert g in (64, 128, 256)
+        x = sc.models[0].x
+        assert 15_000 <= x.shape[0] <= 510_000
+        cell = np.floor(x / sc.dx)
+        assert cell.min() >= 8 and cell.max() < g - 8   # clear of the sticky band
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("substeps", [7, 10])
+@pytest.mark.parametrize("mode", [2, 3])
+def test_graph_batch_matches_separate_engines(substeps, mode):
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+
+    scs = _small_batch()
+    alone = [engine_from_scene(sc) for sc in scs]
+    for e, sc in zip(alone, scs):
+        for k in range(substeps):
+            e.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        e.Synchronize()
+    together = [engine_from_scene(sc) for sc in scs]
+    batch = MPMBatch(together)
+    batch.Run(substeps, mode)
+    batch.Synchronize()
+    per_capture = 1 if mode == MPMBatch.GRAPH else len(scs)
+    assert batch.captures == per_capture
+    for a, b, sc in zip(alone, together, scs):
+        ka, ca = a.layout(0)
+        kb, cb = b.layout(0)
+        assert ca == cb
+        np.testing.assert_array_equal(ka, kb)
+        pa, oa = a.ordering(0)
+        pb, ob = b.ordering(0)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(oa, ob)
+        np.testing.assert_allclose(b.state(0), a.state(0), rtol=0, atol=1e-5)
+    # later runs reuse the graph of their substep parity (one per parity at most)
+    batch.Run(4, mode)
+    batch.Run(5, mode)
+    batch.Run(4, mode)
+    batch.Synchronize()
+    assert batch.captures == 2 * per_capture
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_sequential_and_stream_modes_match_graph():
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+
+    scs = _small_batch()
+    res = {}
+    for mode in (MPMBatch.SEQUENTIAL, MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS):
+        engs = [engine_from_scene(sc) for sc in scs]
+        b = MPMBatch(engs)
+        b.Run(6, mode)
+        b.Synchronize()
+        res[mode] = [e.state(0) for e in engs]
+        b.close()
+    for m in (MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS):
+        for x, y in zip(res[MPMBatch.SEQUENTIAL], res[m]):
+            np.testing.assert_allclose(y, x, rtol=0, atol=1e-5)
