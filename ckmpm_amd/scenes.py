@@ -231,6 +231,31 @@ def elastic_bar(seed: int = 5) -> Scene:
                  [ModelSpec(Material(FIXED_COROTATED, lam, mu), rho * vol, vol, x, v)])
 
 
+def cantilever(seed: int = 6, scale: int = 1, log_e_sigma: float = 0.3) -> Scene:
+    """The differentiation workload: a fixed-corotated bar 0.6 x 0.1 x 0.1 at 8 ppc on a
+    128^3 grid (dx = 1/128), its left end inside the sticky 8-cell boundary band (so it
+    is clamped there and bends under gravity), per-particle Young's modulus with
+    log E ~ N(log 1e4, log_e_sigma), nu = 0.3, per-particle initial velocity (zero).
+    scale > 1 divides the grid and the bar by scale per axis."""
+    g = 128 // scale
+    dx = 1.0 / g
+    rho, nu = 1000.0, 0.3
+    vol = dx ** 3 / 8
+    rng = Splitmix(seed)
+    lo = np.array([int(round(0.03 * g)), int(round(0.45 * g)), int(round(0.45 * g))])
+    ext = np.array([max(8, int(round(0.6 * g))), max(2, int(round(0.1 * g))), max(2, int(round(0.1 * g)))])
+    cells = np.stack(np.meshgrid(*[np.arange(lo[a], lo[a] + ext[a]) for a in range(3)], indexing="ij"), -1).reshape(-1, 3)
+    x = _stratified_cells(rng, cells.astype(np.float64), dx).astype(np.float32)
+    logE = math.log(1e4) + log_e_sigma * rng.normal(x.shape[0])
+    E = np.exp(logE)
+    lam = (E * nu / ((1 + nu) * (1 - 2 * nu))).astype(np.float32)
+    mu = (E / (2 * (1 + nu))).astype(np.float32)
+    lam0, mu0 = lame(1e4, nu)
+    v = np.zeros((x.shape[0], 3), np.float32)
+    m = ModelSpec(Material(FIXED_COROTATED, lam0, mu0), rho * vol, vol, x, v, lam_p=lam, mu_p=mu)
+    return Scene("cantilever", (g // 4, g // 4, g // 4), (0, 0, 0), dx, -9.8, 1e-4, 300, [m])
+
+
 def _body_cells(shape: str, n_target: int, g: int) -> np.ndarray:
     """Cells of a centred box or sphere holding about n_target / 8 cells, kept inside
     the middle 70% of a g^3 grid (clear of the 8-cell sticky band)."""
@@ -283,4 +308,5 @@ SCENES = {
     "jelly_drop": jelly_drop,
     "mixed_dam_sand": mixed_dam_sand,
     "elastic_bar": elastic_bar,
+    "cantilever": cantilever,
 }
