@@ -1,0 +1,76 @@
+"""Differentiable CK-MPM simulation behind torch.autograd.Function.
+
+``simulate(log_E, v0, scene, steps)`` runs `steps` substeps of a one-model fixed-
+corotated (or Neo-Hookean) scene on the GPU with per-particle Young's modulus
+E = exp(log_E) (Poisson ratio fixed) and per-particle initial velocity v0, and returns
+the final positions in input order.  Its backward replays the run from checkpoints
+taken every `segment` substeps, recording each segment's substeps and undoing them with
+the adjoint kernels (ckmpm_adjoint_*), then maps (lambda-bar, mu-bar) to log_E-bar.
+All arithmetic is the library's (FP32 kernels); torch only carries the tensors.
+"""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+import torch
+
+from .engine import engine_from_scene
+from .scenes import Scene
+
+
+def _lame_factors(nu: float):
+    return nu / ((1 + nu) * (1 - 2 * nu)), 1.0 / (2 * (1 + nu))
+
+
+class _Simulate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, log_E, v0, scene: Scene, steps: int, segment: int, nu: float):
+        fl, fm = _lame_factors(nu)
+        E = torch.exp(log_E.detach().double()).cpu().numpy()
+        sc = copy.copy(scene)
+        m = copy.copy(scene.models[0])
+        m.lam_p = (E * fl).astype(np.float32)
+        m.mu_p = (E * fm).astype(np.float32)
+        m.v = v0.detach().cpu().numpy().astype(np.float32).reshape(-1, 3)
+        sc.models = [m]
+        eng = engine_from_scene(sc, adjoint_segment=segment)
+        cks = []
+        for k in range(steps):
+            if k % segment == 0:
+                cks.append(eng.checkpoint())
+            eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        eng.Synchronize()
+        x = eng.positions_input_order(0)
+        ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment = eng, cks, sc, steps, segment
+        ctx.lam, ctx.mu = m.lam_p.astype(np.float64), m.mu_p.astype(np.float64)
+        return torch.from_numpy(x.astype(np.float64)).to(log_E.device)
+
+    @staticmethod
+    def backward(ctx, gx):
+        eng, cks, sc, steps, seg = ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment
+        eng.adjoint_reset()
+        eng.adjoint_seed(0, gx.detach().cpu().numpy().astype(np.float32))
+        for s in reversed(range(len(cks))):
+            eng.restore(cks[s])
+            eng.adjoint_record(True)
+            for k in range(s * seg, min((s + 1) * seg, steps)):
+                eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+            eng.adjoint_record(False)
+            for _ in range(s * seg, min((s + 1) * seg, steps)):
+                eng.adjoint_substep()
+        lb, mb, vb = eng.adjoint_gradients(0)
+        for c in cks:
+            eng.free_checkpoint(c)
+        eng.close()
+        # lambda and mu are both proportional to E: d/dlogE = lambda-bar lambda + mu-bar mu
+        g_logE = torch.from_numpy(lb.astype(np.float64) * ctx.lam + mb.astype(np.float64) * ctx.mu)
+        g_v0 = torch.from_numpy(vb.astype(np.float64))
+        return g_logE.to(gx.device), g_v0.to(gx.device), None, None, None, None
+
+
+def simulate(log_E: torch.Tensor, v0: torch.Tensor, scene: Scene, steps: int, segment: int = 25,
+             nu: float = 0.3) -> torch.Tensor:
+    """Final positions (n, 3, float64, input order) of `steps` substeps of `scene`'s
+    model with Young's modulus exp(log_E) and initial velocities v0 (n, 3)."""
+    return _Simulate.apply(log_E, v0, scene, steps, segment, nu)

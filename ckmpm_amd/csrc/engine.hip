@@ -12,106 +12,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ckmpm.h"
-#include "kernels.h"
+#include "engine_impl.h"
 
 using namespace ckmpm;
 
 namespace {
 
-struct ModelBuf {
-    MaterialDev mat{};
-    int kind = 0;
-    int nattr = 0;
-    uint32_t n = 0;
-    float* st[2] = {nullptr, nullptr};
-    uint32_t* perm[2] = {nullptr, nullptr};
-    uint32_t* off[2] = {nullptr, nullptr};
-    uint32_t* code = nullptr;
-    uint32_t* cnt = nullptr;
-    uint32_t* base = nullptr;
-    uint32_t* cnt_e = nullptr;
-    uint32_t* cntn = nullptr;
-    int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
-    // setup inputs (host), released after the initial setup
-    std::vector<float> x0, v0, lam0, mu0;
-    uint32_t nvel = 0;
-};
 
-struct LayoutBuf {
-    uint32_t* keys = nullptr;
-    uint32_t* table = nullptr;
-    LayoutCounts* lc = nullptr;
-};
-
-int AttrCount(int kind) {
-    return kind == kDruckerPrager ? 13 : (kind == kFluid ? 4 : 12);
-}
-
-}  // namespace
-
-struct ckmpm_engine {
-    DomainDev dom{};
-    ckmpm_sim_desc sim{};
-    uint32_t cap = 0;
-    hipStream_t stream = nullptr;
-    LayoutBuf L[2];
-    int cur = 0;          // current layout; 1 - cur is the previous one
-    float* M[2] = {nullptr, nullptr};
-    int mprev = 0;        // M[mprev] holds the previous layout's mass / momentum
-    float* V = nullptr;
-    Status* status = nullptr;
-    uint32_t* bitmap = nullptr;
-    uint32_t nwords = 0;
-    uint32_t* word_off = nullptr;
-    uint32_t* occ = nullptr;
-    uint32_t* newidx = nullptr;
-    uint32_t* scan_tmp = nullptr;
-    uint32_t* total = nullptr;
-    std::vector<ModelBuf> models;
-    int sc = 0;           // state / ordering buffer index of the current step
-    bool setup_done = false;
-    uint32_t a_hint = 0;  // particle-block count at the last host read
-    uint64_t substeps = 0;
-    double dt_default = 0;
-    double time = 0, cur_dt = 0;
-    std::string err;
-    std::vector<void*> allocs;
-    // phase timing: 4 events per substep (before grid, after grid, after G2P2G, after rebin)
-    bool profile = false;
-    std::vector<hipEvent_t> events;
-    size_t ev_used = 0;
-};
-
-namespace {
-
-int Fail(ckmpm_engine* e, int code, const std::string& msg) {
-    if (e) e->err = msg;
-    return code;
-}
-
-#define CK_HIP(e, call)                                                                                  \
-    do {                                                                                                 \
-        hipError_t _r = (call);                                                                          \
-        if (_r != hipSuccess) return Fail((e), CKMPM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_r)); \
-    } while (0)
-
-template <typename T>
-int Alloc(ckmpm_engine* e, T** p, size_t count) {
-    void* q = nullptr;
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t r = hipMalloc(&q, bytes);
-    if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(r));
-    e->allocs.push_back(q);
-    *p = static_cast<T*>(q);
-    return CKMPM_OK;
-}
-
-#define CK_TRY(x)                   \
-    do {                            \
-        int _rc = (x);              \
-        if (_rc != CKMPM_OK) return _rc; \
-    } while (0)
 
 uint32_t GridFor(uint32_t blocks) {
     uint32_t g = std::max<uint32_t>(blocks, 8u);
@@ -398,6 +305,7 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
         uint32_t* vals = m.perm[0];
         CK_HIP(e, hipMemcpyAsync(xyz, m.x0.data(), 3 * sizeof(float) * m.n, hipMemcpyHostToDevice, s));
         SetupOrder(e->dom, xyz, m.n, key_bits, keys, vals, tmp, e->bitmap, e->status, s);
+        AdjointOnSetupOrder(e, i, vals, xyz);
         if (m.pp_attr) {
             CK_HIP(e, hipMemcpyAsync(lam, m.lam0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
             CK_HIP(e, hipMemcpyAsync(mu, m.mu0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
@@ -451,8 +359,10 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
 
 int ckmpm_substep(ckmpm_engine* e, double dt, double next_dt) {
     CK_TRY(CheckReady(e));
+    CK_TRY(AdjointBeforeSubstep(e, dt, next_dt));
     PhaseGrid(e, dt);
     PhaseParticles(e, dt, next_dt);
+    AdjointAfterSubstep(e);
     const hipError_t r = hipGetLastError();
     if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
     return CKMPM_OK;
@@ -474,7 +384,9 @@ int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
                 return Fail(e, CKMPM_ERR_NONFINITE, "velocity reached infinity");
             double next_dt = std::min(e->dt_default, frame_end - e->time);
             if (vmax > 0.0f) next_dt = std::min(next_dt, double(e->sim.dx) * e->sim.cfl / vmax);
+            CK_TRY(AdjointBeforeSubstep(e, e->cur_dt, next_dt));
             PhaseParticles(e, e->cur_dt, next_dt);
+            AdjointAfterSubstep(e);
             e->cur_dt = next_dt;
             e->time += e->cur_dt;
         }
@@ -640,6 +552,7 @@ int ckmpm_get_phase_times(ckmpm_engine* e, double ms[3], uint64_t* substeps) {
 int ckmpm_destroy(ckmpm_engine* e) {
     if (!e) return CKMPM_ERR_INVALID;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    AdjointDestroy(e);
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -785,7 +698,7 @@ int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch
     if (!engines || !out || count == 0) return CKMPM_ERR_INVALID;
     *out = nullptr;
     for (uint32_t i = 0; i < count; ++i)
-        if (!engines[i] || !engines[i]->setup_done) return CKMPM_ERR_INVALID;
+        if (!engines[i] || !engines[i]->setup_done || engines[i]->adj) return CKMPM_ERR_INVALID;
     ckmpm_batch* b = new ckmpm_batch();
     b->eng.assign(engines, engines + count);
     b->join.resize(count, nullptr);

@@ -1,0 +1,122 @@
+// Internal layout of an engine, shared by engine.hip and adjoint.hip (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ckmpm.h"
+#include "kernels.h"
+
+namespace ckmpm {
+
+struct ModelBuf {
+    MaterialDev mat{};
+    int kind = 0;
+    int nattr = 0;
+    uint32_t n = 0;
+    float* st[2] = {nullptr, nullptr};
+    uint32_t* perm[2] = {nullptr, nullptr};
+    uint32_t* off[2] = {nullptr, nullptr};
+    uint32_t* code = nullptr;
+    uint32_t* cnt = nullptr;
+    uint32_t* base = nullptr;
+    uint32_t* cnt_e = nullptr;
+    uint32_t* cntn = nullptr;
+    int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
+    // setup inputs (host), released after the initial setup
+    std::vector<float> x0, v0, lam0, mu0;
+    uint32_t nvel = 0;
+};
+
+struct LayoutBuf {
+    uint32_t* keys = nullptr;
+    uint32_t* table = nullptr;
+    LayoutCounts* lc = nullptr;
+};
+
+inline int AttrCount(int kind) {
+    return kind == kDruckerPrager ? 13 : (kind == kFluid ? 4 : 12);
+}
+
+
+
+}  // namespace ckmpm
+
+using namespace ckmpm;
+
+struct AdjointState;  // adjoint.hip
+
+struct ckmpm_engine {
+    DomainDev dom{};
+    ckmpm_sim_desc sim{};
+    uint32_t cap = 0;
+    hipStream_t stream = nullptr;
+    LayoutBuf L[2];
+    int cur = 0;          // current layout; 1 - cur is the previous one
+    float* M[2] = {nullptr, nullptr};
+    int mprev = 0;        // M[mprev] holds the previous layout's mass / momentum
+    float* V = nullptr;
+    Status* status = nullptr;
+    uint32_t* bitmap = nullptr;
+    uint32_t nwords = 0;
+    uint32_t* word_off = nullptr;
+    uint32_t* occ = nullptr;
+    uint32_t* newidx = nullptr;
+    uint32_t* scan_tmp = nullptr;
+    uint32_t* total = nullptr;
+    std::vector<ModelBuf> models;
+    int sc = 0;           // state / ordering buffer index of the current step
+    bool setup_done = false;
+    uint32_t a_hint = 0;  // particle-block count at the last host read
+    uint64_t substeps = 0;
+    double dt_default = 0;
+    double time = 0, cur_dt = 0;
+    std::string err;
+    std::vector<void*> allocs;
+    // phase timing: 4 events per substep (before grid, after grid, after G2P2G, after rebin)
+    bool profile = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+    // adjoint support (null unless ckmpm_adjoint_enable was called)
+    AdjointState* adj = nullptr;
+};
+
+namespace ckmpm {
+
+inline int Fail(ckmpm_engine* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+#define CK_HIP(e, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t _r = (call);                                                                          \
+        if (_r != hipSuccess) return Fail((e), CKMPM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_r)); \
+    } while (0)
+
+
+template <typename T>
+inline int Alloc(ckmpm_engine* e, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t r = hipMalloc(&q, bytes);
+    if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(r));
+    e->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return CKMPM_OK;
+}
+
+// adjoint hooks (adjoint.hip); no-ops unless ckmpm_adjoint_enable was called
+void AdjointOnSetupOrder(ckmpm_engine* e, size_t model, const uint32_t* order, const float* xyz);
+int AdjointBeforeSubstep(ckmpm_engine* e, double dt, double nd);
+void AdjointAfterSubstep(ckmpm_engine* e);
+void AdjointDestroy(ckmpm_engine* e);
+
+#define CK_TRY(x)                   \
+    do {                            \
+        int _rc = (x);              \
+        if (_rc != CKMPM_OK) return _rc; \
+    } while (0)
+
+}  // namespace ckmpm

@@ -11,6 +11,7 @@
 //   grid velocity    V[block][ch 3][node 128]
 //   layout           keys[list index] -> linear block key, table[key] -> list index
 #include "kernels.h"
+#include "stencil.h"
 #include "material.h"
 
 namespace ckmpm {
@@ -18,7 +19,6 @@ namespace ckmpm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr float kInv2Pi = 0.15915494309189535f;
 
 __device__ __forceinline__ uint32_t LaneId() { return __lane_id(); }
 
@@ -300,52 +300,6 @@ __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const f
     }
 }
 
-// Stencil of one sub-grid at an FP32 position (xs = x * invDx): base cell c,
-// fractional offset t and node-0 weight w0 per axis.  mpm_algorithm.cuh:1555-1569,
-// mpm_kernel.cuh:91-114 (w0 = 1 - t + sin(2 pi t) / (2 pi)).
-struct Stencil32 {
-    int c[3];
-    float t[3];
-    float w0[3];
-};
-template <int g>
-__device__ __forceinline__ void MakeStencil(const float (&xs)[3], Stencil32& s) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float shift = g == 0 ? 0.25f : -0.25f;
-        const int c = static_cast<int>(floorf(__fadd_rn(xs[a], shift)));
-        const float t = __fadd_rn(xs[a] - static_cast<float>(c), shift);
-        s.c[a] = c;
-        s.t[a] = t;
-        s.w0[a] = 1.0f - t + sinpif(2.0f * t) * kInv2Pi;
-    }
-}
-// Both sub-grid stencils from one sine per axis.  With x/dx < 2^22 every step is exact
-// in FP32: grid 1 has t1 = t0 - 1/2 (same base cell) when t0 >= 1/2, else t1 = t0 + 1/2
-// (base cell one lower), which equals MakeStencil<1>'s c and t bit for bit; and
-// sin(2 pi t1) = -sin(2 pi t0).
-__device__ __forceinline__ void MakeStencilPair(const float (&xs)[3], Stencil32& s0, Stencil32& s1) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int c = static_cast<int>(floorf(__fadd_rn(xs[a], 0.25f)));
-        const float t = __fadd_rn(xs[a] - static_cast<float>(c), 0.25f);
-        const float sn = sinpif(2.0f * t) * kInv2Pi;
-        const bool hi = t >= 0.5f;
-        s0.c[a] = c;
-        s0.t[a] = t;
-        s0.w0[a] = 1.0f - t + sn;
-        const float t1 = hi ? t - 0.5f : t + 0.5f;
-        s1.c[a] = hi ? c : c - 1;
-        s1.t[a] = t1;
-        s1.w0[a] = 1.0f - t1 - sn;
-    }
-}
-__device__ __forceinline__ float NodeWeight(const Stencil32& s, int dx, int dy, int dz) {
-    const float wx = dx ? 1.0f - s.w0[0] : s.w0[0];
-    const float wy = dy ? 1.0f - s.w0[1] : s.w0[1];
-    const float wz = dz ? 1.0f - s.w0[2] : s.w0[2];
-    return wx * wy * wz;
-}
 
 // mpm_algorithm.cuh:871-947 (Rasterize): m w and m w v0 on both grids, global atomics.
 __global__ void RasterizeKernel(DomainDev d, const float* xyz, const float* vel, uint32_t n, uint32_t nvel, float mass,

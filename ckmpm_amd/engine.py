@@ -144,6 +144,50 @@ class MPMEngine:
         except Exception:
             pass
 
+    # -- adjoint ------------------------------------------------------------------
+    def adjoint_enable(self, segment: int):
+        """Before InitialSetup: keep a tape of up to `segment` substeps and particle identities."""
+        self._check(self._lib.ckmpm_adjoint_enable(self._h, int(segment)), "ckmpm_adjoint_enable")
+
+    def checkpoint(self) -> ctypes.c_void_p:
+        c = ctypes.c_void_p()
+        self._check(self._lib.ckmpm_checkpoint_save(self._h, ctypes.byref(c)), "ckmpm_checkpoint_save")
+        return c
+
+    def restore(self, ck: ctypes.c_void_p):
+        self._check(self._lib.ckmpm_checkpoint_restore(self._h, ck), "ckmpm_checkpoint_restore")
+
+    def free_checkpoint(self, ck: ctypes.c_void_p):
+        self._lib.ckmpm_checkpoint_free(ck)
+
+    def adjoint_record(self, on: bool):
+        self._check(self._lib.ckmpm_adjoint_record(self._h, int(on)), "ckmpm_adjoint_record")
+
+    def adjoint_reset(self):
+        self._check(self._lib.ckmpm_adjoint_reset(self._h), "ckmpm_adjoint_reset")
+
+    def adjoint_seed(self, model: int, xbar: np.ndarray):
+        xb = _f32(xbar).reshape(-1)
+        self._check(self._lib.ckmpm_adjoint_seed(self._h, model, _ptr(xb, _f32p)), "ckmpm_adjoint_seed")
+
+    def adjoint_substep(self):
+        self._check(self._lib.ckmpm_adjoint_substep(self._h), "ckmpm_adjoint_substep")
+
+    def adjoint_gradients(self, model: int):
+        n = self.n(model)
+        lb = np.empty(n, np.float32)
+        mb = np.empty(n, np.float32)
+        vb = np.empty((n, 3), np.float32)
+        self._check(self._lib.ckmpm_adjoint_gradients(self._h, model, _ptr(lb, _f32p), _ptr(mb, _f32p),
+                                                      _ptr(vb, _f32p)), "ckmpm_adjoint_gradients")
+        return lb, mb, vb
+
+    def positions_input_order(self, model: int) -> np.ndarray:
+        out = np.empty((self.n(model), 3), np.float32)
+        self._check(self._lib.ckmpm_get_positions_input_order(self._h, model, _ptr(out, _f32p)),
+                    "ckmpm_get_positions_input_order")
+        return out
+
     # -- state access ---------------------------------------------------------------
     def n(self, model: int) -> int:
         n = ctypes.c_uint64()
@@ -273,8 +317,9 @@ class MPMBatch:
             pass
 
 
-def engine_from_scene(scene: Scene, capacity: Optional[int] = None) -> MPMEngine:
-    """Build and set up an engine for a synthetic scene."""
+def engine_from_scene(scene: Scene, capacity: Optional[int] = None, adjoint_segment: int = 0) -> MPMEngine:
+    """Build and set up an engine for a synthetic scene (adjoint_segment > 0: with an
+    adjoint tape of that many substeps)."""
     eng = MPMEngine()
     eng.Initialize(MPMDomain(scene.dim, scene.offset),
                    MPMConfig(dx=scene.dx, gravity=scene.gravity, mass_clamp=scene.mass_clamp, dt=scene.dt,
@@ -285,5 +330,7 @@ def engine_from_scene(scene: Scene, capacity: Optional[int] = None) -> MPMEngine
                           m.material.viscosity, m.material.J0, m.material.friction_deg)
         models.append(MPMModel(m.mass, m.volume, m.x, m.v, mat, m.lam_p, m.mu_p))
     eng.InitializeParticle(models)
+    if adjoint_segment:
+        eng.adjoint_enable(adjoint_segment)
     eng.InitialSetup()
     return eng

@@ -34,6 +34,9 @@ EXPORTS = [
     "ckmpm_hash_destroy", "ckmpm_hash_last_error",
     "ckmpm_batch_create", "ckmpm_batch_run", "ckmpm_batch_synchronize", "ckmpm_batch_captures",
     "ckmpm_batch_destroy", "ckmpm_batch_last_error",
+    "ckmpm_adjoint_enable", "ckmpm_checkpoint_save", "ckmpm_checkpoint_restore", "ckmpm_checkpoint_free",
+    "ckmpm_adjoint_record", "ckmpm_adjoint_reset", "ckmpm_adjoint_seed", "ckmpm_adjoint_substep",
+    "ckmpm_adjoint_gradients", "ckmpm_get_positions_input_order",
 ]
 
 
@@ -125,6 +128,16 @@ def load():
     L.ckmpm_batch_destroy.argtypes = [_vp]
     L.ckmpm_batch_last_error.argtypes = [_vp]
     L.ckmpm_batch_last_error.restype = ctypes.c_char_p
+    L.ckmpm_adjoint_enable.argtypes = [_vp, ctypes.c_uint32]
+    L.ckmpm_checkpoint_save.argtypes = [_vp, ctypes.POINTER(_vp)]
+    L.ckmpm_checkpoint_restore.argtypes = [_vp, _vp]
+    L.ckmpm_checkpoint_free.argtypes = [_vp]
+    L.ckmpm_adjoint_record.argtypes = [_vp, ctypes.c_int32]
+    L.ckmpm_adjoint_reset.argtypes = [_vp]
+    L.ckmpm_adjoint_seed.argtypes = [_vp, ctypes.c_int32, _f32p]
+    L.ckmpm_adjoint_substep.argtypes = [_vp]
+    L.ckmpm_adjoint_gradients.argtypes = [_vp, ctypes.c_int32, _f32p, _f32p, _f32p]
+    L.ckmpm_get_positions_input_order.argtypes = [_vp, ctypes.c_int32, _f32p]
     L.ckmpm_hash_last_error.argtypes = [_vp]
     L.ckmpm_hash_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

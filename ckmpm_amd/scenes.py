@@ -242,7 +242,8 @@ def cantilever(seed: int = 6, scale: int = 1, log_e_sigma: float = 0.3) -> Scene
     rho, nu = 1000.0, 0.3
     vol = dx ** 3 / 8
     rng = Splitmix(seed)
-    lo = np.array([int(round(0.03 * g)), int(round(0.45 * g)), int(round(0.45 * g))])
+    # cells 0 and 1 bin below block 0, so the clamped end starts at cell 2 at least
+    lo = np.array([max(2, int(round(0.03 * g))), int(round(0.45 * g)), int(round(0.45 * g))])
     ext = np.array([max(8, int(round(0.6 * g))), max(2, int(round(0.1 * g))), max(2, int(round(0.1 * g)))])
     cells = np.stack(np.meshgrid(*[np.arange(lo[a], lo[a] + ext[a]) for a in range(3)], indexing="ij"), -1).reshape(-1, 3)
     x = _stratified_cells(rng, cells.astype(np.float64), dx).astype(np.float32)

@@ -159,6 +159,36 @@ int ckmpm_get_phase_times(ckmpm_engine* eng, double ms[3], uint64_t* substeps);
 int ckmpm_destroy(ckmpm_engine* eng);
 const char* ckmpm_last_error(const ckmpm_engine* eng);
 
+/* ---------------------------------------------------------------- adjoint
+ * No reference counterpart (the reference has no differentiation).  Reverse-mode
+ * substeps for fixed-corotated and Neo-Hookean models with per-particle lambda and mu
+ * (gradients w.r.t. those and the per-particle initial velocity).  Usage:
+ *   ckmpm_adjoint_enable (after ckmpm_add_model, before ckmpm_initial_setup);
+ *   forward: ckmpm_checkpoint_save every S substeps, ckmpm_substep;
+ *   backward: ckmpm_adjoint_reset, ckmpm_adjoint_seed (dL/dx at the last state), then
+ *   per segment from the last: ckmpm_checkpoint_restore, ckmpm_adjoint_record(1), the
+ *   segment's ckmpm_substep calls again, ckmpm_adjoint_record(0), one
+ *   ckmpm_adjoint_substep per recorded substep; finally ckmpm_adjoint_gradients. */
+typedef struct ckmpm_checkpoint ckmpm_checkpoint;
+/* tape of up to `segment` substeps; also keeps particle identities (input order) */
+int ckmpm_adjoint_enable(ckmpm_engine* eng, uint32_t segment);
+int ckmpm_checkpoint_save(ckmpm_engine* eng, ckmpm_checkpoint** out);
+int ckmpm_checkpoint_restore(ckmpm_engine* eng, const ckmpm_checkpoint* ck);
+int ckmpm_checkpoint_free(ckmpm_checkpoint* ck);
+/* on != 0: the following substeps are recorded (the tape restarts empty) */
+int ckmpm_adjoint_record(ckmpm_engine* eng, int32_t on);
+/* zero every particle and grid adjoint */
+int ckmpm_adjoint_reset(ckmpm_engine* eng);
+/* dL/dx of the model's particles at the current state, input order, 3 floats each */
+int ckmpm_adjoint_seed(ckmpm_engine* eng, int32_t model, const float* xbar);
+/* back-propagate through the last recorded substep and drop it from the tape */
+int ckmpm_adjoint_substep(ckmpm_engine* eng);
+/* after the first substep has been undone: dL/dlambda, dL/dmu (n each) and dL/dv0
+ * (3 n), input order; any pointer may be null */
+int ckmpm_adjoint_gradients(ckmpm_engine* eng, int32_t model, float* lambda_bar, float* mu_bar, float* v0_bar);
+/* positions (3 per particle) in input order; needs the adjoint enabled */
+int ckmpm_get_positions_input_order(ckmpm_engine* eng, int32_t model, float* xyz);
+
 /* ---------------------------------------------------------------- batched scenes
  * No reference counterpart (the reference engine is a singleton, mpm_engine.cuh:80-84):
  * many independent engines advanced together.  The batch does not own the engines;

@@ -56,6 +56,7 @@ def test_forward_matches_cpp_oracle():
     for k in range(steps):
         ids = ids[o.perm(0) & SLOT]
         o.substep(0.0 if k == 0 else sc.dt, sc.dt)
+    assert o.flags == 0
     st = o.state(0)
     x_o = np.empty((st.shape[1], 3))
     F_o = np.empty((st.shape[1], 9))
