@@ -6,6 +6,8 @@ E = exp(log_E) (Poisson ratio fixed) and per-particle initial velocity v0, and r
 the final positions in input order.  Its backward replays the run from checkpoints
 taken every `segment` substeps, recording each segment's substeps and undoing them with
 the adjoint kernels (ckmpm_adjoint_*), then maps (lambda-bar, mu-bar) to log_E-bar.
+With segment=None the forward records every substep (a tape of all of them, which
+HBM holds for scenes of this size) and the backward undoes them without replaying.
 All arithmetic is the library's (FP32 kernels); torch only carries the tensors.
 """
 from __future__ import annotations
@@ -34,12 +36,17 @@ class _Simulate(torch.autograd.Function):
         m.mu_p = (E * fm).astype(np.float32)
         m.v = v0.detach().cpu().numpy().astype(np.float32).reshape(-1, 3)
         sc.models = [m]
-        eng = engine_from_scene(sc, adjoint_segment=segment)
+        full = segment is None
+        eng = engine_from_scene(sc, adjoint_segment=steps if full else segment)
         cks = []
+        if full:
+            eng.adjoint_record(True)
         for k in range(steps):
-            if k % segment == 0:
+            if not full and k % segment == 0:
                 cks.append(eng.checkpoint())
             eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        if full:
+            eng.adjoint_record(False)
         eng.Synchronize()
         x = eng.positions_input_order(0)
         ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment = eng, cks, sc, steps, segment
@@ -51,6 +58,9 @@ class _Simulate(torch.autograd.Function):
         eng, cks, sc, steps, seg = ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment
         eng.adjoint_reset()
         eng.adjoint_seed(0, gx.detach().cpu().numpy().astype(np.float32))
+        if seg is None:
+            for _ in range(steps):
+                eng.adjoint_substep()
         for s in reversed(range(len(cks))):
             eng.restore(cks[s])
             eng.adjoint_record(True)
@@ -69,7 +79,7 @@ class _Simulate(torch.autograd.Function):
         return g_logE.to(gx.device), g_v0.to(gx.device), None, None, None, None
 
 
-def simulate(log_E: torch.Tensor, v0: torch.Tensor, scene: Scene, steps: int, segment: int = 25,
+def simulate(log_E: torch.Tensor, v0: torch.Tensor, scene: Scene, steps: int, segment=25,
              nu: float = 0.3) -> torch.Tensor:
     """Final positions (n, 3, float64, input order) of `steps` substeps of `scene`'s
     model with Young's modulus exp(log_E) and initial velocities v0 (n, 3)."""

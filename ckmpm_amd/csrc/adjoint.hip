@@ -40,6 +40,17 @@ __global__ void FillZeroKernel(float* p, size_t n) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = 0.0f;
 }
+// copy / clear the first Nb blocks (device count) of a per-block array
+__global__ void CopyBlocksKernel(const float* src, float* dst, const LayoutCounts* lc, uint32_t cap, int fpb) {
+    const size_t n = static_cast<size_t>(min(lc->Nb, cap)) * fpb;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+__global__ void ZeroBlocksKernel(float* dst, const LayoutCounts* lc, uint32_t cap, int fpb) {
+    const size_t n = static_cast<size_t>(min(lc->Nb, cap)) * fpb;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = 0.0f;
+}
 __global__ void IdsPermuteKernel(const uint32_t* perm, const uint32_t* ids, uint32_t n, uint32_t* out) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
         out[t] = ids[perm[t] & kSlotMask];
@@ -124,6 +135,7 @@ struct AdjArgs {
     float* Eb_out;
     float* tb_out;
     float dt, next_dt;
+    uint32_t split;          // workgroups per particle block (each takes every split-th chunk)
 };
 
 __device__ __forceinline__ float W0p(float t) { return -1.0f + cospif(2.0f * t); }
@@ -173,8 +185,9 @@ __device__ __forceinline__ void Cofactor(const float (&F)[9], float (&c)[9]) {
     c[8] = F[0] * F[4] - F[1] * F[3];
 }
 
+// 2 waves per SIMD: 256 VGPRs (a few spills for FC) instead of ~275 and one wave
 template <int KIND>
-__global__ __launch_bounds__(kAT) void G2P2GBackwardKernel(AdjArgs a) {
+__global__ __launch_bounds__(kAT) __attribute__((amdgpu_waves_per_eu(2, 2))) void G2P2GBackwardKernel(AdjArgs a) {
     __shared__ float sV[kVChannels * 2 * kAWin];
     __shared__ float sMb[kMChannels * 2 * kAWin];
     __shared__ double sVb[kVChannels * 2 * kAWin];
@@ -184,9 +197,11 @@ __global__ __launch_bounds__(kAT) void G2P2GBackwardKernel(AdjArgs a) {
     const float dx = a.dom.dx, inv_dx = a.dom.inv_dx;
     const size_t N = a.n;
     const float mass = a.mat.mass;
-    for (uint32_t j = blockIdx.x; j < A; j += gridDim.x) {
+    // workgroup w serves part w % split of blocks w / split, w / split + gridDim / split, ...
+    const uint32_t part = blockIdx.x % a.split, per = gridDim.x / a.split;
+    for (uint32_t j = blockIdx.x / a.split; j < A; j += per) {
         const uint32_t beg = a.off[j], end = a.off[j + 1];
-        if (beg == end) continue;
+        if (beg + part * kAT >= end) continue;  // no chunk for this part
         int bb[3];
         CoordsOf(a.dom, a.keys[j], bb[0], bb[1], bb[2]);
         if (tid < 8) {
@@ -210,7 +225,7 @@ __global__ __launch_bounds__(kAT) void G2P2GBackwardKernel(AdjArgs a) {
             if (ch < 3) sV[l] = bi != kEmpty ? a.V[static_cast<size_t>(bi) * kVFloats + r] : 0.0f;
         }
         __syncthreads();
-        for (uint32_t c0 = beg; c0 < end; c0 += kAT) {
+        for (uint32_t c0 = beg + part * kAT; c0 < end; c0 += a.split * kAT) {
             const uint32_t t = c0 + tid;
             if (t < end) {
                 const uint32_t pe = a.perm[t];
@@ -689,27 +704,53 @@ bool DevAlloc(std::vector<void*>& list, T** p, size_t count) {
     return true;
 }
 
-bool AllocSnapshot(ckmpm_engine* e, Snapshot& s, std::vector<void*>& list) {
-    bool ok = true;
-    for (int l = 0; l < 2; ++l) {
-        ok = ok && DevAlloc(list, &s.keys[l], e->cap);
-        ok = ok && DevAlloc(list, &s.table[l], e->dom.nblocks);
-        ok = ok && DevAlloc(list, &s.lc[l], 1);
+// Carves snapshots out of one allocation (a tape of hundreds of snapshots would
+// otherwise cost thousands of hipMalloc / hipFree calls).
+struct Carver {
+    char* base = nullptr;
+    size_t used = 0;
+    template <typename T>
+    T* take(size_t count) {
+        T* p = reinterpret_cast<T*>(base ? base + used : nullptr);
+        used += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+        return p;
     }
-    ok = ok && DevAlloc(list, &s.M, static_cast<size_t>(e->cap) * kMFloats);
+};
+
+void CarveSnapshot(ckmpm_engine* e, Snapshot& s, Carver& c) {
+    for (int l = 0; l < 2; ++l) {
+        s.keys[l] = c.take<uint32_t>(e->cap);
+        s.table[l] = c.take<uint32_t>(e->dom.nblocks);
+        s.lc[l] = c.take<LayoutCounts>(1);
+    }
+    s.M = c.take<float>(static_cast<size_t>(e->cap) * kMFloats);
     const size_t nm = e->models.size();
     s.perm.assign(nm, nullptr);
     s.off.assign(nm, nullptr);
     s.ids.assign(nm, nullptr);
     s.st.assign(nm, nullptr);
-    for (size_t i = 0; ok && i < nm; ++i) {
+    for (size_t i = 0; i < nm; ++i) {
         const ModelBuf& m = e->models[i];
-        ok = ok && DevAlloc(list, &s.perm[i], m.n);
-        ok = ok && DevAlloc(list, &s.off[i], static_cast<size_t>(e->cap) + 1);
-        ok = ok && DevAlloc(list, &s.ids[i], m.n);
-        ok = ok && DevAlloc(list, &s.st[i], static_cast<size_t>(m.n) * m.nattr);
+        s.perm[i] = c.take<uint32_t>(m.n);
+        s.off[i] = c.take<uint32_t>(static_cast<size_t>(e->cap) + 1);
+        s.ids[i] = c.take<uint32_t>(m.n);
+        s.st[i] = c.take<float>(static_cast<size_t>(m.n) * m.nattr);
     }
-    return ok;
+}
+
+// count snapshots in one device allocation (recorded in list)
+bool AllocSnapshots(ckmpm_engine* e, Snapshot* s, size_t count, std::vector<void*>& list) {
+    Carver sizer;
+    Snapshot probe;
+    CarveSnapshot(e, probe, sizer);
+    const size_t per = sizer.used;
+    char* base = nullptr;
+    if (!DevAlloc(list, &base, per * count)) return false;
+    for (size_t k = 0; k < count; ++k) {
+        Carver c{base + k * per, 0};
+        CarveSnapshot(e, s[k], c);
+    }
+    return true;
 }
 
 int Take(ckmpm_engine* e, Snapshot& s) {
@@ -721,7 +762,9 @@ int Take(ckmpm_engine* e, Snapshot& s) {
         CK_HIP(e, hipMemcpyAsync(s.table[l], L[l]->table, sizeof(uint32_t) * e->dom.nblocks, hipMemcpyDeviceToDevice, st));
         CK_HIP(e, hipMemcpyAsync(s.lc[l], L[l]->lc, sizeof(LayoutCounts), hipMemcpyDeviceToDevice, st));
     }
-    CK_HIP(e, hipMemcpyAsync(s.M, e->M[e->mprev], sizeof(float) * kMFloats * e->cap, hipMemcpyDeviceToDevice, st));
+    // M_k is indexed by the previous layout: its Nb blocks are the ones in use
+    CopyBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(e->M[e->mprev], s.M,
+                                                                                    L[1]->lc, e->cap, kMFloats);
     for (size_t i = 0; i < e->models.size(); ++i) {
         const ModelBuf& m = e->models[i];
         CK_HIP(e, hipMemcpyAsync(s.perm[i], m.perm[e->sc], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
@@ -753,7 +796,8 @@ int Put(ckmpm_engine* e, const Snapshot& s) {
         CK_HIP(e, hipMemcpyAsync(L[l]->table, s.table[l], sizeof(uint32_t) * e->dom.nblocks, hipMemcpyDeviceToDevice, st));
         CK_HIP(e, hipMemcpyAsync(L[l]->lc, s.lc[l], sizeof(LayoutCounts), hipMemcpyDeviceToDevice, st));
     }
-    CK_HIP(e, hipMemcpyAsync(e->M[e->mprev], s.M, sizeof(float) * kMFloats * e->cap, hipMemcpyDeviceToDevice, st));
+    CopyBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(s.M, e->M[e->mprev], s.lc[1],
+                                                                                    e->cap, kMFloats);
     for (size_t i = 0; i < e->models.size(); ++i) {
         ModelBuf& m = e->models[i];
         CK_HIP(e, hipMemcpyAsync(m.perm[e->sc], s.perm[i], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
@@ -840,8 +884,7 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
     AdjointState* A = new AdjointState();
     A->seg = segment;
     A->tape.resize(segment);
-    bool ok = true;
-    for (Snapshot& s : A->tape) ok = ok && AllocSnapshot(e, s, A->allocs);
+    bool ok = AllocSnapshots(e, A->tape.data(), segment, A->allocs);
     const size_t nm = e->models.size();
     for (int k = 0; k < 2; ++k) {
         A->ids[k].assign(nm, nullptr);
@@ -882,7 +925,7 @@ int ckmpm_checkpoint_save(ckmpm_engine* e, ckmpm_checkpoint** out) {
     if (!e || !out) return CKMPM_ERR_INVALID;
     if (!e->adj || !e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "checkpoints need the adjoint enabled and set up");
     ckmpm_checkpoint* c = new ckmpm_checkpoint();
-    if (!AllocSnapshot(e, c->snap, c->allocs)) {
+    if (!AllocSnapshots(e, &c->snap, 1, c->allocs)) {
         ckmpm_checkpoint_free(c);
         return Fail(e, CKMPM_ERR_HIP, "checkpoint allocation failed");
     }
@@ -963,8 +1006,7 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
     g.st = A->st_scratch;
     const uint32_t grid = GridSize(e);
     LaunchGridUpdate(g, std::max<uint32_t>(2 * grid, 64u), st);
-    FillZeroKernel<<<AGrid(static_cast<size_t>(e->cap) * kVFloats), kAT, 0, st>>>(A->Vbar,
-                                                                                 static_cast<size_t>(e->cap) * kVFloats);
+    ZeroBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kVFloats), kAT, 0, st>>>(A->Vbar, s.lc[0], e->cap, kVFloats);
     const int nb = 1 - A->ab;
     for (size_t i = 0; i < e->models.size(); ++i) {
         const ModelBuf& m = e->models[i];
@@ -990,11 +1032,16 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
         a.tb_out = A->tb[nb][i];
         a.dt = static_cast<float>(s.dt);
         a.next_dt = static_cast<float>(s.nd);
-        if (m.kind == kFixedCorotated) G2P2GBackwardKernel<kFixedCorotated><<<grid, kAT, 0, st>>>(a);
-        else G2P2GBackwardKernel<kNeoHookean><<<grid, kAT, 0, st>>>(a);
+        // few particle blocks (small scenes): several workgroups per block so that the
+        // launch still covers the 256 CUs
+        a.split = std::min<uint32_t>(8, std::max<uint32_t>(1, (2048 + e->a_hint) / std::max<uint32_t>(e->a_hint, 1)));
+        const uint32_t bgrid = std::min<uint32_t>(grid * a.split, 65536u / a.split * a.split);
+        if (m.kind == kFixedCorotated) G2P2GBackwardKernel<kFixedCorotated><<<bgrid, kAT, 0, st>>>(a);
+        else G2P2GBackwardKernel<kNeoHookean><<<bgrid, kAT, 0, st>>>(a);
     }
     float* Mout = A->Mbar[1 - A->mb];
-    FillZeroKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(Mout, static_cast<size_t>(e->cap) * kMFloats);
+    // M_k-bar lives in the previous layout; blocks beyond its Nb are never read
+    ZeroBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(Mout, s.lc[1], e->cap, kMFloats);
     GridBwdArgs gb{};
     gb.dom = e->dom;
     gb.keys = s.keys[0];
