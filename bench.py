@@ -250,8 +250,11 @@ def main():
     eng = engine_from_scene(sc)
     print(f"[bench] engine set up, {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     dt = sc.dt
-    for k in range(args.warmup):
-        eng.Substep(0.0 if k == 0 else dt, dt)
+    # warm-up: the autotuner's substeps (two candidate workgroup sizes) count toward it
+    tune = max(1, args.warmup // 4)
+    eng.autotune(tune)
+    for k in range(max(0, args.warmup - 2 * tune)):
+        eng.Substep(dt, dt)
     eng.Synchronize()
     torch.cuda.synchronize()
     print(f"[bench] {args.warmup} warmup substeps done, timing {args.steps}", file=sys.stderr, flush=True)
@@ -297,6 +300,7 @@ def main():
         "phase_ms_per_substep": {"grid_update": ms_grid / max(nsub, 1), "g2p2g": ms_g2p2g / max(nsub, 1),
                                  "rebin": ms_rebin / max(nsub, 1)},
         "blocks": {"active": c1.active_blocks, "neighbor": c1.neighbor_blocks, "exterior": c1.exterior_blocks},
+        "g2p2g_workgroup": [eng.tuning(m) for m in range(len(sc.models))],
     }
     traffic = _measured_traffic(args.traffic, sc.name)
     if traffic is not None:

@@ -115,7 +115,9 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         a.dt = static_cast<float>(dt);
         a.next_dt = static_cast<float>(next_dt);
         a.st = e->status;
-        LaunchG2P2G(a, grid, s);
+        if (e->tuning) (void)hipEventRecord(m.tune_ev[0], s);
+        LaunchG2P2G(a, grid, m.threads, s);
+        if (e->tuning) (void)hipEventRecord(m.tune_ev[1], s);
     }
     Mark(e);
     // ---- rebin: new layout in P (the previous layout's buffers are free now)
@@ -394,6 +396,61 @@ int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
     return ckmpm_synchronize(e);
 }
 
+// Autotuner: each candidate G2P2G workgroup size runs `substeps` real substeps (the
+// simulation advances as with ckmpm_substep at the engine's dt); each model keeps the
+// size with the smallest total G2P2G device time.
+int ckmpm_autotune(ckmpm_engine* e, uint32_t substeps) {
+    CK_TRY(CheckReady(e));
+    if (substeps == 0) return CKMPM_OK;
+    static const int kCandidates[2] = {256, 512};
+    const size_t nm = e->models.size();
+    for (ModelBuf& m : e->models)
+        for (hipEvent_t& ev : m.tune_ev)
+            if (!ev) CK_HIP(e, hipEventCreate(&ev));
+    std::vector<double> best(nm, 1e300);
+    std::vector<int> pick(nm, 256);
+    e->tuning = true;
+    for (int c : kCandidates) {
+        for (ModelBuf& m : e->models) m.threads = c;
+        std::vector<double> tot(nm, 0.0);
+        for (uint32_t k = 0; k < substeps; ++k) {
+            const double dt = e->substeps == 0 ? 0.0 : e->dt_default;
+            const int rc = ckmpm_substep(e, dt, e->dt_default);
+            if (rc != CKMPM_OK) {
+                e->tuning = false;
+                return rc;
+            }
+            CK_HIP(e, hipStreamSynchronize(e->stream));
+            for (size_t i = 0; i < nm; ++i) {
+                float ms = 0.0f;
+                CK_HIP(e, hipEventElapsedTime(&ms, e->models[i].tune_ev[0], e->models[i].tune_ev[1]));
+                tot[i] += ms;
+            }
+        }
+        for (size_t i = 0; i < nm; ++i)
+            if (tot[i] < best[i]) {
+                best[i] = tot[i];
+                pick[i] = c;
+            }
+    }
+    e->tuning = false;
+    for (size_t i = 0; i < nm; ++i) e->models[i].threads = pick[i];
+    return ckmpm_synchronize(e);
+}
+
+int ckmpm_set_tuning(ckmpm_engine* e, int32_t model, int32_t threads) {
+    if (!e || model < 0 || model >= static_cast<int32_t>(e->models.size())) return CKMPM_ERR_INVALID;
+    if (threads != 256 && threads != 512) return Fail(e, CKMPM_ERR_INVALID, "G2P2G workgroup size must be 256 or 512");
+    e->models[model].threads = threads;
+    return CKMPM_OK;
+}
+
+int ckmpm_get_tuning(ckmpm_engine* e, int32_t model, int32_t* threads) {
+    if (!e || !threads || model < 0 || model >= static_cast<int32_t>(e->models.size())) return CKMPM_ERR_INVALID;
+    *threads = e->models[model].threads;
+    return CKMPM_OK;
+}
+
 int ckmpm_synchronize(ckmpm_engine* e) {
     if (!e) return CKMPM_ERR_INVALID;
     Status st;
@@ -554,6 +611,9 @@ int ckmpm_destroy(ckmpm_engine* e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     AdjointDestroy(e);
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
+    for (ModelBuf& m : e->models)
+        for (hipEvent_t ev : m.tune_ev)
+            if (ev) (void)hipEventDestroy(ev);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

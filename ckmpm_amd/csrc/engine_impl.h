@@ -24,6 +24,8 @@ struct ModelBuf {
     uint32_t* cnt_e = nullptr;
     uint32_t* cntn = nullptr;
     int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
+    int threads = 256;    // G2P2G workgroup size (256 or 512; ckmpm_autotune chooses)
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};  // around the G2P2G launch while tuning
     // setup inputs (host), released after the initial setup
     std::vector<float> x0, v0, lam0, mu0;
     uint32_t nvel = 0;
@@ -76,6 +78,7 @@ struct ckmpm_engine {
     std::vector<void*> allocs;
     // phase timing: 4 events per substep (before grid, after grid, after G2P2G, after rebin)
     bool profile = false;
+    bool tuning = false;  // record tune_ev around each model's G2P2G launch
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
     // adjoint support (null unless ckmpm_adjoint_enable was called)

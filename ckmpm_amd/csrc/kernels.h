@@ -102,7 +102,8 @@ struct G2P2GArgs {
     float dt, next_dt;
     Status* st;
 };
-void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, hipStream_t s);
+// threads per workgroup: 256 or 512 (one particle block per workgroup either way)
+void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s);
 
 struct RebinModels {
     int nmodels;

@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
         float v[3] = {0.0f, 0.0f, 0.0f};
         float v2 = 0.0f;
         if (m > a.mass_clamp) {
-            const float im = 1.0f / m;
+            const float im = __builtin_amdgcn_rcpf(m);
 #pragma unroll
             for (int k = 0; k < 3; ++k) v[k] = p[k] * im;
             v[1] += a.gravity * a.dt;
@@ -514,12 +514,12 @@ constexpr int kWin = 512;  // 8^3 nodes of one sub-grid window
 using LdsAcc = double;
 __device__ __forceinline__ void LdsAdd(LdsAcc* p, float v) { atomicAdd(p, static_cast<LdsAcc>(v)); }
 
-template <int KIND>
-__global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
+template <int KIND, int THREADS>
+__global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
     __shared__ float sV[kVChannels * 2 * kWin];
     __shared__ LdsAcc sM[kMChannels * 2 * kWin];
     __shared__ uint32_t sHalo[8];
-    __shared__ uint32_t sWave[kThreads / 64][32];
+    __shared__ uint32_t sWave[THREADS / 64][32];
     __shared__ uint32_t sRun[32];
 
     const uint32_t A = a.lc->A, Nb = a.lc->Nb;
@@ -548,10 +548,10 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
             sHalo[tid] = bi;
         }
         if (tid < 32) sRun[tid] = 0;
-        for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) sM[i] = 0;
+        for (int i = tid; i < kMChannels * 2 * kWin; i += THREADS) sM[i] = 0;
         __syncthreads();
         // velocity halo: 8 blocks x 3 channels x 128 nodes
-        for (int i = tid; i < 8 * kVFloats; i += kThreads) {
+        for (int i = tid; i < 8 * kVFloats; i += THREADS) {
             const int o = i / kVFloats, r = i % kVFloats;
             const int ch = r >> 7, node = r & 127, g = node >> 6;
             const int lx = ((node >> 4) & 3) + 4 * ((o >> 2) & 1);
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
         }
         __syncthreads();
 
-        for (uint32_t c0 = beg; c0 < end; c0 += kThreads) {
+        for (uint32_t c0 = beg; c0 < end; c0 += THREADS) {
             const uint32_t t = c0 + tid;
             const bool active = t < end;
             uint32_t dir = 31u;
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
                 }
             }
             // ---------------- deterministic rank among this block's particles with the same move
-            if (tid < kThreads / 64 * 32) (&sWave[0][0])[tid] = 0;
+            if (tid < THREADS / 64 * 32) (&sWave[0][0])[tid] = 0;
             __syncthreads();
             const uint64_t peers = MatchPeers<5>(dir);
             const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
@@ -812,13 +812,13 @@ __global__ __launch_bounds__(kThreads) void G2P2GKernel(G2P2GArgs a) {
             __syncthreads();
             if (tid < 27) {
                 uint32_t add = 0;
-                for (uint32_t w = 0; w < kThreads / 64; ++w) add += sWave[w][tid];
+                for (uint32_t w = 0; w < THREADS / 64; ++w) add += sWave[w][tid];
                 sRun[tid] += add;
             }
             __syncthreads();
         }
         // ---------------- flush: one global atomic per non-zero node and channel
-        for (int i = tid; i < kMChannels * 2 * kWin; i += kThreads) {
+        for (int i = tid; i < kMChannels * 2 * kWin; i += THREADS) {
             const float val = static_cast<float>(sM[i]);
             if (val == 0.0f) continue;
             const int ch = i / (2 * kWin), g = (i / kWin) & 1, l = i & (kWin - 1);
@@ -1075,14 +1075,19 @@ void LaunchBeginSubstep(Status* st, hipStream_t s) { BeginSubstepKernel<<<1, 64,
 void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s) {
     GridUpdateKernel<<<grid, kThreads, 0, s>>>(a);
 }
-void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
-    grid = ((grid + 7) / 8) * 8;
+template <int T>
+void LaunchG2P2GT(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
     switch (a.mat.kind) {
-        case kFixedCorotated: G2P2GKernel<kFixedCorotated><<<grid, kThreads, 0, s>>>(a); break;
-        case kNeoHookean: G2P2GKernel<kNeoHookean><<<grid, kThreads, 0, s>>>(a); break;
-        case kDruckerPrager: G2P2GKernel<kDruckerPrager><<<grid, kThreads, 0, s>>>(a); break;
-        default: G2P2GKernel<kFluid><<<grid, kThreads, 0, s>>>(a); break;
+        case kFixedCorotated: G2P2GKernel<kFixedCorotated, T><<<grid, T, 0, s>>>(a); break;
+        case kNeoHookean: G2P2GKernel<kNeoHookean, T><<<grid, T, 0, s>>>(a); break;
+        case kDruckerPrager: G2P2GKernel<kDruckerPrager, T><<<grid, T, 0, s>>>(a); break;
+        default: G2P2GKernel<kFluid, T><<<grid, T, 0, s>>>(a); break;
     }
+}
+void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s) {
+    grid = ((grid + 7) / 8) * 8;
+    if (threads == 512) LaunchG2P2GT<512>(a, grid, s);
+    else LaunchG2P2GT<256>(a, grid, s);
 }
 void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
                       const RebinModels& rm, uint32_t* occ, uint32_t cap, hipStream_t s) {

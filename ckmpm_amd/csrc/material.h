@@ -28,7 +28,8 @@ __device__ __forceinline__ void InverseSym3(const float (&a)[9], float (&r)[9]) 
     const float c11 = a[0] * a[8] - a[2] * a[6];
     const float c12 = a[2] * a[3] - a[0] * a[5];
     const float c22 = a[0] * a[4] - a[1] * a[3];
-    const float id = 1.0f / (a[0] * c00 + a[1] * (a[5] * a[6] - a[3] * a[8]) + a[2] * (a[3] * a[7] - a[4] * a[6]));
+    // v_rcp_f32 (1 ulp) instead of an IEEE division sequence
+    const float id = __builtin_amdgcn_rcpf(a[0] * c00 + a[1] * (a[5] * a[6] - a[3] * a[8]) + a[2] * (a[3] * a[7] - a[4] * a[6]));
     r[0] = id * c00;
     r[1] = r[3] = id * c01;
     r[2] = r[6] = id * c02;

@@ -8,6 +8,24 @@ namespace ckmpm {
 
 constexpr float kInv2Pi = 0.15915494309189535f;
 
+// sin(2 pi t) / (2 pi) for t in [0, 1): with u = t - 1/2, sin(2 pi t) = -sin(2 pi u);
+// |u| > 1/4 folds to 1/2 sgn(u) - u (sin(pi - a) = sin a), so x = 2 pi u' lies in
+// [-pi/2, pi/2], where the Taylor series of sin x / x through x^10 is within 4e-8
+// (the next term, x^12 / 13!).  Result = -u' (1 - z/3! + z^2/5! - ... - z^5/11!), z = x^2.
+__device__ __forceinline__ float SinTwoPiFrac(float t) {
+    float u = t - 0.5f;
+    if (u > 0.25f) u = 0.5f - u;
+    else if (u < -0.25f) u = -0.5f - u;
+    const float z = 39.47841760435743f * u * u;  // (2 pi u)^2
+    float p = -2.505210838544172e-08f;           // -1 / 11!
+    p = p * z + 2.755731922398589e-06f;          //  1 / 9!
+    p = p * z - 1.984126984126984e-04f;          // -1 / 7!
+    p = p * z + 8.333333333333333e-03f;          //  1 / 5!
+    p = p * z - 1.666666666666667e-01f;          // -1 / 3!
+    p = p * z + 1.0f;
+    return -u * p;
+}
+
 // Stencil of one sub-grid at an FP32 position (xs = x * invDx): base cell c,
 // fractional offset t and node-0 weight w0 per axis.  mpm_algorithm.cuh:1555-1569,
 // mpm_kernel.cuh:91-114 (w0 = 1 - t + sin(2 pi t) / (2 pi)).
@@ -25,7 +43,7 @@ __device__ __forceinline__ void MakeStencil(const float (&xs)[3], Stencil32& s) 
         const float t = __fadd_rn(xs[a] - static_cast<float>(c), shift);
         s.c[a] = c;
         s.t[a] = t;
-        s.w0[a] = 1.0f - t + sinpif(2.0f * t) * kInv2Pi;
+        s.w0[a] = 1.0f - t + SinTwoPiFrac(t);
     }
 }
 // Both sub-grid stencils from one sine per axis.  With x/dx < 2^22 every step is exact
@@ -37,7 +55,7 @@ __device__ __forceinline__ void MakeStencilPair(const float (&xs)[3], Stencil32&
     for (int a = 0; a < 3; ++a) {
         const int c = static_cast<int>(floorf(__fadd_rn(xs[a], 0.25f)));
         const float t = __fadd_rn(xs[a] - static_cast<float>(c), 0.25f);
-        const float sn = sinpif(2.0f * t) * kInv2Pi;
+        const float sn = SinTwoPiFrac(t);
         const bool hi = t >= 0.5f;
         s0.c[a] = c;
         s0.t[a] = t;

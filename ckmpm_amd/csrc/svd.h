@@ -39,9 +39,11 @@ template <int p, int q, int r>
 __device__ __forceinline__ void JacobiConjugate(float (&S)[9], float (&V)[9]) {
     const float a = S[p * 3 + p], d = S[q * 3 + q], b = S[p * 3 + q];
     const float dm = d - a;
-    const float den = fabsf(dm) + sqrtf(dm * dm + 4.0f * b * b);
-    const float t = den > 0.0f ? (dm >= 0.0f ? -2.0f : 2.0f) * b / den : 0.0f;
-    const float c = rsqrtf(1.0f + t * t);
+    // single-instruction sqrt / reciprocal / rsqrt (1 ulp): the rotation only has to
+    // be orthogonal, which c = rsq(1 + t^2), s = t c gives whatever t is
+    const float den = fabsf(dm) + __builtin_amdgcn_sqrtf(dm * dm + 4.0f * b * b);
+    const float t = den > 0.0f ? (dm >= 0.0f ? -2.0f : 2.0f) * b * __builtin_amdgcn_rcpf(den) : 0.0f;
+    const float c = __builtin_amdgcn_rsqf(1.0f + t * t);
     const float s = t * c;
     const float cc = c * c, ss = s * s, cs = c * s;
     const float spp = a * cc + 2.0f * b * cs + d * ss;
@@ -84,7 +86,7 @@ __device__ __forceinline__ void SortPair(float (&B)[9], float (&V)[9], float (&r
 template <int p, int q, int col>
 __device__ __forceinline__ void QrGivens(float (&B)[9], float (&U)[9]) {
     const float a1 = B[p * 3 + col], a2 = B[q * 3 + col];
-    const float rho = sqrtf(a1 * a1 + a2 * a2);
+    const float rho = __builtin_amdgcn_sqrtf(a1 * a1 + a2 * a2);
     float sh = rho > kQrEps ? a2 : 0.0f;
     float ch = fabsf(a1) + fmaxf(rho, kQrEps);
     if (a1 < 0.0f) {
@@ -92,7 +94,7 @@ __device__ __forceinline__ void QrGivens(float (&B)[9], float (&U)[9]) {
         sh = ch;
         ch = tmp;
     }
-    const float w = rsqrtf(ch * ch + sh * sh);
+    const float w = __builtin_amdgcn_rsqf(ch * ch + sh * sh);
     ch *= w;
     sh *= w;
     const float c = ch * ch - sh * sh, s = 2.0f * ch * sh;

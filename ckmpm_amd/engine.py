@@ -123,6 +123,18 @@ class MPMEngine:
         self._check(self._lib.ckmpm_get_counters(self._h, ctypes.byref(c)), "ckmpm_get_counters")
         return c
 
+    def autotune(self, substeps: int = 3):
+        """Pick each model's G2P2G workgroup size by timing real substeps (the scene advances)."""
+        self._check(self._lib.ckmpm_autotune(self._h, int(substeps)), "ckmpm_autotune")
+
+    def set_tuning(self, model: int, threads: int):
+        self._check(self._lib.ckmpm_set_tuning(self._h, model, int(threads)), "ckmpm_set_tuning")
+
+    def tuning(self, model: int) -> int:
+        t = ctypes.c_int32()
+        self._check(self._lib.ckmpm_get_tuning(self._h, model, ctypes.byref(t)), "ckmpm_get_tuning")
+        return int(t.value)
+
     def set_profiling(self, enable: bool):
         self._check(self._lib.ckmpm_set_profiling(self._h, int(enable)), "set_profiling")
 

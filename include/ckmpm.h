@@ -155,6 +155,15 @@ int ckmpm_export_grid_v(ckmpm_engine* eng, float* grid, uint64_t capacity_blocks
 int ckmpm_set_profiling(ckmpm_engine* eng, int32_t enable);
 int ckmpm_get_phase_times(ckmpm_engine* eng, double ms[3], uint64_t* substeps);
 
+/* Tuning (no reference counterpart: its launch shapes are compile-time constants).
+ * The G2P2G workgroup size of each model is 256 or 512 threads (one particle block per
+ * workgroup).  ckmpm_autotune runs `substeps` real substeps per candidate (the scene
+ * advances at sim.dt, the first with dt = 0) and keeps, per model, the size with the
+ * least G2P2G device time. */
+int ckmpm_autotune(ckmpm_engine* eng, uint32_t substeps);
+int ckmpm_set_tuning(ckmpm_engine* eng, int32_t model, int32_t threads);
+int ckmpm_get_tuning(ckmpm_engine* eng, int32_t model, int32_t* threads);
+
 /* MPMEngine destructor (the reference frees almost nothing) */
 int ckmpm_destroy(ckmpm_engine* eng);
 const char* ckmpm_last_error(const ckmpm_engine* eng);

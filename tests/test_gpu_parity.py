@@ -180,3 +180,34 @@ def test_out_of_window_is_reported_not_faulting():
     with pytest.raises(CkmpmError) as ei:
         eng.Synchronize()
     assert ei.value.code == ERR_OUT_OF_WINDOW
+
+
+@pytest.mark.gpu
+def test_workgroup_sizes_agree_and_autotune():
+    """G2P2G with 512-thread workgroups gives the same orderings (bit-exact) and state
+    (to the order of the float atomics) as with 256; the autotuner picks one of them."""
+    from ckmpm_amd.engine import engine_from_scene
+
+    sc = scenes.mixed_dam_sand(n_fluid=9000, n_sand=5000, scale=8)
+    a, b = engine_from_scene(sc), engine_from_scene(sc)
+    for m in range(2):
+        b.set_tuning(m, 512)
+    for k in range(6):
+        for e in (a, b):
+            e.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+    a.Synchronize()
+    b.Synchronize()
+    ka, ca = a.layout(0)
+    kb, cb = b.layout(0)
+    assert ca == cb
+    np.testing.assert_array_equal(ka, kb)
+    for m in range(2):
+        pa, oa = a.ordering(m)
+        pb, ob = b.ordering(m)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(oa, ob)
+        np.testing.assert_allclose(b.state(m), a.state(m), rtol=0, atol=1e-5)
+    a.autotune(2)
+    assert all(a.tuning(m) in (256, 512) for m in range(2))
+    a.close()
+    b.close()
