@@ -83,7 +83,8 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
     g.mass_clamp = e->sim.mass_clamp;
     g.cap = e->cap;
     g.st = e->status;
-    LaunchGridUpdate(g, GridFor(std::max<uint32_t>(e->a_hint * 2u, 64u)), s);
+    // 2 blocks x 4 chains per workgroup and pass: 0.19 A workgroups cover Nb (about 1.2 A)
+    LaunchGridUpdate(g, GridFor(e->a_hint * 3u / 16u + 8u), s);
     Mark(e);
 }
 

@@ -440,50 +440,74 @@ __global__ void BeginSubstepKernel(Status* st) {
 // through the previous key table, writes velocities (p/m, gravity on y, sticky
 // boundary within 2 blocks of the domain edge), clears the next P2G target and
 // reduces max |v|^2 (NaN -> +inf).  128 threads per block, 2 blocks per workgroup.
+// Each thread serves one node of kGridIlp blocks at a time: the key -> table -> mass /
+// momentum load chains of those blocks are issued together, so a wave has kGridIlp
+// independent latency chains in flight instead of one.
+constexpr int kGridIlp = 4;
 __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
     const uint32_t n = threadIdx.x & 127u;
+    const uint32_t stride = gridDim.x * 2;
     float vmax = 0.0f;
-    for (uint32_t i = blockIdx.x * 2 + (threadIdx.x >> 7); i < Nb; i += gridDim.x * 2) {
-        const uint32_t key = a.keys[i];
-        int b[3];
-        CoordsOf(a.dom, key, b[0], b[1], b[2]);
-        bool oob = false;
+    for (uint32_t i0 = blockIdx.x * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
+        uint32_t key[kGridIlp], io[kGridIlp];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) oob |= (b[k] + a.dom.off[k] < 2) || (b[k] + a.dom.off[k] + 2 >= a.dom.dim[k]);
-        const uint32_t io = a.table_prev[key];
-        float m = 0.0f, p[3] = {0.0f, 0.0f, 0.0f};
-        if (io != kEmpty && io < Nbp) {
-            const float* src = a.M_prev + static_cast<size_t>(io) * kMFloats;
-            m = src[n];
-            p[0] = src[128 + n];
-            p[1] = src[256 + n];
-            p[2] = src[384 + n];
+        for (int u = 0; u < kGridIlp; ++u) {
+            const uint32_t i = i0 + u * stride;
+            key[u] = i < Nb ? a.keys[i] : 0u;
         }
-        float v[3] = {0.0f, 0.0f, 0.0f};
-        float v2 = 0.0f;
-        if (m > a.mass_clamp) {
-            const float im = __builtin_amdgcn_rcpf(m);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) v[k] = p[k] * im;
-            v[1] += a.gravity * a.dt;
-            if (oob) v[0] = v[1] = v[2] = 0.0f;
-            v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        for (int u = 0; u < kGridIlp; ++u) {
+            const uint32_t i = i0 + u * stride;
+            io[u] = i < Nb ? a.table_prev[key[u]] : kEmpty;
         }
-        float* dst = a.V + static_cast<size_t>(i) * kVFloats;
-        dst[n] = v[0];
-        dst[128 + n] = v[1];
-        dst[256 + n] = v[2];
-        float* mz = a.M_zero + static_cast<size_t>(i) * kMFloats;
-        mz[n] = 0.0f;
-        mz[128 + n] = 0.0f;
-        mz[256 + n] = 0.0f;
-        mz[384 + n] = 0.0f;
-        if (isnan(v2)) {
-            v2 = INFINITY;
-            SetFlag(a.st, kFlagNonFinite);
+        float m[kGridIlp], p[kGridIlp][3];
+#pragma unroll
+        for (int u = 0; u < kGridIlp; ++u) {
+            m[u] = 0.0f;
+            p[u][0] = p[u][1] = p[u][2] = 0.0f;
+            if (io[u] != kEmpty && io[u] < Nbp) {
+                const float* src = a.M_prev + static_cast<size_t>(io[u]) * kMFloats;
+                m[u] = src[n];
+                p[u][0] = src[128 + n];
+                p[u][1] = src[256 + n];
+                p[u][2] = src[384 + n];
+            }
         }
-        vmax = fmaxf(vmax, v2);
+#pragma unroll
+        for (int u = 0; u < kGridIlp; ++u) {
+            const uint32_t i = i0 + u * stride;
+            if (i >= Nb) continue;
+            int b[3];
+            CoordsOf(a.dom, key[u], b[0], b[1], b[2]);
+            bool oob = false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) oob |= (b[k] + a.dom.off[k] < 2) || (b[k] + a.dom.off[k] + 2 >= a.dom.dim[k]);
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            float v2 = 0.0f;
+            if (m[u] > a.mass_clamp) {
+                const float im = __builtin_amdgcn_rcpf(m[u]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[k] = p[u][k] * im;
+                v[1] += a.gravity * a.dt;
+                if (oob) v[0] = v[1] = v[2] = 0.0f;
+                v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            }
+            float* dst = a.V + static_cast<size_t>(i) * kVFloats;
+            dst[n] = v[0];
+            dst[128 + n] = v[1];
+            dst[256 + n] = v[2];
+            float* mz = a.M_zero + static_cast<size_t>(i) * kMFloats;
+            mz[n] = 0.0f;
+            mz[128 + n] = 0.0f;
+            mz[256 + n] = 0.0f;
+            mz[384 + n] = 0.0f;
+            if (isnan(v2)) {
+                v2 = INFINITY;
+                SetFlag(a.st, kFlagNonFinite);
+            }
+            vmax = fmaxf(vmax, v2);
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
@@ -492,9 +516,9 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     if (LaneId() == 0) sMax[threadIdx.x >> 6] = vmax;
     __syncthreads();
     if (threadIdx.x == 0) {
-        float m = 0.0f;
-        for (int w = 0; w < kThreads / 64; ++w) m = fmaxf(m, sMax[w]);
-        if (m > 0.0f) atomicMax(&a.st->vmax2_bits, __float_as_uint(m));
+        float mx = 0.0f;
+        for (int w = 0; w < kThreads / 64; ++w) mx = fmaxf(mx, sMax[w]);
+        if (mx > 0.0f) atomicMax(&a.st->vmax2_bits, __float_as_uint(mx));
     }
 }
 
