@@ -531,6 +531,28 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
 // particles with the same move, for the counting-sort rebin.
 constexpr int kWin = 512;  // 8^3 nodes of one sub-grid window
 
+// LDS windows of a particle block (node coordinates relative to the block's first
+// node, 0..7 over the block and its +1 neighbours).  Stencil bases lie in [1,6] for
+// grid 0 and [0,6] for grid 1 (window check), so grid 0 uses nodes [1,7]^3 and grid 1
+// nodes [0,7]^3: 343 + 512 nodes per channel instead of 2 x 512.
+constexpr int kM0Win = 343;              // 7^3
+constexpr int kMWin = kM0Win + 512;      // both sub-grids, one channel
+// window entry r in [0, kMWin) -> sub-grid and node coordinates
+__device__ __forceinline__ void WindowNode(int r, int& g, int& lx, int& ly, int& lz) {
+    if (r >= kM0Win) {
+        const int l = r - kM0Win;
+        g = 1;
+        lx = l >> 6;
+        ly = (l >> 3) & 7;
+        lz = l & 7;
+    } else {
+        g = 0;
+        lx = r / 49 + 1;
+        ly = (r / 7) % 7 + 1;
+        lz = r % 7 + 1;
+    }
+}
+
 // P2G accumulator in LDS.  gfx950 executes ds_add_f32 lane by lane (about 190 cycles
 // per wavefront instruction whatever the addresses) but ds_add_f64 at about 20, so
 // the deposit accumulates in FP64 (which also makes the LDS sum nearly independent
@@ -538,10 +560,11 @@ constexpr int kWin = 512;  // 8^3 nodes of one sub-grid window
 using LdsAcc = double;
 __device__ __forceinline__ void LdsAdd(LdsAcc* p, float v) { atomicAdd(p, static_cast<LdsAcc>(v)); }
 
+// 4 waves per SIMD: LDS allows 4 workgroups per CU; this caps the VGPRs at 128
 template <int KIND, int THREADS>
-__global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
-    __shared__ float sV[kVChannels * 2 * kWin];
-    __shared__ LdsAcc sM[kMChannels * 2 * kWin];
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GKernel(G2P2GArgs a) {
+    __shared__ float sV[kVChannels * kMWin];
+    __shared__ LdsAcc sM[kMChannels * kMWin];
     __shared__ uint32_t sHalo[8];
     __shared__ uint32_t sWave[THREADS / 64][32];
     __shared__ uint32_t sRun[32];
@@ -572,17 +595,17 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
             sHalo[tid] = bi;
         }
         if (tid < 32) sRun[tid] = 0;
-        for (int i = tid; i < kMChannels * 2 * kWin; i += THREADS) sM[i] = 0;
+        for (int i = tid; i < kMChannels * kMWin; i += THREADS) sM[i] = 0;
         __syncthreads();
-        // velocity halo: 8 blocks x 3 channels x 128 nodes
-        for (int i = tid; i < 8 * kVFloats; i += THREADS) {
-            const int o = i / kVFloats, r = i % kVFloats;
-            const int ch = r >> 7, node = r & 127, g = node >> 6;
-            const int lx = ((node >> 4) & 3) + 4 * ((o >> 2) & 1);
-            const int ly = ((node >> 2) & 3) + 4 * ((o >> 1) & 1);
-            const int lz = (node & 3) + 4 * (o & 1);
+        // velocity halo in the same windows as the P2G accumulator (3 channels)
+        for (int i = tid; i < kVChannels * kMWin; i += THREADS) {
+            const int ch = i / kMWin, r = i % kMWin;
+            int g, lx, ly, lz;
+            WindowNode(r, g, lx, ly, lz);
+            const int o = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
+            const int node = g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
             const uint32_t bi = sHalo[o];
-            sV[(ch * 2 + g) * kWin + lx * 64 + ly * 8 + lz] = bi != kEmpty ? a.V[static_cast<size_t>(bi) * kVFloats + r] : 0.0f;
+            sV[i] = bi != kEmpty ? a.V[static_cast<size_t>(bi) * kVFloats + ch * 128 + node] : 0.0f;
         }
         if (tid == 0) {
             bool miss = false;
@@ -640,8 +663,12 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
                         const int lc = sg.c[k];
-                        bad |= lc < 0 || lc > 6;
-                        l[k] = min(max(lc, 0), 6);
+                        // the binned cell (2..5 in the frame of the block that binned the
+                        // particle) can move by one in this block's frame through FP32
+                        // rounding of + 0.25, so the bases lie in [1,6] (grid 0), [0,6] (grid 1)
+                        const int lo = g ? 0 : 1;
+                        bad |= lc < lo || lc > 6;
+                        l[k] = min(max(lc, lo), 6) - lo;
                         // D closed form per axis: second moment on the diagonal, products of
                         // first moments off the diagonal (units of dx^2)
                         const float w0 = sg.w0[k], w1 = 1.0f - w0, tt = sg.t[k];
@@ -653,7 +680,8 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
                     D[5] += m1[1] * m1[2];
                     // B = sum_i v_i (x_i - x_p)^T w_i with x_i - x_p = (o - t) dx per axis:
                     // B_rc = dx (S_rc - vg_r t_c), S_rc = sum over the nodes with o_c = 1 of w v_r
-                    const float* base = sV + g * kWin + l[0] * 64 + l[1] * 8 + l[2];
+                    const int s1 = g ? 8 : 7, s2 = g ? 64 : 49;
+                    const float* base = sV + (g ? kM0Win : 0) + l[0] * s2 + l[1] * s1 + l[2];
                     float vg[3] = {0.f, 0.f, 0.f}, S[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     float wxy[4];
 #pragma unroll
@@ -663,8 +691,8 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
                     for (int o = 0; o < 8; ++o) {
                         const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
                         const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
-                        const int off = ox * 64 + oy * 8 + oz;
-                        const float wv[3] = {w * base[off], w * base[2 * kWin + off], w * base[4 * kWin + off]};
+                        const int off = ox * s2 + oy * s1 + oz;
+                        const float wv[3] = {w * base[off], w * base[kMWin + off], w * base[2 * kMWin + off]};
 #pragma unroll
                         for (int r = 0; r < 3; ++r) {
                             vg[r] += wv[r];
@@ -794,7 +822,9 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
 #pragma unroll
                     for (int g = 0; g < 2; ++g) {
                         const Stencil32& sg = sn[g];
-                        LdsAcc* base = sM + g * kWin + sg.c[0] * 64 + sg.c[1] * 8 + sg.c[2];
+                        // grid 0: nodes [1,7]^3 (7 wide), grid 1: nodes [0,7]^3 (8 wide)
+                        const int s1 = g ? 8 : 7, s2 = g ? 64 : 49, lo = g ? 0 : 1;
+                        LdsAcc* base = sM + (g ? kM0Win : 0) + (sg.c[0] - lo) * s2 + (sg.c[1] - lo) * s1 + (sg.c[2] - lo);
                         float pa[3];
 #pragma unroll
                         for (int r = 0; r < 3; ++r)
@@ -807,7 +837,7 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
                         for (int o = 0; o < 8; ++o) {
                             const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
                             const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
-                            const int off = ox * 64 + oy * 8 + oz;
+                            const int off = ox * s2 + oy * s1 + oz;
                             LdsAdd(base + off, w * a.mat.mass);
 #pragma unroll
                             for (int r = 0; r < 3; ++r) {
@@ -815,7 +845,7 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
                                 if (ox) pr += kc[0][r];
                                 if (oy) pr += kc[1][r];
                                 if (oz) pr += kc[2][r];
-                                LdsAdd(base + (2 * (r + 1)) * kWin + off, w * pr);
+                                LdsAdd(base + (r + 1) * kMWin + off, w * pr);
                             }
                         }
                     }
@@ -842,11 +872,12 @@ __global__ __launch_bounds__(THREADS) void G2P2GKernel(G2P2GArgs a) {
             __syncthreads();
         }
         // ---------------- flush: one global atomic per non-zero node and channel
-        for (int i = tid; i < kMChannels * 2 * kWin; i += THREADS) {
+        for (int i = tid; i < kMChannels * kMWin; i += THREADS) {
             const float val = static_cast<float>(sM[i]);
             if (val == 0.0f) continue;
-            const int ch = i / (2 * kWin), g = (i / kWin) & 1, l = i & (kWin - 1);
-            const int lx = l >> 6, ly = (l >> 3) & 7, lz = l & 7;
+            const int ch = i / kMWin, r = i % kMWin;
+            int g, lx, ly, lz;
+            WindowNode(r, g, lx, ly, lz);
             const int o = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
             const uint32_t bi = sHalo[o];
             if (bi == kEmpty) continue;
