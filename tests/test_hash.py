@@ -212,3 +212,32 @@ def test_gpu_rejects_duplicates_and_outside_keys():
     g.Insert(np.array([7, 1], np.uint32))
     np.testing.assert_array_equal(g.Find(np.array([1, 7, 2], np.uint32)), [1, 0, EMPTY])
     g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_layout_hash_occupancy_every_substep():
+    """Neo-Hookean drop: every substep, the HIP build of the hash over the engine's
+    exterior block list has exactly the oracle's slots, stash and occupancy for the same
+    list (the lists themselves equal the oracle's bit for bit in the parity tests)."""
+    from ckmpm_amd import scenes
+    from ckmpm_amd.engine import engine_from_scene
+    from ckmpm_amd.hashtable import BlockHashTable
+
+    sc = scenes.jelly_drop(side_cells=12)
+    eng = engine_from_scene(sc)
+    cap = sc.block_capacity()
+    g = BlockHashTable(sc.dim, cap)
+    ho = HashOracle(sc.dim, cap)
+    for k in range(8):
+        eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        eng.Synchronize()
+        keys_g, (A, Nb, Ex) = eng.layout(0)
+        vals = np.arange(Ex, dtype=np.uint32)
+        g.Insert(keys_g[:Ex], vals)
+        ho.build(keys_g[:Ex], vals)
+        slots, stash = g.export()
+        np.testing.assert_array_equal(slots, ho.slots, err_msg=f"substep {k}")
+        np.testing.assert_array_equal(stash, ho.stash[:ho.nstash])
+        occ = (ho.slots != U64_EMPTY).reshape(-1, 16).sum(1)
+        np.testing.assert_array_equal(np.array(g.info().fill_histogram), np.bincount(occ, minlength=17))
+    g.close()
