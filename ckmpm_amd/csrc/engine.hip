@@ -42,15 +42,14 @@ int ReadStatus(ckmpm_engine* e, Status& st, LayoutCounts& lc) {
     return CKMPM_OK;
 }
 
-// Neighbour and exterior extension of layout L (its A keys already written).
-void ExtendLayout(ckmpm_engine* e, LayoutBuf& L) {
+// Neighbour and exterior extension of layout L (its A keys already written, and in
+// the table unless insert_a).
+void ExtendLayout(ckmpm_engine* e, LayoutBuf& L, bool insert_a) {
     hipStream_t s = e->stream;
-    LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
-    for (int pass = 0; pass < 2; ++pass) {
-        LaunchMarkExtension(e->dom, L.keys, L.lc, e->cap, L.table, e->bitmap, pass == 0 ? 0 : -1, s);
-        ExclusiveScan(e->bitmap, e->word_off, nullptr, e->nwords, e->nwords, e->total, true, e->scan_tmp, s);
-        LaunchBitmapEmit(e->bitmap, e->nwords, e->word_off, e->total, L.keys, L.table, L.lc, e->cap, pass, e->status, s);
-    }
+    if (insert_a) LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
+    for (int pass = 0; pass < 2; ++pass)
+        LaunchExtensionPass(e->dom, L.keys, L.table, L.lc, e->cap, e->bitmap, e->emit_tiles, e->tile_cnt, pass,
+                            e->status, s);
 }
 
 void Mark(ckmpm_engine* e) {
@@ -128,23 +127,21 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         ModelBuf& m = e->models[i];
         rm.cnt[i] = m.cnt;
         rm.cnt_e[i] = m.cnt_e;
-        rm.cntn[i] = m.cntn;
         rm.base[i] = m.base;
         rm.off_next[i] = m.off[sn];
+        rm.off_cur[i] = m.off[sc];
+        rm.code[i] = m.code;
+        rm.perm_next[i] = m.perm[sn];
+        rm.n[i] = m.n;
     }
-    LaunchTableClear(P.keys, P.lc, e->cap, P.table, s);
-    LaunchRebinCount(e->dom, C.keys, C.table, C.lc, rm, e->occ, e->cap, s);
-    ExclusiveScan(e->occ, e->newidx, &C.lc->Ex, 0, e->cap, e->total, false, e->scan_tmp, s);
-    LaunchSetCount(P.lc, 0, e->total, e->cap, e->status, s);
-    LaunchRebinCompact(C.keys, C.lc, e->occ, e->newidx, rm, P.keys, e->cap, s);
-    for (int i = 0; i < rm.nmodels; ++i) {
-        ModelBuf& m = e->models[i];
-        ExclusiveScan(m.cntn, m.off[sn], &P.lc->A, 0, e->cap, nullptr, false, e->scan_tmp, s);
-        LaunchFinishOffsets(m.off[sn], P.lc, m.n, s);
-    }
+    // exterior blocks are about 1.4 A; the kernels loop over any count
+    LaunchRebinCount(e->dom, C.keys, C.table, C.lc, rm, e->occ, e->tsum, P.keys, P.lc, P.table, e->cap,
+                     2u * e->a_hint + 64u, s);
+    LaunchRebinScan(C.keys, C.lc, rm, e->tsum, e->newidx, P.keys, P.table, P.lc, e->cap, e->status, s);
     LaunchRebinBase(e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap, s);
-    for (ModelBuf& m : e->models) LaunchRebinScatter(C.lc, m.off[sc], m.code, m.base, m.perm[sn], m.n, grid, s);
-    ExtendLayout(e, P);
+    // a wavefront per (model, block), 4 per workgroup
+    LaunchRebinScatter(C.lc, rm, GridFor((e->a_hint + e->a_hint / 4) * static_cast<uint32_t>(rm.nmodels) / 4u + 8u), s);
+    ExtendLayout(e, P, false);
     Mark(e);
     e->cur = 1 - e->cur;
     e->sc = sn;
@@ -196,7 +193,10 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
     e->nwords = static_cast<uint32_t>((nblocks + 31) / 32);
     check(Alloc(e, &e->V, static_cast<size_t>(e->cap) * kVFloats));
     check(Alloc(e, &e->status, 1));
-    check(Alloc(e, &e->bitmap, e->nwords));
+    e->emit_tiles = (e->nwords + kEmitTileWords - 1) / kEmitTileWords;
+    check(Alloc(e, &e->bitmap, static_cast<size_t>(e->emit_tiles) * kEmitTileWords));
+    check(Alloc(e, &e->tile_cnt, 2 * static_cast<size_t>(e->emit_tiles)));
+    check(Alloc(e, &e->tsum, RebinTileSumWords(e->cap)));
     check(Alloc(e, &e->word_off, e->nwords));
     check(Alloc(e, &e->occ, e->cap));
     check(Alloc(e, &e->newidx, e->cap));
@@ -207,7 +207,9 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
             (void)hipMemsetAsync(e->L[i].table, 0xFF, nblocks * sizeof(uint32_t), e->stream);
             (void)hipMemsetAsync(e->L[i].lc, 0, sizeof(LayoutCounts), e->stream);
         }
-        (void)hipMemsetAsync(e->bitmap, 0, e->nwords * sizeof(uint32_t), e->stream);
+        (void)hipMemsetAsync(e->bitmap, 0, static_cast<size_t>(e->emit_tiles) * kEmitTileWords * sizeof(uint32_t),
+                             e->stream);
+        (void)hipMemsetAsync(e->tile_cnt, 0, 2 * static_cast<size_t>(e->emit_tiles) * sizeof(uint32_t), e->stream);
         (void)hipMemsetAsync(e->status, 0, sizeof(Status), e->stream);
         if (hipStreamSynchronize(e->stream) != hipSuccess) rc = Fail(e, CKMPM_ERR_HIP, "init");
     }
@@ -271,7 +273,6 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     CK_TRY(Alloc(e, &m.cnt, static_cast<size_t>(e->cap) * kDirections));
     CK_TRY(Alloc(e, &m.base, static_cast<size_t>(e->cap) * kDirections));
     CK_TRY(Alloc(e, &m.cnt_e, e->cap));
-    CK_TRY(Alloc(e, &m.cntn, e->cap));
     e->models.push_back(std::move(m));
     if (model_index) *model_index = static_cast<int32_t>(e->models.size() - 1);
     return CKMPM_OK;
@@ -320,7 +321,7 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
     LaunchBitmapToKeysA(e->bitmap, e->nwords, e->word_off, e->total, L.keys, L.lc, e->cap, e->status, s);
     for (size_t i = 0; i < e->models.size(); ++i)
         LaunchSetupOffsets(sorted[i], e->models[i].n, L.keys, L.lc, e->cap, e->models[i].off[0], s);
-    ExtendLayout(e, L);
+    ExtendLayout(e, L, true);
     CK_HIP(e, hipMemsetAsync(e->M[0], 0, static_cast<size_t>(e->cap) * kMFloats * sizeof(float), s));
     for (ModelBuf& m : e->models) {
         CK_HIP(e, hipMemcpyAsync(xyz, m.x0.data(), 3 * sizeof(float) * m.n, hipMemcpyHostToDevice, s));

@@ -22,7 +22,6 @@ struct ModelBuf {
     uint32_t* cnt = nullptr;
     uint32_t* base = nullptr;
     uint32_t* cnt_e = nullptr;
-    uint32_t* cntn = nullptr;
     int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
     int threads = 256;    // G2P2G workgroup size (256 or 512; ckmpm_autotune chooses)
     hipEvent_t tune_ev[2] = {nullptr, nullptr};  // around the G2P2G launch while tuning
@@ -63,6 +62,9 @@ struct ckmpm_engine {
     uint32_t* bitmap = nullptr;
     uint32_t nwords = 0;
     uint32_t* word_off = nullptr;
+    uint32_t emit_tiles = 0;          // bitmap tiles of kEmitTileWords words
+    uint32_t* tile_cnt = nullptr;     // [2][emit_tiles] newly marked bits per tile and pass
+    uint32_t* tsum = nullptr;         // rebin count tile sums
     uint32_t* occ = nullptr;
     uint32_t* newidx = nullptr;
     uint32_t* scan_tmp = nullptr;

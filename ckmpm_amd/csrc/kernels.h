@@ -53,16 +53,18 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
                      const uint32_t* table, const LayoutCounts* lc, float* M, Status* st, hipStream_t s);
 
 // ---------------------------------------------------------------- layout
-// Emit the set bits of a domain bitmap in ascending key order after position
-// base (= counts.A for the neighbour pass, counts.Nb for the exterior pass),
-// register them in the table and clear the bitmap.  pass 0 sets Nb, pass 1 sets Ex.
-void LaunchTableClear(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
+// Bitmap words per emit tile; the domain bitmap is allocated in whole tiles (zero
+// padded) and each extension pass has emit_tiles counters of newly set bits.
+constexpr uint32_t kEmitTileWords = 2048;
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
-void LaunchMarkExtension(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
-                         const uint32_t* table, uint32_t* bitmap, int lo, hipStream_t s);
-void LaunchBitmapEmit(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
-                      uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap, int pass, Status* st,
-                      hipStream_t s);
+// One extension pass: mark the candidates that are not in the table (pass 0: b + {0,1}^3
+// of the particle blocks, pass 1: b + {-1,0,1}^3), then emit them in ascending key order
+// after counts.A (pass 0, sets Nb) or counts.Nb (pass 1, sets Ex), registering them in
+// the table and clearing the bitmap.  tile_cnt holds 2 x emit_tiles counters, zero on
+// the first call (each pass clears the other's).
+void LaunchExtensionPass(const DomainDev& d, uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap,
+                         uint32_t* bitmap, uint32_t emit_tiles, uint32_t* tile_cnt, int pass, Status* st,
+                         hipStream_t s);
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
 
@@ -109,19 +111,27 @@ struct RebinModels {
     int nmodels;
     const uint32_t* cnt[kMaxModels];   // [A_cur][27] direction counts per old block
     uint32_t* cnt_e[kMaxModels];       // [Ex_cur] particles entering each exterior block
-    uint32_t* cntn[kMaxModels];        // [A_new] same, compacted
     uint32_t* base[kMaxModels];        // [A_cur][27] first destination of (old block, direction)
-    const uint32_t* off_next[kMaxModels];
+    uint32_t* off_next[kMaxModels];    // [A_new + 1] block offsets of the next layout
+    const uint32_t* off_cur[kMaxModels];
+    const uint32_t* code[kMaxModels];  // [n] move | rank << 5 per processing position
+    uint32_t* perm_next[kMaxModels];
+    uint32_t n[kMaxModels];
 };
+// count tile sums: kTileSums words per tile of 32 exterior blocks
+constexpr int kTileSums = 1 + kMaxModels;
+size_t RebinTileSumWords(uint32_t cap);
+// Entering counts per exterior block of the current layout (occ, cnt_e, tile sums);
+// also clears the next layout's table entries.  ex_hint: expected exterior blocks.
 void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                      const RebinModels& rm, uint32_t* occ, uint32_t cap, hipStream_t s);
-void LaunchRebinCompact(const uint32_t* keys, const LayoutCounts* lc, const uint32_t* occ, const uint32_t* newidx,
-                        const RebinModels& rm, uint32_t* keys_next, uint32_t cap, hipStream_t s);
+                      const RebinModels& rm, uint32_t* occ, uint32_t* tsum, const uint32_t* keys_next,
+                      const LayoutCounts* lc_next, uint32_t* table_next, uint32_t cap, uint32_t ex_hint,
+                      hipStream_t s);
+// Next layout's particle blocks (keys, table, A) and every model's block offsets.
+void LaunchRebinScan(const uint32_t* keys, const LayoutCounts* lc, const RebinModels& rm, const uint32_t* tsum, uint32_t* newidx, uint32_t* keys_next, uint32_t* table_next,
+                     LayoutCounts* lc_next, uint32_t cap, Status* st, hipStream_t s);
 void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
                      const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap, hipStream_t s);
-void LaunchRebinScatter(const LayoutCounts* lc, const uint32_t* off, const uint32_t* code, const uint32_t* base,
-                        uint32_t* perm_next, uint32_t n, uint32_t grid, hipStream_t s);
-void LaunchSetCount(LayoutCounts* lc, int field, const uint32_t* total, uint32_t cap, Status* st, hipStream_t s);
-void LaunchFinishOffsets(uint32_t* off, const LayoutCounts* lc, uint32_t n, hipStream_t s);
+void LaunchRebinScatter(const LayoutCounts* lc, const RebinModels& rm, uint32_t grid, hipStream_t s);
 
 }  // namespace ckmpm

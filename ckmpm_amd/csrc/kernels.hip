@@ -11,6 +11,8 @@
 //   grid velocity    V[block][ch 3][node 128]
 //   layout           keys[list index] -> linear block key, table[key] -> list index
 #include "kernels.h"
+
+#include <algorithm>
 #include "stencil.h"
 #include "material.h"
 
@@ -19,6 +21,13 @@ namespace ckmpm {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kEmitItems = 8;                          // bitmap words per thread of the emit kernel
+static_assert(kEmitTileWords == kThreads * kEmitItems, "emit tile");
+constexpr int kCountThreads = 1024;                    // rebin count workgroup
+constexpr int kCountTile = kCountThreads / 32;         // exterior blocks per count tile
+constexpr int kRebinScanItems = 8;
+constexpr uint32_t kRebinScanTile = kThreads * kRebinScanItems;
+static_assert(kRebinScanTile % kCountTile == 0, "scan tiles hold whole count tiles");
 
 __device__ __forceinline__ uint32_t LaneId() { return __lane_id(); }
 
@@ -357,20 +366,20 @@ __global__ __launch_bounds__(kThreads) void ExportPositionsKernel(DomainDev d, c
 }
 
 // ============================================================== layout kernels
-__global__ void TableClearKernel(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table) {
-    const uint32_t ex = min(lc->Ex, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ex; i += gridDim.x * blockDim.x) table[keys[i]] = kEmpty;
-}
-
 __global__ void TableInsertAKernel(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table) {
     const uint32_t A = min(lc->A, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) table[keys[i]] = i;
 }
 
 // Neighbour (lo = 0: b + {0,1}^3) or exterior (lo = -1: b + {-1,0,1}^3) candidates of
-// the particle blocks that are not yet in the table: mpm_algorithm.cuh:657-695.
+// the particle blocks that are not yet in the table: mpm_algorithm.cuh:657-695.  Each
+// bit this kernel sets is counted in tile_cnt[word / kEmitTileWords], so that the emit
+// kernel knows every tile's offset without a separate scan pass over the bitmap.  The
+// counts are aggregated per wavefront (its candidates span one to three tiles): one
+// atomic per distinct tile instead of one per bit, which would serialize on the few
+// counter addresses.
 __global__ void MarkExtensionKernel(DomainDev d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
-                                    const uint32_t* table, uint32_t* bitmap, int lo) {
+                                    const uint32_t* table, uint32_t* bitmap, uint32_t* tile_cnt, int lo) {
     const uint32_t A = min(lc->A, cap);
     const int span = 2 - lo;
     const int per = span * span * span;
@@ -383,9 +392,104 @@ __global__ void MarkExtensionKernel(DomainDev d, const uint32_t* keys, const Lay
         bx += o / (span * span) + lo;
         by += (o / span) % span + lo;
         bz += o % span + lo;
-        if (!InDomain(d, bx, by, bz)) continue;
-        const uint32_t key = KeyOf(d, bx, by, bz);
-        if (table[key] == kEmpty) atomicOr(&bitmap[key >> 5], 1u << (key & 31u));
+        uint32_t tile = kEmpty;  // tile of a bit set by this lane
+        if (InDomain(d, bx, by, bz)) {
+            const uint32_t key = KeyOf(d, bx, by, bz);
+            if (table[key] == kEmpty) {
+                const uint32_t bit = 1u << (key & 31u);
+                if (!(atomicOr(&bitmap[key >> 5], bit) & bit)) tile = (key >> 5) / kEmitTileWords;
+            }
+        }
+        uint64_t pending = __ballot(tile != kEmpty);
+        while (pending) {
+            const int leader = __builtin_ctzll(pending);
+            const uint32_t t = __shfl(tile, leader, 64);
+            const uint64_t same = __ballot(tile == t);
+            if (LaneId() == static_cast<uint32_t>(leader)) atomicAdd(&tile_cnt[t], static_cast<uint32_t>(__popcll(same)));
+            pending &= ~same;
+        }
+    }
+}
+
+// Reduce per-thread values over a workgroup into red[0, nq) (all threads see the result
+// after the call).  red must hold nq entries.
+template <int NQ>
+__device__ __forceinline__ void BlockSums(const uint32_t (&v)[NQ], int nq, uint32_t* red) {
+    __syncthreads();  // earlier reads of red are done
+    if (threadIdx.x < static_cast<uint32_t>(nq)) red[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        if (j < nq) {
+            uint32_t x = v[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            if (LaneId() == 0 && x) atomicAdd(&red[j], x);
+        }
+    }
+    __syncthreads();
+}
+
+// Extension pass (0: neighbours after A, sets Nb; 1: exterior after Nb, sets Ex) in one
+// kernel: workgroup w owns bitmap words [w, w+1) * kEmitTileWords, takes its first
+// position from the tile counts of the marking kernel, scans its words' popcounts and
+// writes the set bits in ascending key order to keys[] and the table, clearing the
+// bitmap.  Tiles without a set bit return at once.  Workgroup 0 also sets the count
+// and clears the other pass's tile counts (zero again before that pass marks).
+__global__ __launch_bounds__(kThreads) void BitmapScanEmitKernel(uint32_t* bitmap, const uint32_t* tile_cnt,
+                                                                 uint32_t* tile_cnt_other, uint32_t ntiles,
+                                                                 uint32_t* keys, uint32_t* table, LayoutCounts* lc,
+                                                                 uint32_t cap, int pass, Status* st) {
+    __shared__ uint32_t sh[kThreads / 64];
+    __shared__ uint32_t red[1];
+    const uint32_t w = blockIdx.x, tid = threadIdx.x;
+    const uint32_t base = pass == 0 ? lc->A : lc->Nb;
+    if (w == 0) {
+        uint32_t v[1] = {0u};
+        for (uint32_t t = tid; t < ntiles; t += kThreads) v[0] += tile_cnt[t];
+        BlockSums<1>(v, 1, red);
+        if (tid == 0) {
+            uint32_t c = base + red[0];
+            if (c > cap) {
+                SetFlag(st, kFlagCapacity);
+                c = cap;
+            }
+            if (pass == 0) lc->Nb = c;
+            else lc->Ex = c;
+        }
+        for (uint32_t t = tid; t < ntiles; t += kThreads) tile_cnt_other[t] = 0u;
+    }
+    if (tile_cnt[w] == 0u) return;
+    uint32_t v[1] = {0u};
+    for (uint32_t t = tid; t < w; t += kThreads) v[0] += tile_cnt[t];
+    BlockSums<1>(v, 1, red);
+    const uint32_t w0 = w * kEmitTileWords + tid * kEmitItems;
+    const uint4* src = reinterpret_cast<const uint4*>(bitmap + w0);
+    const uint4 qa = src[0], qb = src[1];
+    const uint32_t words[kEmitItems] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+    uint32_t pc = 0;
+#pragma unroll
+    for (int k = 0; k < kEmitItems; ++k) pc += static_cast<uint32_t>(__popc(words[k]));
+    uint32_t tot;
+    uint32_t pos = base + red[0] + BlockExclusiveScan(pc, tot, sh);
+    if (pc == 0) return;
+#pragma unroll
+    for (int k = 0; k < kEmitItems; ++k) {
+        uint32_t bits = words[k];
+        if (!bits) continue;
+        bitmap[w0 + k] = 0u;
+        while (bits) {
+            const uint32_t b = static_cast<uint32_t>(__ffs(bits) - 1);
+            bits &= bits - 1u;
+            const uint32_t key = (w0 + k) * 32u + b;
+            if (pos < cap) {
+                keys[pos] = key;
+                table[key] = pos;
+            } else {
+                SetFlag(st, kFlagCapacity);
+            }
+            ++pos;
+        }
     }
 }
 
@@ -898,44 +1002,156 @@ __device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
 
 // Particles entering each exterior block e: sum over the 27 possible source blocks
 // (e - delta, particle blocks of the current layout) of their count for move delta.
-__global__ void RebinCountKernel(DomainDev d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                                 RebinModels rm, uint32_t* occ, uint32_t cap) {
-    // a 32-lane group per exterior block, lane = direction: 27 independent lookups
-    // instead of a dependent chain of 27 per thread
+// A 32-lane group per exterior block, lane = direction (27 independent lookups instead
+// of a dependent chain of 27 per thread); a workgroup owns a tile of kCountTile
+// consecutive exterior blocks and writes the tile's sums (occupied blocks, entering
+// particles per model) for the scan kernel.  The same launch clears the next layout's
+// table entries (its keys are still those of two substeps ago).
+__global__ __launch_bounds__(kCountThreads) void RebinCountKernel(DomainDev d, const uint32_t* keys,
+                                                                  const uint32_t* table, const LayoutCounts* lc,
+                                                                  RebinModels rm, uint32_t* occ, uint32_t* tsum,
+                                                                  const uint32_t* keys_next,
+                                                                  const LayoutCounts* lc_next, uint32_t* table_next,
+                                                                  uint32_t cap) {
+    __shared__ uint32_t sG[kCountTile][kTileSums];
+    const uint32_t exn = min(lc_next->Ex, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < exn; i += gridDim.x * blockDim.x)
+        table_next[keys_next[i]] = kEmpty;
     const uint32_t A = lc->A, Ex = min(lc->Ex, cap);
-    const uint32_t lane = threadIdx.x & 31u;
-    const uint32_t grp = (blockIdx.x * blockDim.x + threadIdx.x) >> 5, ngrp = (gridDim.x * blockDim.x) >> 5;
-    for (uint32_t e = grp; e < Ex; e += ngrp) {
-        int bx, by, bz;
-        CoordsOf(d, keys[e], bx, by, bz);
-        uint32_t js = kEmpty;
-        if (lane < static_cast<uint32_t>(kDirections)) {
-            int ox, oy, oz;
-            DirOffset(static_cast<int>(lane), ox, oy, oz);
-            const int sx = bx - ox, sy = by - oy, sz = bz - oz;
-            if (InDomain(d, sx, sy, sz)) js = table[KeyOf(d, sx, sy, sz)];
-        }
-        uint32_t any = 0;
-        for (int m = 0; m < rm.nmodels; ++m) {
-            uint32_t v = js < A ? rm.cnt[m][static_cast<size_t>(js) * kDirections + lane] : 0u;  // also kEmpty
+    const uint32_t lane = threadIdx.x & 31u, grp = threadIdx.x >> 5;
+    const int nq = 1 + rm.nmodels;
+    const uint32_t ntiles = (Ex + kCountTile - 1) / kCountTile;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t e = tile * kCountTile + grp;
+        if (e < Ex) {
+            int bx, by, bz;
+            CoordsOf(d, keys[e], bx, by, bz);
+            uint32_t js = kEmpty;
+            if (lane < static_cast<uint32_t>(kDirections)) {
+                int ox, oy, oz;
+                DirOffset(static_cast<int>(lane), ox, oy, oz);
+                const int sx = bx - ox, sy = by - oy, sz = bz - oz;
+                if (InDomain(d, sx, sy, sz)) js = table[KeyOf(d, sx, sy, sz)];
+            }
+            uint32_t any = 0;
+            for (int m = 0; m < rm.nmodels; ++m) {
+                uint32_t v = js < A ? rm.cnt[m][static_cast<size_t>(js) * kDirections + lane] : 0u;  // also kEmpty
 #pragma unroll
-            for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
-            if (lane == 0) rm.cnt_e[m][e] = v;
-            any |= v;
+                for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                if (lane == 0) {
+                    rm.cnt_e[m][e] = v;
+                    sG[grp][1 + m] = v;
+                }
+                any |= v;
+            }
+            if (lane == 0) {
+                occ[e] = any ? 1u : 0u;
+                sG[grp][0] = any ? 1u : 0u;
+            }
+        } else if (lane < static_cast<uint32_t>(nq)) {
+            sG[grp][lane] = 0u;
         }
-        if (lane == 0) occ[e] = any ? 1u : 0u;
+        __syncthreads();
+        if (threadIdx.x < static_cast<uint32_t>(nq)) {
+            uint32_t sum = 0;
+            for (int g = 0; g < kCountTile; ++g) sum += sG[g][threadIdx.x];
+            tsum[static_cast<size_t>(tile) * kTileSums + threadIdx.x] = sum;
+        }
+        __syncthreads();
     }
 }
 
-__global__ void RebinCompactKernel(const uint32_t* keys, const LayoutCounts* lc, const uint32_t* occ,
-                                   const uint32_t* newidx, RebinModels rm, uint32_t* keys_next, uint32_t cap) {
-    const uint32_t Ex = min(lc->Ex, cap);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < Ex; e += gridDim.x * blockDim.x) {
-        if (!occ[e]) continue;
-        const uint32_t i = newidx[e];
-        if (i >= cap) continue;
-        keys_next[i] = keys[e];
-        for (int m = 0; m < rm.nmodels; ++m) rm.cntn[m][i] = rm.cnt_e[m][e];
+// Next layout's particle blocks and particle offsets in one pass: workgroup w owns
+// exterior blocks [w, w+1) * kRebinScanTile of the current layout, takes the sums of
+// the count tiles before it, and scans the occupied flags (new list index of each
+// occupied block; its key goes to keys_next and the table) and each model's entering
+// counts (offsets of the next layout).  Unoccupied blocks enter no particle of any
+// model, so the scan over all exterior blocks equals the scan over the compacted
+// list.  Workgroup 0 also sets A of the next layout and the end offsets (= n).
+__global__ __launch_bounds__(kThreads) void RebinScanKernel(const uint32_t* keys, const LayoutCounts* lc,
+                                                            RebinModels rm, const uint32_t* tsum, uint32_t* newidx,
+                                                            uint32_t* keys_next, uint32_t* table_next,
+                                                            LayoutCounts* lc_next, uint32_t cap, Status* st) {
+    __shared__ uint32_t sh[kThreads / 64];
+    __shared__ uint32_t red[kTileSums];
+    __shared__ uint32_t sPre[kTileSums];
+    const uint32_t Ex = min(lc->Ex, cap), tid = threadIdx.x;
+    const int nq = 1 + rm.nmodels;
+    const uint32_t base = blockIdx.x * kRebinScanTile;
+    if (base >= Ex && blockIdx.x != 0) return;
+    // sums of the count tiles before this tile (workgroup 0: of all tiles, the totals)
+    const uint32_t ntc = (Ex + kCountTile - 1) / kCountTile;
+    const uint32_t upto = blockIdx.x == 0 ? ntc : min(ntc, base / kCountTile);
+    // every global load of the workgroup is issued before the first barrier: the count
+    // tiles before this tile, and the keys and entering counts of the thread's blocks
+    // (thread i owns [base + i*8, base + i*8 + 8); a block is occupied iff a model's
+    // particles enter it)
+    uint32_t v[kTileSums];
+#pragma unroll
+    for (int j = 0; j < kTileSums; ++j) v[j] = 0u;
+    for (uint32_t t = tid; t < upto; t += kThreads) {
+#pragma unroll
+        for (int j = 0; j < kTileSums; ++j)
+            if (j < nq) v[j] += tsum[static_cast<size_t>(t) * kTileSums + j];
+    }
+    const uint32_t e0 = base + tid * kRebinScanItems;
+    uint32_t c[kMaxModels][kRebinScanItems], key[kRebinScanItems];
+#pragma unroll
+    for (int k = 0; k < kRebinScanItems; ++k) key[k] = e0 + k < Ex ? keys[e0 + k] : 0u;
+#pragma unroll
+    for (int m = 0; m < kMaxModels; ++m)
+#pragma unroll
+        for (int k = 0; k < kRebinScanItems; ++k) c[m][k] = m < rm.nmodels && e0 + k < Ex ? rm.cnt_e[m][e0 + k] : 0u;
+    BlockSums<kTileSums>(v, nq, red);
+    if (tid < static_cast<uint32_t>(nq)) sPre[tid] = blockIdx.x == 0 ? 0u : red[tid];
+    if (blockIdx.x == 0 && tid == 0) {
+        uint32_t a = red[0];
+        if (a > cap) {
+            SetFlag(st, kFlagCapacity);
+            a = cap;
+        }
+        lc_next->A = a;
+        for (int m = 0; m < rm.nmodels; ++m) rm.off_next[m][a] = rm.n[m];
+    }
+    __syncthreads();
+    uint32_t omask = 0;
+#pragma unroll
+    for (int k = 0; k < kRebinScanItems; ++k) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int m = 0; m < kMaxModels; ++m) any |= c[m][k];
+        if (any) omask |= 1u << k;
+    }
+    uint32_t tot;
+    const uint32_t i0 = sPre[0] + BlockExclusiveScan(static_cast<uint32_t>(__popc(omask)), tot, sh);
+    uint32_t i = i0;
+#pragma unroll
+    for (int k = 0; k < kRebinScanItems; ++k) {
+        if (!((omask >> k) & 1u)) continue;
+        if (i < cap) {
+            keys_next[i] = key[k];
+            table_next[key[k]] = i;
+            newidx[e0 + k] = i;
+        }
+        ++i;
+    }
+#pragma unroll
+    for (int m = 0; m < kMaxModels; ++m) {
+        if (m < rm.nmodels) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int k = 0; k < kRebinScanItems; ++k) sum += c[m][k];
+            uint32_t run = sPre[1 + m] + BlockExclusiveScan(sum, tot, sh);
+            i = i0;
+#pragma unroll
+            for (int k = 0; k < kRebinScanItems; ++k) {
+                if ((omask >> k) & 1u) {
+                    if (i < cap) rm.off_next[m][i] = run;
+                    ++i;
+                }
+                run += c[m][k];
+            }
+        }
     }
 }
 
@@ -979,24 +1195,38 @@ __global__ __launch_bounds__(kThreads) void RebinBaseKernel(DomainDev d, const u
     }
 }
 
-__global__ __launch_bounds__(kThreads) void RebinScatterKernel(const LayoutCounts* lc, const uint32_t* off,
-                                                               const uint32_t* code, const uint32_t* base,
-                                                               uint32_t* perm_next, uint32_t n) {
+// Particle permutation of the next layout for every model: a wavefront per (model,
+// block) pair, kScatterUnroll particles per lane in flight (a block holds a few hundred
+// particles per model, so most pairs take one pass of dependent loads).
+constexpr int kScatterUnroll = 4;
+__global__ __launch_bounds__(kThreads) void RebinScatterKernel(const LayoutCounts* lc, RebinModels rm) {
     const uint32_t A = lc->A;
-    for (uint32_t j = blockIdx.x; j < A; j += gridDim.x) {
+    const uint32_t pairs = A * static_cast<uint32_t>(rm.nmodels);
+    const uint32_t lane = LaneId(), waves = kThreads / 64;
+    for (uint32_t p = blockIdx.x * waves + (threadIdx.x >> 6); p < pairs; p += gridDim.x * waves) {
+        const uint32_t m = p / A, j = p - m * A;
+        const uint32_t* off = rm.off_cur[m];
         const uint32_t beg = off[j], end = off[j + 1];
-        const uint32_t* bj = base + static_cast<size_t>(j) * kDirections;
-        for (uint32_t t = beg + threadIdx.x; t < end; t += kThreads) {
-            const uint32_t c = code[t];
-            if ((c & 31u) >= static_cast<uint32_t>(kDirections)) continue;
-            const uint32_t pos = bj[c & 31u] + (c >> 5);
-            if (pos < n) perm_next[pos] = t | ((c & 31u) << kMoveShift);
+        const uint32_t* code = rm.code[m];
+        const uint32_t* bj = rm.base[m] + static_cast<size_t>(j) * kDirections;
+        uint32_t* perm_next = rm.perm_next[m];
+        const uint32_t n = rm.n[m];
+        for (uint32_t t0 = beg + lane; t0 < end; t0 += kScatterUnroll * 64) {
+            uint32_t c[kScatterUnroll];
+#pragma unroll
+            for (int u = 0; u < kScatterUnroll; ++u) {
+                const uint32_t t = t0 + u * 64;
+                c[u] = t < end ? code[t] : 31u;
+            }
+#pragma unroll
+            for (int u = 0; u < kScatterUnroll; ++u) {
+                const uint32_t dir = c[u] & 31u;
+                if (dir >= static_cast<uint32_t>(kDirections)) continue;
+                const uint32_t pos = bj[dir] + (c[u] >> 5);
+                if (pos < n) perm_next[pos] = (t0 + u * 64) | (dir << kMoveShift);
+            }
         }
     }
-}
-
-__global__ void FinishOffsetsKernel(uint32_t* off, const LayoutCounts* lc, uint32_t n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) off[lc->A] = n;
 }
 
 // rank of each element inside its run of equal keys (from run heads and their positions)
@@ -1018,6 +1248,7 @@ uint32_t Blocks(uint64_t n, uint32_t threads, uint32_t maxb = 4096) {
 // ============================================================== launchers
 uint32_t LaunchBlocks(uint64_t n) { return Blocks(n, kThreads); }
 size_t ScanTempWords(size_t n_cap) { return (n_cap + kScanTile - 1) / kScanTile + 1; }
+size_t RebinTileSumWords(uint32_t cap) { return (static_cast<size_t>(cap) / kCountTile + 1) * kTileSums; }
 
 void ExclusiveScan(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_imm, uint32_t n_cap,
                    uint32_t* total, bool popcount, uint32_t* tmp, hipStream_t s) {
@@ -1101,24 +1332,18 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
                      const uint32_t* table, const LayoutCounts* lc, float* M, Status* st, hipStream_t s) {
     RasterizeKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, xyz, vel, n, nvel, mass, table, lc, M, st);
 }
-void LaunchTableClear(const uint32_t* keys, const LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s) {
-    TableClearKernel<<<Blocks(cap, kThreads, 1024), kThreads, 0, s>>>(keys, lc, cap, table);
-}
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s) {
     TableInsertAKernel<<<Blocks(cap, kThreads, 1024), kThreads, 0, s>>>(keys, lc, cap, table);
 }
-void LaunchMarkExtension(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
-                         const uint32_t* table, uint32_t* bitmap, int lo, hipStream_t s) {
-    const uint32_t per = lo == 0 ? 8 : 27;
-    MarkExtensionKernel<<<Blocks(static_cast<uint64_t>(cap) * per, kThreads), kThreads, 0, s>>>(d, keys, lc, cap, table,
-                                                                                                 bitmap, lo);
-}
-void LaunchBitmapEmit(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
-                      uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap, int pass, Status* st,
-                      hipStream_t s) {
-    BitmapEmitKernel<<<Blocks(nwords, kThreads), kThreads, 0, s>>>(bitmap, nwords, word_off, keys, table, lc, cap, pass,
-                                                                    st);
-    SetCountKernel<<<1, 64, 0, s>>>(lc, pass + 1, total, cap, st);
+void LaunchExtensionPass(const DomainDev& d, uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap,
+                         uint32_t* bitmap, uint32_t emit_tiles, uint32_t* tile_cnt, int pass, Status* st,
+                         hipStream_t s) {
+    const uint32_t per = pass == 0 ? 8 : 27;
+    MarkExtensionKernel<<<Blocks(static_cast<uint64_t>(cap) * per, kThreads), kThreads, 0, s>>>(
+        d, keys, lc, cap, table, bitmap, tile_cnt + pass * emit_tiles, pass == 0 ? 0 : -1);
+    BitmapScanEmitKernel<<<emit_tiles, kThreads, 0, s>>>(bitmap, tile_cnt + pass * emit_tiles,
+                                                         tile_cnt + (1 - pass) * emit_tiles, emit_tiles, keys, table,
+                                                         lc, cap, pass, st);
 }
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s) {
@@ -1145,26 +1370,26 @@ void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s) 
     else LaunchG2P2GT<256>(a, grid, s);
 }
 void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                      const RebinModels& rm, uint32_t* occ, uint32_t cap, hipStream_t s) {
-    RebinCountKernel<<<Blocks(32ull * cap, kThreads), kThreads, 0, s>>>(d, keys, table, lc, rm, occ, cap);
+                      const RebinModels& rm, uint32_t* occ, uint32_t* tsum, const uint32_t* keys_next,
+                      const LayoutCounts* lc_next, uint32_t* table_next, uint32_t cap, uint32_t ex_hint,
+                      hipStream_t s) {
+    // one tile per workgroup for up to ex_hint exterior blocks; the loops cover any count
+    const uint32_t grid = Blocks(std::min(ex_hint, cap), kCountTile, 2048);
+    RebinCountKernel<<<grid, kCountThreads, 0, s>>>(d, keys, table, lc, rm, occ, tsum, keys_next, lc_next, table_next,
+                                                    cap);
 }
-void LaunchRebinCompact(const uint32_t* keys, const LayoutCounts* lc, const uint32_t* occ, const uint32_t* newidx,
-                        const RebinModels& rm, uint32_t* keys_next, uint32_t cap, hipStream_t s) {
-    RebinCompactKernel<<<Blocks(cap, kThreads), kThreads, 0, s>>>(keys, lc, occ, newidx, rm, keys_next, cap);
+void LaunchRebinScan(const uint32_t* keys, const LayoutCounts* lc, const RebinModels& rm, const uint32_t* tsum, uint32_t* newidx, uint32_t* keys_next, uint32_t* table_next,
+                     LayoutCounts* lc_next, uint32_t cap, Status* st, hipStream_t s) {
+    const uint32_t grid = (cap + kRebinScanTile - 1) / kRebinScanTile;
+    RebinScanKernel<<<grid ? grid : 1, kThreads, 0, s>>>(keys, lc, rm, tsum, newidx, keys_next, table_next, lc_next,
+                                                         cap, st);
 }
 void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
                      const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap, hipStream_t s) {
     RebinBaseKernel<<<Blocks(cap, 8), kThreads, 0, s>>>(d, keys, table, lc, occ, newidx, rm, cap);
 }
-void LaunchRebinScatter(const LayoutCounts* lc, const uint32_t* off, const uint32_t* code, const uint32_t* base,
-                        uint32_t* perm_next, uint32_t n, uint32_t grid, hipStream_t s) {
-    RebinScatterKernel<<<grid, kThreads, 0, s>>>(lc, off, code, base, perm_next, n);
-}
-void LaunchSetCount(LayoutCounts* lc, int field, const uint32_t* total, uint32_t cap, Status* st, hipStream_t s) {
-    SetCountKernel<<<1, 64, 0, s>>>(lc, field, total, cap, st);
-}
-void LaunchFinishOffsets(uint32_t* off, const LayoutCounts* lc, uint32_t n, hipStream_t s) {
-    FinishOffsetsKernel<<<1, 64, 0, s>>>(off, lc, n);
+void LaunchRebinScatter(const LayoutCounts* lc, const RebinModels& rm, uint32_t grid, hipStream_t s) {
+    RebinScatterKernel<<<grid, kThreads, 0, s>>>(lc, rm);
 }
 
 }  // namespace ckmpm
