@@ -36,10 +36,16 @@ struct LayoutCounts {
 };
 
 // Device-resident engine status.
+constexpr int kMaxModels = 8;
+
 struct Status {
     uint32_t flags;
+    uint32_t p2g_global;  // particles deposited through the FP32 global-atomic path
     uint32_t vmax2_bits;  // max |v|^2 of the last grid update (non-negative float bits)
-    uint32_t pad[2];
+    uint32_t pad;
+    // per model: largest P2G deposit bound of the previous substep (float bits), reduced
+    // by the grid update from the per-block maxima; sets the fixed-point scale of P2G
+    uint32_t dep_bits[kMaxModels];
 };
 
 struct DomainDev {

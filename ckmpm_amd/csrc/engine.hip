@@ -82,6 +82,8 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
     g.mass_clamp = e->sim.mass_clamp;
     g.cap = e->cap;
     g.st = e->status;
+    g.nmodels = static_cast<int>(e->models.size());
+    for (int i = 0; i < g.nmodels; ++i) g.blkmax[i] = e->models[i].blkmax;
     // 2 blocks x 4 chains per workgroup and pass: 0.19 A workgroups cover Nb (about 1.2 A)
     LaunchGridUpdate(g, GridFor(e->a_hint * 3u / 16u + 8u), s);
     Mark(e);
@@ -95,7 +97,8 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
     float* Mout = e->M[1 - e->mprev];
     const int sc = e->sc, sn = 1 - e->sc;
     const uint32_t grid = GridFor(e->a_hint + e->a_hint / 4 + 8);
-    for (ModelBuf& m : e->models) {
+    for (size_t mi = 0; mi < e->models.size(); ++mi) {
+        ModelBuf& m = e->models[mi];
         G2P2GArgs a{};
         a.dom = e->dom;
         a.mat = m.mat;
@@ -115,6 +118,8 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         a.dt = static_cast<float>(dt);
         a.next_dt = static_cast<float>(next_dt);
         a.st = e->status;
+        a.model = static_cast<int>(mi);
+        a.blkmax = m.blkmax;
         if (e->tuning) (void)hipEventRecord(m.tune_ev[0], s);
         LaunchG2P2G(a, grid, m.threads, s);
         if (e->tuning) (void)hipEventRecord(m.tune_ev[1], s);
@@ -273,6 +278,8 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     CK_TRY(Alloc(e, &m.cnt, static_cast<size_t>(e->cap) * kDirections));
     CK_TRY(Alloc(e, &m.base, static_cast<size_t>(e->cap) * kDirections));
     CK_TRY(Alloc(e, &m.cnt_e, e->cap));
+    CK_TRY(Alloc(e, &m.blkmax, e->cap));
+    CK_HIP(e, hipMemsetAsync(m.blkmax, 0, static_cast<size_t>(e->cap) * sizeof(float), e->stream));
     e->models.push_back(std::move(m));
     if (model_index) *model_index = static_cast<int32_t>(e->models.size() - 1);
     return CKMPM_OK;
@@ -473,14 +480,14 @@ int ckmpm_get_counters(ckmpm_engine* e, ckmpm_counters* out) {
     out->exterior_blocks = lc.Ex;
     out->flags = st.flags;
     out->max_speed = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits));
-    out->reserved = 0;
+    out->p2g_global_deposits = st.p2g_global;
     out->substeps = e->substeps;
     return CKMPM_OK;
 }
 
 int ckmpm_clear_flags(ckmpm_engine* e) {
     if (!e) return CKMPM_ERR_INVALID;
-    CK_HIP(e, hipMemsetAsync(&e->status->flags, 0, sizeof(uint32_t), e->stream));
+    CK_HIP(e, hipMemsetAsync(&e->status->flags, 0, 2 * sizeof(uint32_t), e->stream));  // flags, p2g_global
     CK_HIP(e, hipStreamSynchronize(e->stream));
     return CKMPM_OK;
 }

@@ -22,6 +22,7 @@ struct ModelBuf {
     uint32_t* cnt = nullptr;
     uint32_t* base = nullptr;
     uint32_t* cnt_e = nullptr;
+    float* blkmax = nullptr;  // [cap] largest P2G deposit bound per particle block
     int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
     int threads = 256;    // G2P2G workgroup size (256 or 512; ckmpm_autotune chooses)
     hipEvent_t tune_ev[2] = {nullptr, nullptr};  // around the G2P2G launch while tuning

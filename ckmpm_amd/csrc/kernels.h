@@ -5,7 +5,6 @@
 
 namespace ckmpm {
 
-constexpr int kMaxModels = 8;
 
 // ---------------------------------------------------------------- scans
 // Exclusive scan of n u32 values (n read from n_dev when non-null, else n_imm; the
@@ -81,6 +80,9 @@ struct GridUpdateArgs {
     float dt, gravity, mass_clamp;
     uint32_t cap;
     Status* st;
+    // per-block P2G deposit bounds of the previous layout's G2P2G, reduced into st->dep_bits
+    int nmodels;
+    const float* blkmax[kMaxModels];
 };
 void LaunchBeginSubstep(Status* st, hipStream_t s);
 void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s);
@@ -103,6 +105,8 @@ struct G2P2GArgs {
     int pp_attr;  // attribute index of per-particle (lambda, mu) in the state, 0: none
     float dt, next_dt;
     Status* st;
+    int model;      // index of st->dep_bits
+    float* blkmax;  // [A] largest deposit bound per particle block (for the next substep)
 };
 // threads per workgroup: 256 or 512 (one particle block per workgroup either way)
 void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s);

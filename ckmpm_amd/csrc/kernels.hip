@@ -536,14 +536,25 @@ __global__ void SetCountKernel(LayoutCounts* lc, int field, const uint32_t* tota
 
 // ============================================================== grid update
 __global__ void BeginSubstepKernel(Status* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->vmax2_bits = 0u;
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x == 0) st->vmax2_bits = 0u;
+    if (threadIdx.x < static_cast<uint32_t>(kMaxModels)) st->dep_bits[threadIdx.x] = 0u;
+}
+
+// atomicMax of a non-negative float (as bits) that skips the atomic when the stored
+// value is already at least v: the stored value only grows during a launch, so a
+// stale read can only cause an unneeded atomic, never a missed one.
+__device__ __forceinline__ void MaxBits(uint32_t* p, float v) {
+    if (v > 0.0f && __float_as_uint(v) > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(p, __float_as_uint(v));
 }
 
 // mpm_algorithm.cuh:738-869 fused with CopySelectedGridBlocks (:1920-1945): each grid
 // block of the current layout reads its mass / momentum from the previous layout
 // through the previous key table, writes velocities (p/m, gravity on y, sticky
 // boundary within 2 blocks of the domain edge), clears the next P2G target and
-// reduces max |v|^2 (NaN -> +inf).  128 threads per block, 2 blocks per workgroup.
+// reduces max |v|^2 (NaN -> +inf) and the P2G deposit bounds of the previous substep.
+// 128 threads per block, 2 blocks per workgroup.
 // Each thread serves one node of kGridIlp blocks at a time: the key -> table -> mass /
 // momentum load chains of those blocks are issued together, so a wave has kGridIlp
 // independent latency chains in flight instead of one.
@@ -613,16 +624,35 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
             vmax = fmaxf(vmax, v2);
         }
     }
+    // largest deposit bound per model over the previous layout's particle blocks
+    const uint32_t Ap = a.lc_prev->A;
+    float bmax[kMaxModels];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-    // one atomic per workgroup: the per-wave atomics on one word serialise
-    __shared__ float sMax[kThreads / 64];
-    if (LaneId() == 0) sMax[threadIdx.x >> 6] = vmax;
+    for (int m = 0; m < kMaxModels; ++m) {
+        bmax[m] = 0.0f;
+        if (m < a.nmodels)
+            for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < Ap; j += gridDim.x * kThreads)
+                bmax[m] = fmaxf(bmax[m], a.blkmax[m][j]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+#pragma unroll
+        for (int m = 0; m < kMaxModels; ++m)
+            if (m < a.nmodels) bmax[m] = fmaxf(bmax[m], __shfl_xor(bmax[m], o, 64));
+    }
+    // one atomic per workgroup and value at most: the per-wave atomics on one word serialise
+    __shared__ float sMax[1 + kMaxModels][kThreads / 64];
+    if (LaneId() == 0) {
+        sMax[0][threadIdx.x >> 6] = vmax;
+#pragma unroll
+        for (int m = 0; m < kMaxModels; ++m) sMax[1 + m][threadIdx.x >> 6] = bmax[m];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x <= static_cast<uint32_t>(a.nmodels)) {
         float mx = 0.0f;
-        for (int w = 0; w < kThreads / 64; ++w) mx = fmaxf(mx, sMax[w]);
-        if (mx > 0.0f) atomicMax(&a.st->vmax2_bits, __float_as_uint(mx));
+        for (int w = 0; w < kThreads / 64; ++w) mx = fmaxf(mx, sMax[threadIdx.x][w]);
+        MaxBits(threadIdx.x == 0 ? &a.st->vmax2_bits : &a.st->dep_bits[threadIdx.x - 1], mx);
     }
 }
 
@@ -657,12 +687,81 @@ __device__ __forceinline__ void WindowNode(int r, int& g, int& lx, int& ly, int&
     }
 }
 
-// P2G accumulator in LDS.  gfx950 executes ds_add_f32 lane by lane (about 190 cycles
-// per wavefront instruction whatever the addresses) but ds_add_f64 at about 20, so
-// the deposit accumulates in FP64 (which also makes the LDS sum nearly independent
-// of the order of the atomics) and the flush rounds each node once to FP32.
-using LdsAcc = double;
-__device__ __forceinline__ void LdsAdd(LdsAcc* p, float v) { atomicAdd(p, static_cast<LdsAcc>(v)); }
+// P2G accumulator in LDS: 64-bit fixed point.  On gfx950 a wavefront's ds_add_f32
+// runs lane by lane (about 190 cycles whatever the addresses), ds_add_f64 takes about
+// 16 cycles and ds_add_u64 about 8, independent of the number of active lanes, so the
+// deposit is an integer add: value v goes in as round(v 2^s).  Integer sums are exact,
+// so a node's LDS total does not depend on the order of the atomics; the flush rounds
+// it once to FP32.
+//   mass      deposits are weights w in [0, 1] (the mass is a per-model constant), at
+//             s = kq = 62 - ceil(log2 n) for a block of n particles: a node total is at
+//             most n 2^kq <= 2^62.
+//   momentum  s = kq - e with one e per launch, read before the first deposit (a
+//             workgroup-wide agreement on a scale inside the particle loop would need a
+//             barrier between the constitutive update and the deposit, which serialises
+//             the workgroup's VALU and LDS phases and costs more than the FP64 atomics):
+//             2^e = 2 max(32 b_prev, 8 m vmax), b_prev the largest deposit bound of the
+//             previous substep and vmax the grid's current max speed.  A particle whose
+//             bound b = max_r (|m v_r| + sum_c |dx K_rc|) (|w (m v + K (x_i - x_p))_r| <= b)
+//             reaches 2^(e-1) deposits its momentum with FP32 global atomics instead, so a
+//             sudden growth of the deposits is handled exactly, only more slowly.  The
+//             momentum resolution 2^(e - kq) is about 1e-14 of the largest deposit.
+// Values are scaled by exact powers of two (the particle's m v and dx K e_c once, not
+// each deposit) and truncated: a bias below one unit of the resolution.  e >= -64 keeps
+// every scale factor 2^(kq - e) and its inverse a normal FP32 number (kq <= 62).
+using LdsAcc = long long;
+constexpr int kMinExp = -64;
+__device__ __forceinline__ void LdsAdd(LdsAcc* p, float scaled) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(static_cast<long long>(scaled)));
+}
+
+// Deposit of one particle into both sub-grid windows: node momentum pm - sum_c kc_c t_c
+// + sum_c o_c kc_c at stencil offset o (pm = m v, kc_c = dx K e_c); weights scaled by
+// wscale.  Momentum: pm and kc already scaled to fixed point, or (SLOW) unscaled and
+// added straight to the grid M of the halo blocks.
+template <bool SLOW>
+__device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2], const float (&pm)[3],
+                                           const float (&kc)[3][3], float wscale, const uint32_t* sHalo,
+                                           float* M) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const Stencil32& sg = sn[g];
+        // grid 0: nodes [1,7]^3 (7 wide), grid 1: nodes [0,7]^3 (8 wide)
+        const int s1 = g ? 8 : 7, s2 = g ? 64 : 49, lo = g ? 0 : 1;
+        LdsAcc* base = sM + (g ? kM0Win : 0) + (sg.c[0] - lo) * s2 + (sg.c[1] - lo) * s1 + (sg.c[2] - lo);
+        float pa[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) pa[r] = pm[r] - (kc[0][r] * sg.t[0] + kc[1][r] * sg.t[1] + kc[2][r] * sg.t[2]);
+        float wxy[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            wxy[q] = ((q >> 1) ? 1.0f - sg.w0[0] : sg.w0[0]) * ((q & 1) ? 1.0f - sg.w0[1] : sg.w0[1]);
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
+            const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
+            const int off = ox * s2 + oy * s1 + oz;
+            LdsAdd(base + off, w * wscale);
+            uint32_t bi = kEmpty;
+            int node = 0;
+            if (SLOW) {
+                // node coordinates relative to the block's first node: 0..7
+                const int lx = sg.c[0] + ox, ly = sg.c[1] + oy, lz = sg.c[2] + oz;
+                bi = sHalo[((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2)];
+                node = g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                float pr = pa[r];
+                if (ox) pr += kc[0][r];
+                if (oy) pr += kc[1][r];
+                if (oz) pr += kc[2][r];
+                if (!SLOW) LdsAdd(base + (r + 1) * kMWin + off, w * pr);
+                else if (bi != kEmpty) unsafeAtomicAdd(M + static_cast<size_t>(bi) * kMFloats + (r + 1) * 128 + node, w * pr);
+            }
+        }
+    }
+}
 
 // 4 waves per SIMD: LDS allows 4 workgroups per CU; this caps the VGPRs at 128
 template <int KIND, int THREADS>
@@ -672,12 +771,25 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     __shared__ uint32_t sHalo[8];
     __shared__ uint32_t sWave[THREADS / 64][32];
     __shared__ uint32_t sRun[32];
+    __shared__ uint32_t sBlkMax;  // largest deposit bound of the block (FP32 bits)
+    __shared__ uint32_t sGlobal;  // particles of the block deposited with global atomics
 
     const uint32_t A = a.lc->A, Nb = a.lc->Nb;
     const uint32_t tid = threadIdx.x, lane = LaneId(), wave = tid >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
     const float dx = a.dom.dx, inv_dx = a.dom.inv_dx;
     const size_t N = a.n;
+    // momentum fixed-point exponent of this launch (LdsAcc): deposits with a bound below
+    // 2^(mexp-1) go to LDS
+    int mexp = kMinExp;
+    {
+        const float hint = __uint_as_float(a.st->dep_bits[a.model]);
+        const float vmax = sqrtf(__uint_as_float(a.st->vmax2_bits));
+        float ref = fmaxf(32.0f * hint, 8.0f * a.mat.mass * vmax);
+        if (!(ref <= 1.0e37f)) ref = 1.0e37f;  // a non-finite grid is already flagged
+        if (ref > 0.0f) mexp = max(kMinExp, static_cast<int>((__float_as_uint(ref) >> 23) & 0xFFu) - 127 + 2);
+    }
+    const float fast_limit = __int_as_float((mexp - 1 + 127) << 23);  // 2^(mexp-1)
 
     // Round-robin over the block list: particle counts per block vary by orders of
     // magnitude along the list (a model occupies only part of it), so contiguous
@@ -687,6 +799,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         if (beg == end) {
             // no particle of this model here: only the (zero) move counts are written
             if (tid < 27) a.cnt[static_cast<size_t>(j) * kDirections + tid] = 0u;
+            if (tid == 0) a.blkmax[j] = 0.0f;
             continue;
         }
         int bb[3];
@@ -699,6 +812,16 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
             sHalo[tid] = bi;
         }
         if (tid < 32) sRun[tid] = 0;
+        if (tid == 0) {
+            sBlkMax = 0u;
+            sGlobal = 0u;
+        }
+        // fixed-point scale of this block's weights (LdsAcc)
+        const uint32_t nblk = end - beg;
+        const int kq = 62 - (nblk > 1u ? 32 - __clz(nblk - 1u) : 0);
+        const float wscale = __int_as_float((kq + 127) << 23);         // 2^kq
+        const float qscale = __int_as_float((kq - mexp + 127) << 23);  // 2^(kq - mexp)
+        float tmax = 0.0f;  // largest deposit bound of this thread's particles
         for (int i = tid; i < kMChannels * kMWin; i += THREADS) sM[i] = 0;
         __syncthreads();
         // velocity halo in the same windows as the P2G accumulator (3 channels)
@@ -915,42 +1038,41 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                     const float sv = a.next_dt * a.mat.volume;
 #pragma unroll
                     for (int k = 0; k < 9; ++k) K[k] = C[k] * a.mat.mass - sv * DT[k];
-                    const float mv[3] = {a.mat.mass * v[0], a.mat.mass * v[1], a.mat.mass * v[2]};
+                    float pm[3], kc[3][3];  // m v and dx K e_c
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) pm[r] = a.mat.mass * v[r];
                     // node momentum m v + K (x_i - x_p), x_i - x_p = (o - t) dx:
                     // = a + sum_c o_c kc_c with a = m v - dx K t and kc_c = dx K e_c
-                    float kc[3][3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
 #pragma unroll
                         for (int r = 0; r < 3; ++r) kc[c][r] = dx * K[r * 3 + c];
+                    float bound = 0.0f, bsum = fabsf(x[0]) + fabsf(x[1]) + fabsf(x[2]);
 #pragma unroll
-                    for (int g = 0; g < 2; ++g) {
-                        const Stencil32& sg = sn[g];
-                        // grid 0: nodes [1,7]^3 (7 wide), grid 1: nodes [0,7]^3 (8 wide)
-                        const int s1 = g ? 8 : 7, s2 = g ? 64 : 49, lo = g ? 0 : 1;
-                        LdsAcc* base = sM + (g ? kM0Win : 0) + (sg.c[0] - lo) * s2 + (sg.c[1] - lo) * s1 + (sg.c[2] - lo);
-                        float pa[3];
-#pragma unroll
-                        for (int r = 0; r < 3; ++r)
-                            pa[r] = mv[r] - (kc[0][r] * sg.t[0] + kc[1][r] * sg.t[1] + kc[2][r] * sg.t[2]);
-                        float wxy[4];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            wxy[q] = ((q >> 1) ? 1.0f - sg.w0[0] : sg.w0[0]) * ((q & 1) ? 1.0f - sg.w0[1] : sg.w0[1]);
-#pragma unroll
-                        for (int o = 0; o < 8; ++o) {
-                            const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
-                            const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
-                            const int off = ox * s2 + oy * s1 + oz;
-                            LdsAdd(base + off, w * a.mat.mass);
+                    for (int r = 0; r < 3; ++r) {
+                        const float b = fabsf(pm[r]) + fabsf(kc[0][r]) + fabsf(kc[1][r]) + fabsf(kc[2][r]);
+                        bound = fmaxf(bound, b);
+                        bsum += b;
+                    }
+                    // a non-finite deposit has no fixed-point value: report it (the sum is
+                    // NaN or inf whenever an input is; fmaxf would drop a NaN)
+                    if (!(bsum <= 3.0e38f)) {
+                        SetFlag(a.st, kFlagNonFinite);
+                    } else {
+                        tmax = fmaxf(tmax, bound);
+                        if (bound < fast_limit) {
 #pragma unroll
                             for (int r = 0; r < 3; ++r) {
-                                float pr = pa[r];
-                                if (ox) pr += kc[0][r];
-                                if (oy) pr += kc[1][r];
-                                if (oz) pr += kc[2][r];
-                                LdsAdd(base + (r + 1) * kMWin + off, w * pr);
+                                pm[r] *= qscale;
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) kc[c][r] *= qscale;
                             }
+                            DepositP2G<false>(sM, sn, pm, kc, wscale, sHalo, a.M);
+                        } else {
+                            DepositP2G<true>(sM, sn, pm, kc, wscale, sHalo, a.M);
+                            const uint64_t on = __ballot(1);
+                            if (lane == static_cast<uint32_t>(__builtin_ctzll(on)))
+                                atomicAdd(&sGlobal, static_cast<uint32_t>(__popcll(on)));
                         }
                     }
                 }
@@ -975,11 +1097,18 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
             }
             __syncthreads();
         }
+        // ---------------- largest deposit bound of the block (for the next substep's scale)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
+        if (lane == 0 && tmax > 0.0f) atomicMax(&sBlkMax, __float_as_uint(tmax));
         // ---------------- flush: one global atomic per non-zero node and channel
         for (int i = tid; i < kMChannels * kMWin; i += THREADS) {
-            const float val = static_cast<float>(sM[i]);
-            if (val == 0.0f) continue;
+            const LdsAcc qv = sM[i];
+            if (qv == 0) continue;
             const int ch = i / kMWin, r = i % kMWin;
+            const float val = ch == 0 ? static_cast<float>(qv) * __int_as_float((127 - kq) << 23) * a.mat.mass
+                                      : static_cast<float>(qv) * __int_as_float((mexp - kq + 127) << 23);
+            if (val == 0.0f) continue;
             int g, lx, ly, lz;
             WindowNode(r, g, lx, ly, lz);
             const int o = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
@@ -989,6 +1118,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
             unsafeAtomicAdd(a.M + static_cast<size_t>(bi) * kMFloats + ch * 128 + node, val);
         }
         if (tid < 27) a.cnt[static_cast<size_t>(j) * kDirections + tid] = sRun[tid];
+        __syncthreads();
+        if (tid == 0) {
+            a.blkmax[j] = __uint_as_float(sBlkMax);
+            if (sGlobal) atomicAdd(&a.st->p2g_global, sGlobal);
+        }
         __syncthreads();
     }
 }

@@ -65,7 +65,7 @@ class HashInfo(ctypes.Structure):
 class Counters(ctypes.Structure):
     _fields_ = [("active_blocks", ctypes.c_uint32), ("neighbor_blocks", ctypes.c_uint32),
                 ("exterior_blocks", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("max_speed", ctypes.c_float),
-                ("reserved", ctypes.c_uint32), ("substeps", ctypes.c_uint64)]
+                ("p2g_global_deposits", ctypes.c_uint32), ("substeps", ctypes.c_uint64)]
 
 
 _lib = None

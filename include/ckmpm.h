@@ -84,7 +84,9 @@ typedef struct {
     uint32_t exterior_blocks; /* activePartitionExteriorBlockCount_ */
     uint32_t flags;           /* CKMPM_FLAG_* accumulated since the last clear */
     float max_speed;          /* sqrt(max |v|^2) of the last grid update */
-    uint32_t reserved;
+    uint32_t p2g_global_deposits; /* particles whose P2G momentum went to the grid with FP32
+                                     global atomics (deposit above the fixed-point range of
+                                     the launch) since the last clear; 0 in steady runs */
     uint64_t substeps;
 } ckmpm_counters;
 

@@ -56,7 +56,7 @@ def _import(eng, orc, nmodels):
         orc.set_state(m, eng.global_state(m))
 
 
-def _lockstep(scene, steps, dts, state_atol):
+def _lockstep(scene, steps, dts, state_atol, after=None):
     eng = engine_from_scene(scene)
     orc = Oracle.from_scene(scene)
     orc.setup()
@@ -73,6 +73,8 @@ def _lockstep(scene, steps, dts, state_atol):
         orc.physics(dt, ndt)
         eng.Substep(dt, ndt)
         eng.Synchronize()
+        if after is not None:
+            after(k, eng)
         # physics on identical inputs
         _grid_close(eng.grid_m(), orc.grid_mnext())
         for m in range(nm):
@@ -137,7 +139,7 @@ def test_colliding_spheres_trajectory_100_substeps():
         assert abs(disp - 99 * sc.dt) < 0.05 * 99 * sc.dt
 
 
-def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5):
+def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5, j0=1.0):
     """A small box of one material with random initial velocities (exercises C and stress)."""
     dx = 1.0 / (4 * dim)
     rng = scenes.Splitmix(seed)
@@ -147,7 +149,7 @@ def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5)
     v = (speed * rng.normal(3 * x.shape[0]).reshape(-1, 3)).astype(np.float32)
     vol = dx ** 3 / ppc_side ** 3
     if kind == scenes.FLUID:
-        mat = scenes.Material(kind, bulk=1e4, gamma=7.0, viscosity=0.1)
+        mat = scenes.Material(kind, bulk=1e4, gamma=7.0, viscosity=0.1, J0=j0)
     else:
         lam, mu = scenes.lame(2e5, 0.3)
         mat = scenes.Material(kind, lam, mu, friction_deg=30.0)
@@ -159,6 +161,21 @@ def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5)
 def test_material_box_lockstep(kind):
     sc = _box_scene(kind, seed=10 + kind)
     _lockstep(sc, 6, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+
+
+def test_prestressed_fluid_global_deposits_then_fixed_point():
+    """A compressed fluid (J0 = 0.95) at rest.  The first substep's deposits are pressure
+    impulses while no deposit bound has been recorded and the grid is at rest, so the
+    fixed-point range of that launch is empty and every particle's P2G momentum goes to
+    the grid through FP32 global atomics; from the second substep on, the bound recorded
+    by the first sets the range and the LDS fixed point takes every particle.  Each
+    substep is checked against the oracle either way."""
+    sc = _box_scene(scenes.FLUID, seed=31, speed=0.0, j0=0.95)
+    n = sc.n_particles
+    seen = []
+    _lockstep(sc, 4, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5),
+              after=lambda k, eng: seen.append(eng.counters().p2g_global_deposits))
+    assert seen == [n, n, n, n], seen
 
 
 def test_mixed_models_lockstep():
