@@ -9,6 +9,12 @@ the adjoint kernels (ckmpm_adjoint_*), then maps (lambda-bar, mu-bar) to log_E-b
 With segment=None the forward records every substep (a tape of all of them, which
 HBM holds for scenes of this size) and the backward undoes them without replaying.
 All arithmetic is the library's (FP32 kernels); torch only carries the tensors.
+
+The engine of a call is kept after its backward and reused by the next call on the
+same scene (positions, domain, material and tape length): it is rewound to its setup
+state with the new velocities and Lame parameters (ckmpm_adjoint_rewind) instead of
+being rebuilt, and checkpoint memory is reused, so a gradient loop allocates nothing
+after its first iteration.
 """
 from __future__ import annotations
 
@@ -19,6 +25,48 @@ import torch
 
 from .engine import engine_from_scene
 from .scenes import Scene
+
+
+class _EngineCache:
+    """The engine of the last finished call, keyed by everything its setup depends on
+    except the velocities and Lame parameters."""
+
+    def __init__(self):
+        self.key = None
+        self.x = None
+        self.eng = None
+
+    @staticmethod
+    def key_of(scene: Scene, tape: int):
+        m = scene.models[0]
+        mat = m.material
+        return (tuple(scene.dim), tuple(scene.offset), scene.dx, scene.gravity, scene.mass_clamp, scene.dt,
+                scene.block_capacity(), mat.kind, mat.lam, mat.mu, mat.J0, m.mass, m.volume, m.x.shape[0], tape)
+
+    def take(self, scene: Scene, tape: int):
+        """The cached engine if it matches (removed from the cache), else None."""
+        key = self.key_of(scene, tape)
+        if self.eng is not None and self.key == key and np.array_equal(self.x, scene.models[0].x):
+            eng, self.eng = self.eng, None
+            return eng
+        return None
+
+    def give(self, scene: Scene, tape: int, eng):
+        self.clear()
+        self.key, self.x, self.eng = self.key_of(scene, tape), np.array(scene.models[0].x, copy=True), eng
+
+    def clear(self):
+        if self.eng is not None:
+            self.eng.close()
+        self.key = self.x = self.eng = None
+
+
+_CACHE = _EngineCache()
+
+
+def clear_engine_cache():
+    """Release the engine kept for the next simulate() call."""
+    _CACHE.clear()
 
 
 def _lame_factors(nu: float):
@@ -37,7 +85,12 @@ class _Simulate(torch.autograd.Function):
         m.v = v0.detach().cpu().numpy().astype(np.float32).reshape(-1, 3)
         sc.models = [m]
         full = segment is None
-        eng = engine_from_scene(sc, adjoint_segment=steps if full else segment)
+        tape = steps if full else segment
+        eng = _CACHE.take(sc, tape)
+        if eng is not None:
+            eng.adjoint_rewind(m.v, m.lam_p, m.mu_p)
+        else:
+            eng = engine_from_scene(sc, adjoint_segment=tape)
         cks = []
         if full:
             eng.adjoint_record(True)
@@ -49,7 +102,7 @@ class _Simulate(torch.autograd.Function):
             eng.adjoint_record(False)
         eng.Synchronize()
         x = eng.positions_input_order(0)
-        ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment = eng, cks, sc, steps, segment
+        ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment, ctx.tape = eng, cks, sc, steps, segment, tape
         ctx.lam, ctx.mu = m.lam_p.astype(np.float64), m.mu_p.astype(np.float64)
         return torch.from_numpy(x.astype(np.float64)).to(log_E.device)
 
@@ -72,7 +125,7 @@ class _Simulate(torch.autograd.Function):
         lb, mb, vb = eng.adjoint_gradients(0)
         for c in cks:
             eng.free_checkpoint(c)
-        eng.close()
+        _CACHE.give(sc, ctx.tape, eng)
         # lambda and mu are both proportional to E: d/dlogE = lambda-bar lambda + mu-bar mu
         g_logE = torch.from_numpy(lb.astype(np.float64) * ctx.lam + mb.astype(np.float64) * ctx.mu)
         g_v0 = torch.from_numpy(vb.astype(np.float64))

@@ -17,6 +17,7 @@
 // (mpm_algorithm.cuh:1365-1845, mpm_material.cuh:318-357).
 #include <algorithm>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -41,19 +42,40 @@ __global__ void FillZeroKernel(float* p, size_t n) {
         p[i] = 0.0f;
 }
 // copy / clear the first Nb blocks (device count) of a per-block array
-__global__ void CopyBlocksKernel(const float* src, float* dst, const LayoutCounts* lc, uint32_t cap, int fpb) {
-    const size_t n = static_cast<size_t>(min(lc->Nb, cap)) * fpb;
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        dst[i] = src[i];
+// Several device-to-device copies in one launch (a snapshot is a dozen buffers; one
+// hipMemcpyAsync each costs a launch apiece).  blockIdx.y picks the segment; a segment
+// with lc set copies the first min(lc->Nb, cap) blocks of block_bytes each.  Buffers
+// are 256-byte aligned and sizes are multiples of 4 bytes.
+constexpr int kMaxCopySegs = 6 + 1 + 4 * kMaxModels;
+struct CopySeg {
+    const void* src;
+    void* dst;
+    uint64_t bytes;
+    const LayoutCounts* lc;
+    uint32_t block_bytes;
+};
+struct CopyList {
+    CopySeg seg[kMaxCopySegs];
+    int n;
+    uint32_t cap;
+};
+__global__ void MultiCopyKernel(CopyList L) {
+    const CopySeg& c = L.seg[blockIdx.y];
+    const uint64_t bytes = c.lc ? static_cast<uint64_t>(min(c.lc->Nb, L.cap)) * c.block_bytes : c.bytes;
+    const uint64_t n16 = bytes >> 4;
+    const uint4* s16 = static_cast<const uint4*>(c.src);
+    uint4* d16 = static_cast<uint4*>(c.dst);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n16; i += stride) d16[i] = s16[i];
+    // 4-byte tail (at most 3 words)
+    const uint64_t w0 = n16 * 4, nw = bytes >> 2;
+    const uint64_t i = w0 + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < nw) static_cast<uint32_t*>(c.dst)[i] = static_cast<const uint32_t*>(c.src)[i];
 }
 __global__ void ZeroBlocksKernel(float* dst, const LayoutCounts* lc, uint32_t cap, int fpb) {
     const size_t n = static_cast<size_t>(min(lc->Nb, cap)) * fpb;
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         dst[i] = 0.0f;
-}
-__global__ void IdsPermuteKernel(const uint32_t* perm, const uint32_t* ids, uint32_t n, uint32_t* out) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-        out[t] = ids[perm[t] & kSlotMask];
 }
 // adjoint of positions: input order (AoS) -> processing order (SoA)
 __global__ void SeedKernel(const uint32_t* ids, const float* xbar_in, uint32_t n, float* xbar) {
@@ -80,7 +102,7 @@ struct GridBwdArgs {
     const uint32_t* table_prev;  // layout of M_k
     const LayoutCounts* lc_prev;
     const float* M;              // M_k (previous layout)
-    const float* Vbar;           // V_k-bar (layout of substep k)
+    float* Vbar;                 // V_k-bar (layout of substep k), cleared after use
     float* Mbar;                 // out: M_k-bar (previous layout), zeroed beforehand
     float mass_clamp;
     uint32_t cap;
@@ -90,6 +112,12 @@ __global__ __launch_bounds__(kAT) void GridBackwardKernel(GridBwdArgs a) {
     const uint32_t n = threadIdx.x & 127u;
     for (uint32_t i = blockIdx.x * 2 + (threadIdx.x >> 7); i < Nb; i += gridDim.x * 2) {
         const uint32_t key = a.keys[i];
+        // V-bar is read once, here: clear it for the next substep's accumulation
+        float* vb = a.Vbar + static_cast<size_t>(i) * kVFloats;
+        const float v[3] = {vb[n], vb[128 + n], vb[256 + n]};
+        vb[n] = 0.0f;
+        vb[128 + n] = 0.0f;
+        vb[256 + n] = 0.0f;
         int b[3];
         CoordsOf(a.dom, key, b[0], b[1], b[2]);
         bool oob = false;
@@ -101,8 +129,6 @@ __global__ __launch_bounds__(kAT) void GridBackwardKernel(GridBwdArgs a) {
         const float m = src[n];
         if (oob || !(m > a.mass_clamp)) continue;
         const float p[3] = {src[128 + n], src[256 + n], src[384 + n]};
-        const float* vb = a.Vbar + static_cast<size_t>(i) * kVFloats;
-        const float v[3] = {vb[n], vb[128 + n], vb[256 + n]};
         const float im = 1.0f / m;
         float* dst = a.Mbar + static_cast<size_t>(io) * kMFloats;
         // v = p / m (+ g dt): p-bar = v-bar / m, m-bar = -(v-bar . p) / m^2
@@ -120,9 +146,9 @@ struct AdjArgs {
     const uint32_t* keys;    // layout of substep k
     const uint32_t* table;
     const LayoutCounts* lc;
-    const float* V;          // V_k (recomputed)
+    const float* V;          // V_k (recorded)
     const float* Mbar;       // M_{k+1}-bar, same layout
-    float* Vbar;             // V_k-bar, accumulated (zeroed beforehand)
+    float* Vbar;             // V_k-bar, accumulated (zero beforehand: the grid backward clears it)
     const float* st_in;      // particle state read by substep k (slot order)
     const uint32_t* perm;
     const uint32_t* off;
@@ -136,6 +162,9 @@ struct AdjArgs {
     float* tb_out;
     float dt, next_dt;
     uint32_t split;          // workgroups per particle block (each takes every split-th chunk)
+    float* Mzero;            // cleared first (the next grid backward's output), or null
+    const LayoutCounts* lc_zero;  // its Nb blocks
+    uint32_t cap;
 };
 
 __device__ __forceinline__ float W0p(float t) { return -1.0f + cospif(2.0f * t); }
@@ -197,6 +226,11 @@ __global__ __launch_bounds__(kAT) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const float dx = a.dom.dx, inv_dx = a.dom.inv_dx;
     const size_t N = a.n;
     const float mass = a.mat.mass;
+    if (a.Mzero) {
+        const size_t nz = static_cast<size_t>(min(a.lc_zero->Nb, a.cap)) * kMFloats;
+        for (size_t i = blockIdx.x * static_cast<size_t>(kAT) + tid; i < nz; i += static_cast<size_t>(gridDim.x) * kAT)
+            a.Mzero[i] = 0.0f;
+    }
     // workgroup w serves part w % split of blocks w / split, w / split + gridDim / split, ...
     const uint32_t part = blockIdx.x % a.split, per = gridDim.x / a.split;
     for (uint32_t j = blockIdx.x / a.split; j < A; j += per) {
@@ -655,6 +689,7 @@ struct Snapshot {
     uint32_t* table[2] = {nullptr, nullptr};
     LayoutCounts* lc[2] = {nullptr, nullptr};
     float* M = nullptr;
+    float* V = nullptr;  // grid velocities of the substep (current layout)
     std::vector<uint32_t*> perm, off, ids;
     std::vector<float*> st;
     double dt = 0, nd = 0;
@@ -678,19 +713,24 @@ struct AdjointState {
     float* Mbar[2] = {nullptr, nullptr};
     int mb = 0;
     float* Vbar = nullptr;
-    float* Vrec = nullptr;
-    float* Mscratch = nullptr;
-    Status* st_scratch = nullptr;
     float* host_io = nullptr;  // device staging of 3 n floats
     const uint32_t* setup_table = nullptr;    // layout of M_0 (set when the first substep is undone)
     const LayoutCounts* setup_lc = nullptr;
     bool at_setup = false;
     std::vector<void*> allocs;
+    // state right after the initial setup (ckmpm_adjoint_rewind)
+    Snapshot setup;
+    bool has_setup = false;
+    // checkpoint memory: blocks of ck_bytes, reused after ckmpm_checkpoint_free
+    size_t ck_bytes = 0;
+    std::vector<void*> ck_pool;
+    std::set<ckmpm_checkpoint*> live;
 };
 
 struct ckmpm_checkpoint {
     Snapshot snap;
-    std::vector<void*> allocs;
+    void* mem = nullptr;
+    ckmpm_engine* eng = nullptr;  // null once the engine is destroyed (memory freed with it)
 };
 
 namespace {
@@ -717,13 +757,15 @@ struct Carver {
     }
 };
 
-void CarveSnapshot(ckmpm_engine* e, Snapshot& s, Carver& c) {
+// with_v: room for the grid velocities (substep tape; checkpoints do not keep them)
+void CarveSnapshot(ckmpm_engine* e, Snapshot& s, Carver& c, bool with_v) {
     for (int l = 0; l < 2; ++l) {
         s.keys[l] = c.take<uint32_t>(e->cap);
         s.table[l] = c.take<uint32_t>(e->dom.nblocks);
         s.lc[l] = c.take<LayoutCounts>(1);
     }
     s.M = c.take<float>(static_cast<size_t>(e->cap) * kMFloats);
+    s.V = with_v ? c.take<float>(static_cast<size_t>(e->cap) * kVFloats) : nullptr;
     const size_t nm = e->models.size();
     s.perm.assign(nm, nullptr);
     s.off.assign(nm, nullptr);
@@ -739,39 +781,68 @@ void CarveSnapshot(ckmpm_engine* e, Snapshot& s, Carver& c) {
 }
 
 // count snapshots in one device allocation (recorded in list)
-bool AllocSnapshots(ckmpm_engine* e, Snapshot* s, size_t count, std::vector<void*>& list) {
+bool AllocSnapshots(ckmpm_engine* e, Snapshot* s, size_t count, std::vector<void*>& list, bool with_v) {
     Carver sizer;
     Snapshot probe;
-    CarveSnapshot(e, probe, sizer);
+    CarveSnapshot(e, probe, sizer, with_v);
     const size_t per = sizer.used;
     char* base = nullptr;
     if (!DevAlloc(list, &base, per * count)) return false;
     for (size_t k = 0; k < count; ++k) {
         Carver c{base + k * per, 0};
-        CarveSnapshot(e, s[k], c);
+        CarveSnapshot(e, s[k], c, with_v);
     }
     return true;
 }
 
-int Take(ckmpm_engine* e, Snapshot& s) {
-    hipStream_t st = e->stream;
+// The engine buffers of a snapshot, paired with the snapshot's copies: to_snap copies
+// engine -> snapshot (Take), otherwise snapshot -> engine (Put).  The engine's buffer
+// roles (cur, sc, mprev, idc) must already be those of the snapshot.
+void LaunchSnapshotCopy(ckmpm_engine* e, const Snapshot& s, bool to_snap) {
     AdjointState* A = e->adj;
+    CopyList cl{};
+    cl.cap = e->cap;
+    auto add = [&](void* eng, void* snap, uint64_t bytes, const LayoutCounts* lc = nullptr, uint32_t bb = 0) {
+        CopySeg& c = cl.seg[cl.n++];
+        c.src = to_snap ? eng : snap;
+        c.dst = to_snap ? snap : eng;
+        c.bytes = bytes;
+        c.lc = lc;
+        c.block_bytes = bb;
+    };
     const LayoutBuf* L[2] = {&e->L[e->cur], &e->L[1 - e->cur]};
     for (int l = 0; l < 2; ++l) {
-        CK_HIP(e, hipMemcpyAsync(s.keys[l], L[l]->keys, sizeof(uint32_t) * e->cap, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(s.table[l], L[l]->table, sizeof(uint32_t) * e->dom.nblocks, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(s.lc[l], L[l]->lc, sizeof(LayoutCounts), hipMemcpyDeviceToDevice, st));
+        add(L[l]->keys, s.keys[l], sizeof(uint32_t) * e->cap);
+        add(L[l]->table, s.table[l], sizeof(uint32_t) * e->dom.nblocks);
+        add(L[l]->lc, s.lc[l], sizeof(LayoutCounts));
     }
-    // M_k is indexed by the previous layout: its Nb blocks are the ones in use
-    CopyBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(e->M[e->mprev], s.M,
-                                                                                    L[1]->lc, e->cap, kMFloats);
+    // M_k is indexed by the previous layout: its Nb blocks are the ones in use (the count
+    // is read from the side being copied, before the copy of that count lands)
+    add(e->M[e->mprev], s.M, 0, to_snap ? L[1]->lc : s.lc[1], kMFloats * sizeof(float));
+    // V of the current layout: recorded for the backward, not restored (the next grid
+    // phase recomputes it)
+    if (to_snap && s.V) add(e->V, s.V, 0, L[0]->lc, kVFloats * sizeof(float));
     for (size_t i = 0; i < e->models.size(); ++i) {
         const ModelBuf& m = e->models[i];
-        CK_HIP(e, hipMemcpyAsync(s.perm[i], m.perm[e->sc], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(s.off[i], m.off[e->sc], sizeof(uint32_t) * (e->cap + 1), hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(s.ids[i], A->ids[A->idc][i], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(s.st[i], m.st[e->sc], sizeof(float) * m.n * m.nattr, hipMemcpyDeviceToDevice, st));
+        add(m.perm[e->sc], s.perm[i], sizeof(uint32_t) * m.n);
+        add(m.off[e->sc], s.off[i], sizeof(uint32_t) * (static_cast<uint64_t>(e->cap) + 1));
+        add(A->ids[A->idc][i], s.ids[i], sizeof(uint32_t) * m.n);
+        add(m.st[e->sc], s.st[i], sizeof(float) * static_cast<uint64_t>(m.n) * m.nattr);
     }
+    // enough workgroups for about four 16-byte loads per thread in the largest segment
+    uint64_t most = 0;
+    for (int k = 0; k < cl.n; ++k)
+        most = std::max<uint64_t>(most, cl.seg[k].lc ? static_cast<uint64_t>(std::min<uint32_t>(e->cap, 2 * e->a_hint + 64)) *
+                                                           cl.seg[k].block_bytes
+                                                     : cl.seg[k].bytes);
+    const uint32_t gx = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(most / (kAT * 64), 16), 1024));
+    MultiCopyKernel<<<dim3(gx, cl.n), kAT, 0, e->stream>>>(cl);
+}
+
+int Take(ckmpm_engine* e, Snapshot& s) {
+    AdjointState* A = e->adj;
+    LaunchSnapshotCopy(e, s, true);
+    CK_HIP(e, hipGetLastError());
     s.cur = e->cur;
     s.sc = e->sc;
     s.mprev = e->mprev;
@@ -790,21 +861,8 @@ int Put(ckmpm_engine* e, const Snapshot& s) {
     e->sc = s.sc;
     e->mprev = s.mprev;
     A->idc = s.idc;
-    LayoutBuf* L[2] = {&e->L[e->cur], &e->L[1 - e->cur]};
-    for (int l = 0; l < 2; ++l) {
-        CK_HIP(e, hipMemcpyAsync(L[l]->keys, s.keys[l], sizeof(uint32_t) * e->cap, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(L[l]->table, s.table[l], sizeof(uint32_t) * e->dom.nblocks, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(L[l]->lc, s.lc[l], sizeof(LayoutCounts), hipMemcpyDeviceToDevice, st));
-    }
-    CopyBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(s.M, e->M[e->mprev], s.lc[1],
-                                                                                    e->cap, kMFloats);
-    for (size_t i = 0; i < e->models.size(); ++i) {
-        ModelBuf& m = e->models[i];
-        CK_HIP(e, hipMemcpyAsync(m.perm[e->sc], s.perm[i], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(m.off[e->sc], s.off[i], sizeof(uint32_t) * (e->cap + 1), hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(A->ids[A->idc][i], s.ids[i], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
-        CK_HIP(e, hipMemcpyAsync(m.st[e->sc], s.st[i], sizeof(float) * m.n * m.nattr, hipMemcpyDeviceToDevice, st));
-    }
+    LaunchSnapshotCopy(e, s, false);
+    CK_HIP(e, hipGetLastError());
     CK_HIP(e, hipMemsetAsync(e->status, 0, sizeof(Status), st));
     e->substeps = s.substeps;
     e->a_hint = s.a_hint;
@@ -833,6 +891,25 @@ void AdjointOnSetupOrder(ckmpm_engine* e, size_t model, const uint32_t* order, c
     (void)hipMemcpyAsync(A->x0[model], xyz, sizeof(float) * 3 * m.n, hipMemcpyDeviceToDevice, e->stream);
 }
 
+void AdjointAfterSetup(ckmpm_engine* e) {
+    AdjointState* A = e->adj;
+    if (!A) return;
+    Carver sizer;
+    Snapshot probe;
+    CarveSnapshot(e, probe, sizer, false);
+    char* base = nullptr;
+    if (!DevAlloc(A->allocs, &base, sizer.used)) return;  // rewind then reports "not set up"
+    Carver c{base, 0};
+    CarveSnapshot(e, A->setup, c, false);
+    A->has_setup = Take(e, A->setup) == CKMPM_OK;
+}
+
+int AdjointCheckTape(ckmpm_engine* e) {
+    AdjointState* A = e->adj;
+    if (A && A->recording && A->len >= A->seg) return Fail(e, CKMPM_ERR_CAPACITY, "substep tape full");
+    return CKMPM_OK;
+}
+
 int AdjointBeforeSubstep(ckmpm_engine* e, double dt, double nd) {
     AdjointState* A = e->adj;
     if (!A || !A->recording) return CKMPM_OK;
@@ -845,22 +922,29 @@ int AdjointBeforeSubstep(ckmpm_engine* e, double dt, double nd) {
     return CKMPM_OK;
 }
 
+void AdjointIdBuffers(ckmpm_engine* e, size_t model, const uint32_t** in, uint32_t** out) {
+    AdjointState* A = e->adj;
+    *in = A ? A->ids[A->idc][model] : nullptr;
+    *out = A ? A->ids[1 - A->idc][model] : nullptr;
+}
+
 void AdjointAfterSubstep(ckmpm_engine* e) {
     AdjointState* A = e->adj;
     if (!A) return;
-    // the state written by the substep holds, at position t, the particle of slot
-    // perm[t] of the ordering that substep consumed (the roles have flipped already)
-    for (size_t i = 0; i < e->models.size(); ++i) {
-        const ModelBuf& m = e->models[i];
-        IdsPermuteKernel<<<AGrid(m.n), kAT, 0, e->stream>>>(m.perm[1 - e->sc], A->ids[A->idc][i], m.n,
-                                                            A->ids[1 - A->idc][i]);
-    }
+    // G2P2G wrote ids[1 - idc][t] = ids[idc][slot of t]: the state written by the
+    // substep holds, at position t, the particle of that slot
     A->idc = 1 - A->idc;
 }
 
 void AdjointDestroy(ckmpm_engine* e) {
     AdjointState* A = e->adj;
     if (!A) return;
+    for (ckmpm_checkpoint* c : A->live) {
+        (void)hipFree(c->mem);
+        c->mem = nullptr;
+        c->eng = nullptr;
+    }
+    for (void* p : A->ck_pool) (void)hipFree(p);
     for (void* p : A->allocs) (void)hipFree(p);
     delete A;
     e->adj = nullptr;
@@ -884,7 +968,7 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
     AdjointState* A = new AdjointState();
     A->seg = segment;
     A->tape.resize(segment);
-    bool ok = AllocSnapshots(e, A->tape.data(), segment, A->allocs);
+    bool ok = AllocSnapshots(e, A->tape.data(), segment, A->allocs, true);
     const size_t nm = e->models.size();
     for (int k = 0; k < 2; ++k) {
         A->ids[k].assign(nm, nullptr);
@@ -909,9 +993,6 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
     }
     for (int k = 0; k < 2; ++k) ok = ok && DevAlloc(A->allocs, &A->Mbar[k], static_cast<size_t>(e->cap) * kMFloats);
     ok = ok && DevAlloc(A->allocs, &A->Vbar, static_cast<size_t>(e->cap) * kVFloats);
-    ok = ok && DevAlloc(A->allocs, &A->Vrec, static_cast<size_t>(e->cap) * kVFloats);
-    ok = ok && DevAlloc(A->allocs, &A->Mscratch, static_cast<size_t>(e->cap) * kMFloats);
-    ok = ok && DevAlloc(A->allocs, &A->st_scratch, 1);
     ok = ok && DevAlloc(A->allocs, &A->host_io, 3 * static_cast<size_t>(nmax));
     e->adj = A;
     if (!ok) {
@@ -924,11 +1005,26 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
 int ckmpm_checkpoint_save(ckmpm_engine* e, ckmpm_checkpoint** out) {
     if (!e || !out) return CKMPM_ERR_INVALID;
     if (!e->adj || !e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "checkpoints need the adjoint enabled and set up");
-    ckmpm_checkpoint* c = new ckmpm_checkpoint();
-    if (!AllocSnapshots(e, &c->snap, 1, c->allocs)) {
-        ckmpm_checkpoint_free(c);
+    AdjointState* A = e->adj;
+    if (A->ck_bytes == 0) {
+        Carver sizer;
+        Snapshot probe;
+        CarveSnapshot(e, probe, sizer, false);
+        A->ck_bytes = sizer.used;
+    }
+    void* mem = nullptr;
+    if (!A->ck_pool.empty()) {
+        mem = A->ck_pool.back();
+        A->ck_pool.pop_back();
+    } else if (hipMalloc(&mem, A->ck_bytes) != hipSuccess) {
         return Fail(e, CKMPM_ERR_HIP, "checkpoint allocation failed");
     }
+    ckmpm_checkpoint* c = new ckmpm_checkpoint();
+    c->mem = mem;
+    c->eng = e;
+    Carver carve{static_cast<char*>(mem), 0};
+    CarveSnapshot(e, c->snap, carve, false);
+    A->live.insert(c);
     const int rc = Take(e, c->snap);
     if (rc != CKMPM_OK) {
         ckmpm_checkpoint_free(c);
@@ -945,8 +1041,45 @@ int ckmpm_checkpoint_restore(ckmpm_engine* e, const ckmpm_checkpoint* c) {
 
 int ckmpm_checkpoint_free(ckmpm_checkpoint* c) {
     if (!c) return CKMPM_ERR_INVALID;
-    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->eng && c->eng->adj) {
+        // the memory goes back to the engine's pool; a pending copy into it is
+        // stream-ordered before any later checkpoint's use
+        AdjointState* A = c->eng->adj;
+        A->live.erase(c);
+        A->ck_pool.push_back(c->mem);
+    }
     delete c;
+    return CKMPM_OK;
+}
+
+int ckmpm_adjoint_rewind(ckmpm_engine* e, const float* vel, uint64_t nvel, const float* lambda_p, const float* mu_p) {
+    if (!e || !vel || !lambda_p || !mu_p) return CKMPM_ERR_INVALID;
+    AdjointState* A = e->adj;
+    if (!A || !A->has_setup) return Fail(e, CKMPM_ERR_INVALID, "rewind needs the adjoint enabled and set up");
+    if (e->models.size() != 1) return Fail(e, CKMPM_ERR_INVALID, "rewind supports engines with one model");
+    const ModelBuf& m = e->models[0];
+    if (nvel != 1 && nvel != m.n) return Fail(e, CKMPM_ERR_INVALID, "vel holds 1 or n velocities");
+    hipStream_t st = e->stream;
+    CK_TRY(Put(e, A->setup));
+    // initial state with the new Lame parameters (setup order, as the setup wrote it)
+    CK_HIP(e, hipMemcpyAsync(A->host_io, lambda_p, sizeof(float) * m.n, hipMemcpyHostToDevice, st));
+    CK_HIP(e, hipMemcpyAsync(A->host_io + m.n, mu_p, sizeof(float) * m.n, hipMemcpyHostToDevice, st));
+    // (the kernel turns its order argument into the identity ordering: give it a copy in
+    // the move-code buffer, which the next G2P2G overwrites)
+    CK_HIP(e, hipMemcpyAsync(m.code, A->setup_order[0], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
+    LaunchInitState(e->dom, m.kind, A->x0[0], A->host_io, A->host_io + m.n, m.pp_attr, m.code, m.n, m.mat.J0,
+                    m.st[e->sc], st);
+    // grid mass / momentum of the new velocities on the setup layout
+    const LayoutBuf& L = e->L[e->cur];
+    CK_HIP(e, hipMemsetAsync(e->M[e->mprev], 0, static_cast<size_t>(e->cap) * kMFloats * sizeof(float), st));
+    CK_HIP(e, hipMemcpyAsync(A->host_io, vel, sizeof(float) * 3 * nvel, hipMemcpyHostToDevice, st));
+    LaunchRasterize(e->dom, A->x0[0], A->host_io, m.n, static_cast<uint32_t>(nvel), m.mat.mass, L.table, L.lc,
+                    e->M[e->mprev], e->status, st);
+    CK_HIP(e, hipMemsetAsync(m.blkmax, 0, static_cast<size_t>(e->cap) * sizeof(float), st));
+    A->len = 0;
+    A->recording = false;
+    A->at_setup = false;
+    CK_HIP(e, hipStreamSynchronize(st));
     return CKMPM_OK;
 }
 
@@ -968,6 +1101,9 @@ int ckmpm_adjoint_reset(ckmpm_engine* e) {
     }
     FillZeroKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, e->stream>>>(
         A->Mbar[A->mb], static_cast<size_t>(e->cap) * kMFloats);
+    // V-bar accumulates from zero; each grid backward clears what it read
+    FillZeroKernel<<<AGrid(static_cast<size_t>(e->cap) * kVFloats), kAT, 0, e->stream>>>(
+        A->Vbar, static_cast<size_t>(e->cap) * kVFloats);
     A->at_setup = false;
     CK_HIP(e, hipGetLastError());
     return CKMPM_OK;
@@ -989,24 +1125,8 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
     if (A->len == 0) return Fail(e, CKMPM_ERR_INVALID, "no recorded substep to undo");
     const Snapshot& s = A->tape[A->len - 1];
     hipStream_t st = e->stream;
-    // V_k recomputed from the recorded M_k
-    GridUpdateArgs g{};
-    g.dom = e->dom;
-    g.keys = s.keys[0];
-    g.lc = s.lc[0];
-    g.table_prev = s.table[1];
-    g.lc_prev = s.lc[1];
-    g.M_prev = s.M;
-    g.V = A->Vrec;
-    g.M_zero = A->Mscratch;
-    g.dt = static_cast<float>(s.dt);
-    g.gravity = e->sim.gravity;
-    g.mass_clamp = e->sim.mass_clamp;
-    g.cap = e->cap;
-    g.st = A->st_scratch;
     const uint32_t grid = GridSize(e);
-    LaunchGridUpdate(g, std::max<uint32_t>(2 * grid, 64u), st);
-    ZeroBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kVFloats), kAT, 0, st>>>(A->Vbar, s.lc[0], e->cap, kVFloats);
+    float* Mout = A->Mbar[1 - A->mb];
     const int nb = 1 - A->ab;
     for (size_t i = 0; i < e->models.size(); ++i) {
         const ModelBuf& m = e->models[i];
@@ -1016,7 +1136,7 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
         a.keys = s.keys[0];
         a.table = s.table[0];
         a.lc = s.lc[0];
-        a.V = A->Vrec;
+        a.V = s.V;
         a.Mbar = A->Mbar[A->mb];
         a.Vbar = A->Vbar;
         a.st_in = s.st[i];
@@ -1032,6 +1152,10 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
         a.tb_out = A->tb[nb][i];
         a.dt = static_cast<float>(s.dt);
         a.next_dt = static_cast<float>(s.nd);
+        // M_k-bar lives in the previous layout; blocks beyond its Nb are never read
+        a.Mzero = i == 0 ? Mout : nullptr;
+        a.lc_zero = s.lc[1];
+        a.cap = e->cap;
         // few particle blocks (small scenes): several workgroups per block so that the
         // launch still covers the 256 CUs
         a.split = std::min<uint32_t>(8, std::max<uint32_t>(1, (2048 + e->a_hint) / std::max<uint32_t>(e->a_hint, 1)));
@@ -1039,9 +1163,6 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
         if (m.kind == kFixedCorotated) G2P2GBackwardKernel<kFixedCorotated><<<bgrid, kAT, 0, st>>>(a);
         else G2P2GBackwardKernel<kNeoHookean><<<bgrid, kAT, 0, st>>>(a);
     }
-    float* Mout = A->Mbar[1 - A->mb];
-    // M_k-bar lives in the previous layout; blocks beyond its Nb are never read
-    ZeroBlocksKernel<<<AGrid(static_cast<size_t>(e->cap) * kMFloats), kAT, 0, st>>>(Mout, s.lc[1], e->cap, kMFloats);
     GridBwdArgs gb{};
     gb.dom = e->dom;
     gb.keys = s.keys[0];

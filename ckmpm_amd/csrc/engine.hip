@@ -120,6 +120,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         a.st = e->status;
         a.model = static_cast<int>(mi);
         a.blkmax = m.blkmax;
+        AdjointIdBuffers(e, mi, &a.ids_in, &a.ids_out);
         if (e->tuning) (void)hipEventRecord(m.tune_ev[0], s);
         LaunchG2P2G(a, grid, m.threads, s);
         if (e->tuning) (void)hipEventRecord(m.tune_ev[1], s);
@@ -365,13 +366,17 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
     e->cur_dt = 0.0;
     const int code = FlagsToCode(st.flags);
     if (code != CKMPM_OK) return Fail(e, code, "initial setup flagged 0x" + std::to_string(st.flags));
+    AdjointAfterSetup(e);
     return CKMPM_OK;
 }
 
 int ckmpm_substep(ckmpm_engine* e, double dt, double next_dt) {
     CK_TRY(CheckReady(e));
-    CK_TRY(AdjointBeforeSubstep(e, dt, next_dt));
+    CK_TRY(AdjointCheckTape(e));
     PhaseGrid(e, dt);
+    // recorded after the grid phase, which leaves every recorded input unchanged and
+    // provides the grid velocities of the substep
+    CK_TRY(AdjointBeforeSubstep(e, dt, next_dt));
     PhaseParticles(e, dt, next_dt);
     AdjointAfterSubstep(e);
     const hipError_t r = hipGetLastError();
@@ -386,6 +391,7 @@ int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
     for (uint32_t f = 0; f < frames; ++f) {
         const double frame_end = e->time + spf;
         while (e->time < frame_end) {
+            CK_TRY(AdjointCheckTape(e));
             PhaseGrid(e, e->cur_dt);
             Status st;
             LayoutCounts lc;

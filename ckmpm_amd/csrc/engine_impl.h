@@ -117,6 +117,12 @@ inline int Alloc(ckmpm_engine* e, T** p, size_t count) {
 void AdjointOnSetupOrder(ckmpm_engine* e, size_t model, const uint32_t* order, const float* xyz);
 int AdjointBeforeSubstep(ckmpm_engine* e, double dt, double nd);
 void AdjointAfterSubstep(ckmpm_engine* e);
+// after a successful initial setup: keep its state for ckmpm_adjoint_rewind
+void AdjointAfterSetup(ckmpm_engine* e);
+// error when the substep tape is recording and full (checked before a substep starts)
+int AdjointCheckTape(ckmpm_engine* e);
+// identity arrays of a model for the next substep (null, null without differentiation)
+void AdjointIdBuffers(ckmpm_engine* e, size_t model, const uint32_t** in, uint32_t** out);
 void AdjointDestroy(ckmpm_engine* e);
 
 #define CK_TRY(x)                   \

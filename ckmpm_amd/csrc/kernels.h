@@ -107,6 +107,9 @@ struct G2P2GArgs {
     Status* st;
     int model;      // index of st->dep_bits
     float* blkmax;  // [A] largest deposit bound per particle block (for the next substep)
+    // particle identities (differentiation): ids_out[t] = ids_in[slot of t]; null: none
+    const uint32_t* ids_in;
+    uint32_t* ids_out;
 };
 // threads per workgroup: 256 or 512 (one particle block per workgroup either way)
 void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s);

@@ -854,6 +854,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                     s = 0;
                     mv = 13;
                 }
+                if (a.ids_out) a.ids_out[t] = a.ids_in[s];
                 // position relative to this block's origin: the stored value is relative
                 // to the block that wrote it, which sits at this block - move
                 const int mov[3] = {mv / 9 - 1, (mv / 3) % 3 - 1, mv % 3 - 1};

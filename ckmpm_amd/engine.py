@@ -172,6 +172,14 @@ class MPMEngine:
     def free_checkpoint(self, ck: ctypes.c_void_p):
         self._lib.ckmpm_checkpoint_free(ck)
 
+    def adjoint_rewind(self, v: np.ndarray, lam: np.ndarray, mu: np.ndarray):
+        """Back to the state right after InitialSetup (one model) with new initial
+        velocities (1 or n rows) and per-particle Lame parameters; allocates nothing."""
+        vv = _f32(v).reshape(-1, 3)
+        la, mu_ = _f32(lam).reshape(-1), _f32(mu).reshape(-1)
+        self._check(self._lib.ckmpm_adjoint_rewind(self._h, _ptr(vv, _f32p), vv.shape[0], _ptr(la, _f32p),
+                                                   _ptr(mu_, _f32p)), "ckmpm_adjoint_rewind")
+
     def adjoint_record(self, on: bool):
         self._check(self._lib.ckmpm_adjoint_record(self._h, int(on)), "ckmpm_adjoint_record")
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "ckmpm_batch_destroy", "ckmpm_batch_last_error",
     "ckmpm_adjoint_enable", "ckmpm_checkpoint_save", "ckmpm_checkpoint_restore", "ckmpm_checkpoint_free",
     "ckmpm_adjoint_record", "ckmpm_adjoint_reset", "ckmpm_adjoint_seed", "ckmpm_adjoint_substep",
-    "ckmpm_adjoint_gradients", "ckmpm_get_positions_input_order",
+    "ckmpm_adjoint_gradients", "ckmpm_get_positions_input_order", "ckmpm_adjoint_rewind",
 ]
 
 
@@ -141,6 +141,7 @@ def load():
     L.ckmpm_adjoint_substep.argtypes = [_vp]
     L.ckmpm_adjoint_gradients.argtypes = [_vp, ctypes.c_int32, _f32p, _f32p, _f32p]
     L.ckmpm_get_positions_input_order.argtypes = [_vp, ctypes.c_int32, _f32p]
+    L.ckmpm_adjoint_rewind.argtypes = [_vp, _f32p, ctypes.c_uint64, _f32p, _f32p]
     L.ckmpm_hash_last_error.argtypes = [_vp]
     L.ckmpm_hash_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

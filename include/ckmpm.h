@@ -185,7 +185,17 @@ typedef struct ckmpm_checkpoint ckmpm_checkpoint;
 int ckmpm_adjoint_enable(ckmpm_engine* eng, uint32_t segment);
 int ckmpm_checkpoint_save(ckmpm_engine* eng, ckmpm_checkpoint** out);
 int ckmpm_checkpoint_restore(ckmpm_engine* eng, const ckmpm_checkpoint* ck);
+/* the checkpoint's memory returns to its engine for the next ckmpm_checkpoint_save
+ * (freed with the engine; a checkpoint outliving its engine only releases its handle) */
 int ckmpm_checkpoint_free(ckmpm_checkpoint* ck);
+/* No counterpart in the reference, which builds a new engine for every run: puts an
+ * adjoint-enabled engine with one model back to its state right after
+ * ckmpm_initial_setup (same positions, setup order and block layout) with new initial
+ * velocities (nvel = 1 or n, 3 floats each) and new per-particle Lame parameters (n
+ * each, input order), the grid mass/momentum rasterized again and the substep count at
+ * 0.  Allocates nothing, so a gradient loop on one scene reuses one engine. */
+int ckmpm_adjoint_rewind(ckmpm_engine* eng, const float* vel, uint64_t nvel, const float* lambda_p,
+                         const float* mu_p);
 /* on != 0: the following substeps are recorded (the tape restarts empty) */
 int ckmpm_adjoint_record(ckmpm_engine* eng, int32_t on);
 /* zero every particle and grid adjoint */

@@ -94,3 +94,40 @@ def test_checkpoint_restore_replays_the_forward():
     np.testing.assert_allclose(x2, x1, rtol=0, atol=1e-6 * sc.dx)
     eng.free_checkpoint(ck)
     eng.close()
+
+
+@pytest.mark.gpu
+def test_rewound_engine_matches_a_new_one():
+    """simulate() keeps its engine for the next call on the same scene and rewinds it to
+    the setup state with the new velocities and Young's moduli (ckmpm_adjoint_rewind),
+    reusing its checkpoint memory: positions and gradients equal those of a freshly built
+    engine, whichever inputs the engine last ran with."""
+    from ckmpm_amd import autodiff
+
+    sc, logE_a = _case(seed=8)
+    steps, segment = 12, 5
+    rng = np.random.default_rng(3)
+    v_a = sc.models[0].v.astype(np.float64)
+    v_b = v_a + 0.1 * rng.standard_normal(v_a.shape)
+    logE_b = logE_a + 0.2 * rng.standard_normal(logE_a.shape)
+    target = torch.tensor([0.3, 0.5, 0.5], dtype=torch.float64)
+
+    def run(logE, v):
+        lE = torch.tensor(logE, requires_grad=True)
+        vv = torch.tensor(v, requires_grad=True)
+        x = autodiff.simulate(lE, vv, sc, steps, segment)
+        ((x.mean(0) - target) ** 2).sum().backward()
+        return x.detach().numpy(), lE.grad.numpy(), vv.grad.numpy()
+
+    autodiff.clear_engine_cache()
+    b_new = run(logE_b, v_b)  # built for b
+    autodiff.clear_engine_cache()
+    a_new = run(logE_a, v_a)  # built for a
+    b_rew = run(logE_b, v_b)  # a's engine rewound to b
+    a_rew = run(logE_a, v_a)  # and back to a
+    autodiff.clear_engine_cache()
+    for new, rew in ((a_new, a_rew), (b_new, b_rew)):
+        np.testing.assert_allclose(rew[0], new[0], rtol=0, atol=1e-6 * sc.dx)
+        assert _rel(rew[1], new[1]) < 1e-5 and _rel(rew[2], new[2]) < 1e-5
+    # different inputs give different results (the rewind took the new ones)
+    assert np.abs(a_new[0] - b_new[0]).max() > 1e-3 * sc.dx
