@@ -694,8 +694,8 @@ __device__ __forceinline__ void WindowNode(int r, int& g, int& lx, int& ly, int&
 // so a node's LDS total does not depend on the order of the atomics; the flush rounds
 // it once to FP32.
 //   mass      deposits are weights w in [0, 1] (the mass is a per-model constant), at
-//             s = kq = 62 - ceil(log2 n) for a block of n particles: a node total is at
-//             most n 2^kq <= 2^62.
+//             s = kq = min(50, 62 - ceil(log2 n)) for a block of n particles: a deposit
+//             is below 2^51 and a node total at most n 2^kq <= 2^62.
 //   momentum  s = kq - e with one e per launch, read before the first deposit (a
 //             workgroup-wide agreement on a scale inside the particle loop would need a
 //             barrier between the constitutive update and the deposit, which serialises
@@ -705,24 +705,28 @@ __device__ __forceinline__ void WindowNode(int r, int& g, int& lx, int& ly, int&
 //             bound b = max_r (|m v_r| + sum_c |dx K_rc|) (|w (m v + K (x_i - x_p))_r| <= b)
 //             reaches 2^(e-1) deposits its momentum with FP32 global atomics instead, so a
 //             sudden growth of the deposits is handled exactly, only more slowly.  The
-//             momentum resolution 2^(e - kq) is about 1e-14 of the largest deposit.
-// Values are scaled by exact powers of two (the particle's m v and dx K e_c once, not
-// each deposit) and truncated: a bias below one unit of the resolution.  e >= -64 keeps
-// every scale factor 2^(kq - e) and its inverse a normal FP32 number (kq <= 62).
+//             momentum resolution 2^(e - kq) is about 1e-13 of the largest deposit.
+// The conversion round(v 2^s) is one FP64 fma: for |v 2^s| < 2^51, d = v 2^s + 1.5 2^52
+// has a fixed exponent and holds the rounded integer in its mantissa, so the integer is
+// d's bit pattern minus that of 1.5 2^52 (whose low word is zero: one 32-bit subtract).
+// That is 3 VALU instructions per deposit instead of 11 for a float -> int64 cast.
+// e >= -64 keeps every scale 2^(kq - e) and its inverse a normal FP32 number.
 using LdsAcc = long long;
 constexpr int kMinExp = -64;
-__device__ __forceinline__ void LdsAdd(LdsAcc* p, float scaled) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(static_cast<long long>(scaled)));
+__device__ __forceinline__ void LdsAdd(LdsAcc* p, float v, double scale) {
+    const double d = fma(static_cast<double>(v), scale, 6755399441055744.0);  // 1.5 * 2^52
+    atomicAdd(reinterpret_cast<unsigned long long*>(p),
+              static_cast<unsigned long long>(__double_as_longlong(d)) - 0x4338000000000000ull);
 }
 
 // Deposit of one particle into both sub-grid windows: node momentum pm - sum_c kc_c t_c
-// + sum_c o_c kc_c at stencil offset o (pm = m v, kc_c = dx K e_c); weights scaled by
-// wscale.  Momentum: pm and kc already scaled to fixed point, or (SLOW) unscaled and
-// added straight to the grid M of the halo blocks.
+// + sum_c o_c kc_c at stencil offset o (pm = m v, kc_c = dx K e_c); weights at scale
+// wscale, momentum at qscale, or (SLOW) momentum added straight to the grid M of the
+// halo blocks.
 template <bool SLOW>
 __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2], const float (&pm)[3],
-                                           const float (&kc)[3][3], float wscale, const uint32_t* sHalo,
-                                           float* M) {
+                                           const float (&kc)[3][3], double wscale, double qscale,
+                                           const uint32_t* sHalo, float* M) {
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
         const Stencil32& sg = sn[g];
@@ -741,7 +745,7 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
             const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
             const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
             const int off = ox * s2 + oy * s1 + oz;
-            LdsAdd(base + off, w * wscale);
+            LdsAdd(base + off, w, wscale);
             uint32_t bi = kEmpty;
             int node = 0;
             if (SLOW) {
@@ -756,7 +760,7 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
                 if (ox) pr += kc[0][r];
                 if (oy) pr += kc[1][r];
                 if (oz) pr += kc[2][r];
-                if (!SLOW) LdsAdd(base + (r + 1) * kMWin + off, w * pr);
+                if (!SLOW) LdsAdd(base + (r + 1) * kMWin + off, w * pr, qscale);
                 else if (bi != kEmpty) unsafeAtomicAdd(M + static_cast<size_t>(bi) * kMFloats + (r + 1) * 128 + node, w * pr);
             }
         }
@@ -818,9 +822,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         }
         // fixed-point scale of this block's weights (LdsAcc)
         const uint32_t nblk = end - beg;
-        const int kq = 62 - (nblk > 1u ? 32 - __clz(nblk - 1u) : 0);
-        const float wscale = __int_as_float((kq + 127) << 23);         // 2^kq
-        const float qscale = __int_as_float((kq - mexp + 127) << 23);  // 2^(kq - mexp)
+        const int kq = min(50, 62 - (nblk > 1u ? 32 - __clz(nblk - 1u) : 0));
+        const double wscale = static_cast<double>(__int_as_float((kq + 127) << 23));         // 2^kq
+        const double qscale = static_cast<double>(__int_as_float((kq - mexp + 127) << 23));  // 2^(kq - mexp)
         float tmax = 0.0f;  // largest deposit bound of this thread's particles
         for (int i = tid; i < kMChannels * kMWin; i += THREADS) sM[i] = 0;
         __syncthreads();
@@ -1062,15 +1066,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                     } else {
                         tmax = fmaxf(tmax, bound);
                         if (bound < fast_limit) {
-#pragma unroll
-                            for (int r = 0; r < 3; ++r) {
-                                pm[r] *= qscale;
-#pragma unroll
-                                for (int c = 0; c < 3; ++c) kc[c][r] *= qscale;
-                            }
-                            DepositP2G<false>(sM, sn, pm, kc, wscale, sHalo, a.M);
+                            DepositP2G<false>(sM, sn, pm, kc, wscale, qscale, sHalo, a.M);
                         } else {
-                            DepositP2G<true>(sM, sn, pm, kc, wscale, sHalo, a.M);
+                            DepositP2G<true>(sM, sn, pm, kc, wscale, qscale, sHalo, a.M);
                             const uint64_t on = __ballot(1);
                             if (lane == static_cast<uint32_t>(__builtin_ctzll(on)))
                                 atomicAdd(&sGlobal, static_cast<uint32_t>(__popcll(on)));
