@@ -325,9 +325,9 @@ def main():
     print(f"[bench] engine set up, {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     dt = sc.dt
     # warm-up: the autotuner's substeps (two candidate workgroup sizes) count toward it
-    tune = max(1, args.warmup // 4)
+    tune = max(1, args.warmup // 2 - 2)
     eng.autotune(tune)
-    for k in range(max(0, args.warmup - 2 * tune)):
+    for k in range(max(0, args.warmup - 2 * (tune + 1))):
         eng.Substep(dt, dt)
     eng.Synchronize()
     torch.cuda.synchronize()

@@ -422,13 +422,13 @@ int ckmpm_autotune(ckmpm_engine* e, uint32_t substeps) {
     for (ModelBuf& m : e->models)
         for (hipEvent_t& ev : m.tune_ev)
             if (!ev) CK_HIP(e, hipEventCreate(&ev));
-    std::vector<double> best(nm, 1e300);
-    std::vector<int> pick(nm, 256);
+    // Rounds of one substep per candidate, interleaved so that both see the same scene
+    // state; round 0 only warms both kernels up (first launches, caches) and is not timed.
+    std::vector<std::vector<double>> tot(2, std::vector<double>(nm, 0.0));
     e->tuning = true;
-    for (int c : kCandidates) {
-        for (ModelBuf& m : e->models) m.threads = c;
-        std::vector<double> tot(nm, 0.0);
-        for (uint32_t k = 0; k < substeps; ++k) {
+    for (uint32_t round = 0; round <= substeps; ++round) {
+        for (int ci = 0; ci < 2; ++ci) {
+            for (ModelBuf& m : e->models) m.threads = kCandidates[ci];
             const double dt = e->substeps == 0 ? 0.0 : e->dt_default;
             const int rc = ckmpm_substep(e, dt, e->dt_default);
             if (rc != CKMPM_OK) {
@@ -436,20 +436,16 @@ int ckmpm_autotune(ckmpm_engine* e, uint32_t substeps) {
                 return rc;
             }
             CK_HIP(e, hipStreamSynchronize(e->stream));
+            if (round == 0) continue;
             for (size_t i = 0; i < nm; ++i) {
                 float ms = 0.0f;
                 CK_HIP(e, hipEventElapsedTime(&ms, e->models[i].tune_ev[0], e->models[i].tune_ev[1]));
-                tot[i] += ms;
+                tot[ci][i] += ms;
             }
         }
-        for (size_t i = 0; i < nm; ++i)
-            if (tot[i] < best[i]) {
-                best[i] = tot[i];
-                pick[i] = c;
-            }
     }
     e->tuning = false;
-    for (size_t i = 0; i < nm; ++i) e->models[i].threads = pick[i];
+    for (size_t i = 0; i < nm; ++i) e->models[i].threads = tot[1][i] < tot[0][i] ? kCandidates[1] : kCandidates[0];
     return ckmpm_synchronize(e);
 }
 

@@ -159,9 +159,10 @@ int ckmpm_get_phase_times(ckmpm_engine* eng, double ms[3], uint64_t* substeps);
 
 /* Tuning (no reference counterpart: its launch shapes are compile-time constants).
  * The G2P2G workgroup size of each model is 256 or 512 threads (one particle block per
- * workgroup).  ckmpm_autotune runs `substeps` real substeps per candidate (the scene
- * advances at sim.dt, the first with dt = 0) and keeps, per model, the size with the
- * least G2P2G device time. */
+ * workgroup).  ckmpm_autotune runs 1 + `substeps` rounds of one real substep per
+ * candidate, interleaved (the scene advances at sim.dt, the first with dt = 0); the
+ * first round is not timed.  Per model it keeps the size with the least G2P2G device
+ * time: 2 (substeps + 1) substeps in all. */
 int ckmpm_autotune(ckmpm_engine* eng, uint32_t substeps);
 int ckmpm_set_tuning(ckmpm_engine* eng, int32_t model, int32_t threads);
 int ckmpm_get_tuning(ckmpm_engine* eng, int32_t model, int32_t* threads);

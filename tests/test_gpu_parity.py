@@ -224,7 +224,9 @@ def test_workgroup_sizes_agree_and_autotune():
         np.testing.assert_array_equal(pa, pb)
         np.testing.assert_array_equal(oa, ob)
         np.testing.assert_allclose(b.state(m), a.state(m), rtol=0, atol=1e-5)
-    a.autotune(2)
+    s0 = a.counters().substeps
+    a.autotune(2)  # one untimed and two timed rounds of one substep per candidate
+    assert a.counters().substeps - s0 == 2 * (2 + 1)
     assert all(a.tuning(m) in (256, 512) for m in range(2))
     a.close()
     b.close()
