@@ -3,11 +3,14 @@
 // Method: the structure of McAdams, Selle, Tamstorf, Teran, Sifakis, "Computing the
 // Singular Value Decomposition of 3x3 matrices with minimal branching and elementary
 // floating point operations", UW-Madison CS TR1690 (2011):
-//   1. four sweeps of cyclic Jacobi on S = A^T A, accumulating V.  The rotations are
-//      exact Jacobi rotations (Golub & Van Loan, Matrix Computations, sec. 8.5) rather
-//      than the paper's approximate half-angle ones: in FP32 those leave relative
+//   1. up to four sweeps of cyclic Jacobi on S = A^T A, accumulating V.  The rotations
+//      are exact Jacobi rotations (Golub & Van Loan, Matrix Computations, sec. 8.5)
+//      rather than the paper's approximate half-angle ones: in FP32 those leave relative
 //      off-diagonal residues up to ~1e-3 after four sweeps when two singular values
-//      are close, while exact rotations converge fully in four;
+//      are close, while exact rotations converge fully in four.  Jacobi converges
+//      quadratically, so after two sweeps a wavefront whose every lane is diagonal to
+//      FP32 resolution skips the rest (deformation gradients near a rotation, the
+//      common case, need two);
 //   2. B = A V, columns sorted by decreasing norm (swaps negate one column so that
 //      det V stays +1);
 //   3. Givens QR of B (rows (0,1), (0,2), (1,2)); U is the product of the transposed
@@ -60,6 +63,34 @@ __device__ __forceinline__ void JacobiConjugate(float (&S)[9], float (&V)[9]) {
         const float vp = V[k * 3 + p], vq = V[k * 3 + q];
         V[k * 3 + p] = c * vp + s * vq;
         V[k * 3 + q] = -s * vp + c * vq;
+    }
+}
+
+// Every lane of the wavefront has |S_pq| <= 2^-24 min(|S_pp|, |S_qq|) in all three
+// planes: a further rotation would move the eigenvalues by at most an ulp (and rotate
+// eigenvectors only within a plane of equal eigenvalues, to which the stresses are
+// insensitive).  Wave-uniform, so the whole wavefront skips the remaining sweeps.
+__device__ __forceinline__ bool JacobiConverged(const float (&S)[9]) {
+    const float tol = 5.9604645e-8f;  // 2^-24
+    const float a = fabsf(S[0]), b = fabsf(S[4]), c = fabsf(S[8]);
+    const bool ok = fabsf(S[1]) <= tol * fminf(a, b) && fabsf(S[2]) <= tol * fminf(a, c) &&
+                    fabsf(S[5]) <= tol * fminf(b, c);
+    return __all(ok) != 0;
+}
+
+__device__ __forceinline__ void JacobiSweep(float (&S)[9], float (&V)[9]) {
+    JacobiConjugate<0, 1, 2>(S, V);
+    JacobiConjugate<0, 2, 1>(S, V);
+    JacobiConjugate<1, 2, 0>(S, V);
+}
+
+// Two sweeps, then up to two more while some lane of the wavefront is not diagonal.
+__device__ __forceinline__ void JacobiDiagonalise(float (&S)[9], float (&V)[9]) {
+    JacobiSweep(S, V);
+    JacobiSweep(S, V);
+    if (!JacobiConverged(S)) {
+        JacobiSweep(S, V);
+        if (!JacobiConverged(S)) JacobiSweep(S, V);
     }
 }
 
@@ -123,12 +154,7 @@ __device__ __forceinline__ void SignedSvd3(const float (&A)[9], Svd3& out) {
             S[j * 3 + i] = v;
         }
     float V[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
-#pragma unroll
-    for (int sweep = 0; sweep < 4; ++sweep) {
-        JacobiConjugate<0, 1, 2>(S, V);
-        JacobiConjugate<0, 2, 1>(S, V);
-        JacobiConjugate<1, 2, 0>(S, V);
-    }
+    JacobiDiagonalise(S, V);
     float B[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -172,12 +198,7 @@ __device__ __forceinline__ void SignedSvdStrain(const float (&E)[9], Svd3& out, 
             S[j * 3 + i] = v;
         }
     float V[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
-#pragma unroll
-    for (int sweep = 0; sweep < 4; ++sweep) {
-        JacobiConjugate<0, 1, 2>(S, V);
-        JacobiConjugate<0, 2, 1>(S, V);
-        JacobiConjugate<1, 2, 0>(S, V);
-    }
+    JacobiDiagonalise(S, V);
     float B[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -91,7 +91,8 @@ def test_checkpoint_restore_replays_the_forward():
     keys2, c2 = eng.layout(0)
     assert c1 == c2
     np.testing.assert_array_equal(keys1, keys2)
-    np.testing.assert_allclose(x2, x1, rtol=0, atol=1e-6 * sc.dx)
+    # global coordinates (1 ulp near 0.45 is 1e-6 dx here), order-dependent FP32 flush
+    np.testing.assert_allclose(x2, x1, rtol=0, atol=1e-5 * sc.dx)
     eng.free_checkpoint(ck)
     eng.close()
 
@@ -126,8 +127,10 @@ def test_rewound_engine_matches_a_new_one():
     b_rew = run(logE_b, v_b)  # a's engine rewound to b
     a_rew = run(logE_a, v_a)  # and back to a
     autodiff.clear_engine_cache()
+    # positions in global coordinates near 0.45 (1 ulp = 3e-8 = 1e-6 dx here); the P2G
+    # flush's FP32 global atomics are order-dependent, so allow ~10 ulps
     for new, rew in ((a_new, a_rew), (b_new, b_rew)):
-        np.testing.assert_allclose(rew[0], new[0], rtol=0, atol=1e-6 * sc.dx)
+        np.testing.assert_allclose(rew[0], new[0], rtol=0, atol=1e-5 * sc.dx)
         assert _rel(rew[1], new[1]) < 1e-5 and _rel(rew[2], new[2]) < 1e-5
     # different inputs give different results (the rewind took the new ones)
     assert np.abs(a_new[0] - b_new[0]).max() > 1e-3 * sc.dx
