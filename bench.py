@@ -352,7 +352,10 @@ def main():
     substep_bytes = sum(m.x.shape[0] * (2 * STATE_BYTES[m.material.kind] + 12) for m in sc.models) + 7168 * G
     sps = args.steps / elapsed
     out = {
-        "metric": "particle-substeps/s fwd at 8M particles (mixed fluid + sand, 512^3)",
+        "metric": ("particle-substeps/s fwd at 8M particles (mixed fluid + sand, 512^3)"
+                   if args.workload == "mixed_dam_sand" else
+                   f"particle-substeps/s fwd ({sc.name}, {N} particles, "
+                   f"{4 * sc.dim[0]}x{4 * sc.dim[1]}x{4 * sc.dim[2]} cells)"),
         "value": N * args.steps / elapsed,
         "unit": "particle-substeps/s",
         "n_gpus": 1,

@@ -178,6 +178,32 @@ def test_prestressed_fluid_global_deposits_then_fixed_point():
     assert seen == [n, n, n, n], seen
 
 
+def test_jelly_1m_full_size_lockstep_with_hash_occupancy():
+    """The 1M-particle Neo-Hookean jelly (a 50^3-cell cube at 8 ppc on a 256^3 grid,
+    64^3 blocks) at full size for 4 substeps: block lists and particle order bit-exact,
+    grid and state within the bars above, and the hash table built over each substep's
+    exterior block list slot for slot equal to the oracle's."""
+    from ckmpm_amd.hashtable import BlockHashTable
+    from oracle.pyoracle import HashOracle
+
+    sc = scenes.jelly_drop()
+    assert sc.n_particles == 1_000_000
+    cap = sc.block_capacity()
+    table, ho = BlockHashTable(sc.dim, cap), HashOracle(sc.dim, cap)
+
+    def hash_check(k, eng):
+        keys, (A, Nb, Ex) = eng.layout(0)
+        vals = np.arange(Ex, dtype=np.uint32)
+        table.Insert(keys[:Ex], vals)
+        ho.build(keys[:Ex], vals)
+        slots, stash = table.export()
+        np.testing.assert_array_equal(slots, ho.slots, err_msg=f"substep {k}")
+        np.testing.assert_array_equal(stash, ho.stash[:ho.nstash], err_msg=f"substep {k}")
+
+    _lockstep(sc, 4, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5), after=hash_check)
+    table.close()
+
+
 def test_mixed_models_lockstep():
     """Two models (fluid + sand) sharing one grid, as in the 8M mixed workload."""
     sc = scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8)
