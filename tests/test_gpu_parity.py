@@ -204,6 +204,15 @@ def test_jelly_1m_full_size_lockstep_with_hash_occupancy():
     table.close()
 
 
+def test_mixed_8m_full_size_lockstep():
+    """The benchmark scene itself at full size (5M fluid + 3M Drucker-Prager sand, 512^3
+    grid, 128^3 blocks) for its first two substeps (dt = 0, then dt): block lists and
+    particle order bit-exact, grid and state within the bars above."""
+    sc = scenes.mixed_dam_sand()
+    assert sc.n_particles == 8_000_000
+    _lockstep(sc, 2, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+
+
 def test_mixed_models_lockstep():
     """Two models (fluid + sand) sharing one grid, as in the 8M mixed workload."""
     sc = scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8)
