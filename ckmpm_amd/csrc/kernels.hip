@@ -771,7 +771,7 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
 template <int KIND, int THREADS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GKernel(G2P2GArgs a) {
     __shared__ float sV[kVChannels * kMWin];
-    __shared__ LdsAcc sM[kMChannels * kMWin];
+    __shared__ __align__(16) LdsAcc sM[kMChannels * kMWin];
     __shared__ uint32_t sHalo[8];
     __shared__ uint32_t sWave[THREADS / 64][32];
     __shared__ uint32_t sRun[32];
@@ -826,17 +826,19 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         const double wscale = static_cast<double>(__int_as_float((kq + 127) << 23));         // 2^kq
         const double qscale = static_cast<double>(__int_as_float((kq - mexp + 127) << 23));  // 2^(kq - mexp)
         float tmax = 0.0f;  // largest deposit bound of this thread's particles
-        for (int i = tid; i < kMChannels * kMWin; i += THREADS) sM[i] = 0;
+        // 16-byte clears (kMChannels * kMWin is even)
+        for (int i = 2 * tid; i < kMChannels * kMWin; i += 2 * THREADS)
+            *reinterpret_cast<uint4*>(sM + i) = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
-        // velocity halo in the same windows as the P2G accumulator (3 channels)
-        for (int i = tid; i < kVChannels * kMWin; i += THREADS) {
-            const int ch = i / kMWin, r = i % kMWin;
+        // velocity halo in the same windows as the P2G accumulator (3 channels; the
+        // window-node mapping is computed once per node)
+        for (int r = tid; r < kMWin; r += THREADS) {
             int g, lx, ly, lz;
             WindowNode(r, g, lx, ly, lz);
-            const int o = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
-            const int node = g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
-            const uint32_t bi = sHalo[o];
-            sV[i] = bi != kEmpty ? a.V[static_cast<size_t>(bi) * kVFloats + ch * 128 + node] : 0.0f;
+            const uint32_t bi = sHalo[((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2)];
+            const float* src = a.V + static_cast<size_t>(bi) * kVFloats + g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
+#pragma unroll
+            for (int ch = 0; ch < kVChannels; ++ch) sV[ch * kMWin + r] = bi != kEmpty ? src[ch * 128] : 0.0f;
         }
         if (tid == 0) {
             bool miss = false;
@@ -1101,20 +1103,35 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         for (int o = 32; o > 0; o >>= 1) tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
         if (lane == 0 && tmax > 0.0f) atomicMax(&sBlkMax, __float_as_uint(tmax));
         // ---------------- flush: one global atomic per non-zero node and channel
-        for (int i = tid; i < kMChannels * kMWin; i += THREADS) {
-            const LdsAcc qv = sM[i];
-            if (qv == 0) continue;
-            const int ch = i / kMWin, r = i % kMWin;
-            const float val = ch == 0 ? static_cast<float>(qv) * __int_as_float((127 - kq) << 23) * a.mat.mass
-                                      : static_cast<float>(qv) * __int_as_float((mexp - kq + 127) << 23);
-            if (val == 0.0f) continue;
-            int g, lx, ly, lz;
-            WindowNode(r, g, lx, ly, lz);
-            const int o = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
-            const uint32_t bi = sHalo[o];
-            if (bi == kEmpty) continue;
-            const int node = g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
-            unsafeAtomicAdd(a.M + static_cast<size_t>(bi) * kMFloats + ch * 128 + node, val);
+        // (the window-node mapping once per node; fixed point -> FP32 through an exact
+        // double, 2^-kq m for the weights, 2^(mexp-kq) for the momentum)
+        {
+            const double unit[2] = {static_cast<double>(__int_as_float((127 - kq) << 23)) * a.mat.mass,
+                                    static_cast<double>(__int_as_float((mexp - kq + 127) << 23))};
+            for (int r = tid; r < kMWin; r += THREADS) {
+                LdsAcc qv[kMChannels];
+                bool any = false;
+#pragma unroll
+                for (int ch = 0; ch < kMChannels; ++ch) {
+                    qv[ch] = sM[ch * kMWin + r];
+                    any |= qv[ch] != 0;
+                }
+                if (!any) continue;
+                int g, lx, ly, lz;
+                WindowNode(r, g, lx, ly, lz);
+                const uint32_t bi = sHalo[((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2)];
+                if (bi == kEmpty) continue;
+                float* dst = a.M + static_cast<size_t>(bi) * kMFloats + g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
+#pragma unroll
+                for (int ch = 0; ch < kMChannels; ++ch) {
+                    if (qv[ch] == 0) continue;
+                    // |q| < 2^63: hi * 2^32 + lo is exact to double rounding (2^-53)
+                    const double d = fma(static_cast<double>(static_cast<int>(qv[ch] >> 32)), 4294967296.0,
+                                         static_cast<double>(static_cast<uint32_t>(qv[ch])));
+                    const float val = static_cast<float>(d * unit[ch == 0 ? 0 : 1]);
+                    if (val != 0.0f) unsafeAtomicAdd(dst + ch * 128, val);
+                }
+            }
         }
         if (tid < 27) a.cnt[static_cast<size_t>(j) * kDirections + tid] = sRun[tid];
         __syncthreads();
