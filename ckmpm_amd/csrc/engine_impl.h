@@ -54,6 +54,12 @@ struct ckmpm_engine {
     ckmpm_sim_desc sim{};
     uint32_t cap = 0;
     hipStream_t stream = nullptr;
+    // second stream for the rebin's layout extension of large layouts, forked from and
+    // joined back to `stream` inside a substep (default priority, like `stream`; created
+    // on first use)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool side_failed = false;  // creating them failed: stay on one stream
     LayoutBuf L[2];
     int cur = 0;          // current layout; 1 - cur is the previous one
     float* M[2] = {nullptr, nullptr};
