@@ -823,8 +823,10 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         // fixed-point scale of this block's weights (LdsAcc)
         const uint32_t nblk = end - beg;
         const int kq = min(50, 62 - (nblk > 1u ? 32 - __clz(nblk - 1u) : 0));
-        const double wscale = static_cast<double>(__int_as_float((kq + 127) << 23));         // 2^kq
-        const double qscale = static_cast<double>(__int_as_float((kq - mexp + 127) << 23));  // 2^(kq - mexp)
+        // powers of two built from their bit patterns: integer (scalar) arithmetic, so the
+        // block-uniform scales stay in scalar registers across the particle loop
+        const double wscale = __longlong_as_double(static_cast<long long>(kq + 1023) << 52);         // 2^kq
+        const double qscale = __longlong_as_double(static_cast<long long>(kq - mexp + 1023) << 52);  // 2^(kq - mexp)
         float tmax = 0.0f;  // largest deposit bound of this thread's particles
         // 16-byte clears (kMChannels * kMWin is even)
         for (int i = 2 * tid; i < kMChannels * kMWin; i += 2 * THREADS)
