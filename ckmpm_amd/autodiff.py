@@ -6,6 +6,8 @@ E = exp(log_E) (Poisson ratio fixed) and per-particle initial velocity v0, and r
 the final positions in input order.  Its backward replays the run from checkpoints
 taken every `segment` substeps, recording each segment's substeps and undoing them with
 the adjoint kernels (ckmpm_adjoint_*), then maps (lambda-bar, mu-bar) to log_E-bar.
+The forward records the last segment as it runs, so the backward undoes that segment
+at once and replays only the earlier ones.
 With segment=None the forward records every substep (a tape of all of them, which
 HBM holds for scenes of this size) and the backward undoes them without replaying.
 All arithmetic is the library's (FP32 kernels); torch only carries the tensors.
@@ -92,14 +94,15 @@ class _Simulate(torch.autograd.Function):
         else:
             eng = engine_from_scene(sc, adjoint_segment=tape)
         cks = []
-        if full:
-            eng.adjoint_record(True)
+        # first substep of the recorded tail: the whole run, or the last segment
+        tail = 0 if full else (steps - 1) // segment * segment
         for k in range(steps):
-            if not full and k % segment == 0:
+            if k == tail:
+                eng.adjoint_record(True)
+            elif not full and k % segment == 0:
                 cks.append(eng.checkpoint())
             eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
-        if full:
-            eng.adjoint_record(False)
+        eng.adjoint_record(False)
         eng.Synchronize()
         x = eng.positions_input_order(0)
         ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment, ctx.tape = eng, cks, sc, steps, segment, tape
@@ -111,9 +114,9 @@ class _Simulate(torch.autograd.Function):
         eng, cks, sc, steps, seg = ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment
         eng.adjoint_reset()
         eng.adjoint_seed(0, gx.detach().cpu().numpy().astype(np.float32))
-        if seg is None:
-            for _ in range(steps):
-                eng.adjoint_substep()
+        tail = 0 if seg is None else (steps - 1) // seg * seg
+        for _ in range(tail, steps):
+            eng.adjoint_substep()
         for s in reversed(range(len(cks))):
             eng.restore(cks[s])
             eng.adjoint_record(True)

@@ -134,3 +134,31 @@ def test_rewound_engine_matches_a_new_one():
         assert _rel(rew[1], new[1]) < 1e-5 and _rel(rew[2], new[2]) < 1e-5
     # different inputs give different results (the rewind took the new ones)
     assert np.abs(a_new[0] - b_new[0]).max() > 1e-3 * sc.dx
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("segment", [4, 5, 12, 20])
+def test_checkpointed_gradients_match_the_full_tape(segment):
+    """The forward records the last segment (steps 8..11 for segment 4, 10..11 for 5, the
+    whole run for 12 and 20) and the backward replays only the segments before it: the
+    gradients equal those of the run recorded whole (segment=None)."""
+    from ckmpm_amd import autodiff
+
+    sc, logE0 = _case(seed=9)
+    steps = 12
+    v00 = sc.models[0].v.astype(np.float64)
+    target = torch.tensor([0.3, 0.5, 0.5], dtype=torch.float64)
+
+    def run(seg):
+        lE = torch.tensor(logE0, requires_grad=True)
+        vv = torch.tensor(v00, requires_grad=True)
+        x = autodiff.simulate(lE, vv, sc, steps, seg)
+        ((x.mean(0) - target) ** 2).sum().backward()
+        return x.detach().numpy(), lE.grad.numpy(), vv.grad.numpy()
+
+    full = run(None)
+    ck = run(segment)
+    autodiff.clear_engine_cache()
+    np.testing.assert_allclose(ck[0], full[0], rtol=0, atol=1e-5 * sc.dx)
+    assert np.abs(full[1]).max() > 0 and np.abs(full[2]).max() > 0
+    assert _rel(ck[1], full[1]) < 1e-5 and _rel(ck[2], full[2]) < 1e-5
