@@ -1,4 +1,5 @@
-"""Build libckmpm.so (HIP, gfx950) in-tree with hipcc."""
+"""Build libckmpm.so (HIP, gfx950) in-tree with hipcc, and the ckmpm_scene driver (host
+C++ over include/ckmpm.hpp, linked to the library) with the host compiler."""
 from __future__ import annotations
 
 import os
@@ -8,6 +9,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libckmpm.so")
+DRIVER = os.path.join(HERE, "ckmpm_scene")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+CXX = os.environ.get("CXX", "g++")
 SOURCES = ["kernels.hip", "engine.hip", "hash.hip", "adjoint.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -35,6 +39,13 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    tmp = DRIVER + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, os.path.join(CSRC, "ckmpm_scene.cpp"),
+           "-o", tmp, "-L", HERE, "-lckmpm", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
     return LIB
 
 

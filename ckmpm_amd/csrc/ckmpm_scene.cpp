@@ -1,0 +1,213 @@
+// ckmpm_scene: a scene main over the C++ facade (include/ckmpm.hpp).
+//
+// The reference runs each scene from a compiled main that builds its models (positions
+// read from raw float32 xyz assets, e.g. tests/chimney_ball/test_chimney_ball.cuh:140-188)
+// and calls Initialize, InitializeParticle, InitialSetup and Simulate
+// (tests/colliding_sphere/test_colliding_sphere.cu:5-19).  This program does the same for
+// a scene given on the command line, so the reference's assets run without a recompile.
+// Two colliding spheres, on one command line:
+//
+//   ckmpm_scene --domain 16,16,16 --dx 0.015625 --dt 1e-4
+//       --material fc --E 1e4 --nu 0.3 --mass 2.4e-4 --volume 4.8e-7 --velocity 1,0,0
+//       --positions left.bin
+//       --material fc --E 1e4 --nu 0.3 --mass 2.4e-4 --volume 4.8e-7 --velocity -1,0,0
+//       --positions right.bin
+//       --substeps 100 --out result/model
+//
+// Every --positions closes one model with the material options given before it.  The
+// final positions of model i go to <out>i.bin in the same raw format (storage order).
+// One JSON line on stdout reports the run.  Exit status 0 on success, 2 on a usage error,
+// 1 on a library error (its CKMPM_ERR_* code and message on stderr).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ckmpm.hpp"
+
+namespace {
+
+const char kUsage[] =
+    "usage: ckmpm_scene --domain DX,DY,DZ --dx H [scene options] MODEL... (--substeps K | --frames F)\n"
+    "scene options:\n"
+    "  --offset OX,OY,OZ   domain offset in blocks (0,0,0)\n"
+    "  --dt T              default and maximum dt (1e-4)\n"
+    "  --gravity G         acceleration along y (0)\n"
+    "  --mass-clamp M      grid nodes with mass <= M get zero velocity (0)\n"
+    "  --cfl C             CFL number of the frame loop's dt rule (0.5)\n"
+    "  --fps F             frames per second of the frame loop (48)\n"
+    "  --capacity B        block capacity of each layout (65536)\n"
+    "  --autotune K        first pick each model's G2P2G workgroup size (advances the scene)\n"
+    "  --out PREFIX        write PREFIX<i>.bin: final positions of model i, float32 xyz\n"
+    "a model (options before the --positions that closes it):\n"
+    "  --material fc|nh|dp|fluid\n"
+    "  --lambda L --mu M   or  --E E --nu NU   (solids)\n"
+    "  --friction DEG      Drucker-Prager friction angle (30)\n"
+    "  --bulk K --gamma G --viscosity V --J0 J   (fluid)\n"
+    "  --mass M --volume V particle mass and volume\n"
+    "  --velocity VX,VY,VZ one initial velocity for all particles (0,0,0)\n"
+    "  --velocities FILE   per-particle initial velocities, float32 xyz\n"
+    "  --positions FILE    particle positions, float32 xyz (the reference's .bin assets)\n";
+
+struct UsageError {
+    std::string what;
+};
+
+double Number(const char* opt, const char* s) {
+    char* end = nullptr;
+    const double v = std::strtod(s, &end);
+    if (end == s || *end != '\0') throw UsageError{std::string(opt) + " needs a number, got '" + s + "'"};
+    return v;
+}
+
+std::vector<double> Numbers(const char* opt, const char* s, size_t count) {
+    std::vector<double> out;
+    std::string cur;
+    for (const char* p = s;; ++p) {
+        if (*p == ',' || *p == '\0') {
+            out.push_back(Number(opt, cur.c_str()));
+            cur.clear();
+            if (*p == '\0') break;
+        } else {
+            cur += *p;
+        }
+    }
+    if (out.size() != count) throw UsageError{std::string(opt) + " needs " + std::to_string(count) + " comma-separated numbers"};
+    return out;
+}
+
+struct PendingModel {
+    std::string material;
+    double lambda = -1, mu = -1, E = -1, nu = -1, friction = 30;
+    double bulk = 0, gamma = 0, viscosity = 0, J0 = 1;
+    double mass = 0, volume = 0;
+    std::vector<float> velocities{0.0f, 0.0f, 0.0f};
+};
+
+ckmpm::Model Close(const PendingModel& p, const std::string& positions) {
+    ckmpm::Model m;
+    if (p.material != "fc" && p.material != "nh" && p.material != "dp" && p.material != "fluid")
+        throw UsageError{"--positions " + positions + ": --material must be fc, nh, dp or fluid"};
+    float lambda = static_cast<float>(p.lambda), mu = static_cast<float>(p.mu);
+    if (p.material != "fluid") {
+        if (p.E > 0) {
+            const auto lm = ckmpm::LameParameters(p.E, p.nu);
+            lambda = lm[0];
+            mu = lm[1];
+        } else if (p.lambda < 0 || p.mu < 0) {
+            throw UsageError{"a solid needs --lambda and --mu, or --E and --nu"};
+        }
+    }
+    if (p.material == "fc") m.material = ckmpm::Material::FixedCorotated(lambda, mu);
+    else if (p.material == "nh") m.material = ckmpm::Material::NeoHookean(lambda, mu);
+    else if (p.material == "dp") m.material = ckmpm::Material::DruckerPrager(lambda, mu, static_cast<float>(p.friction));
+    else
+        m.material = ckmpm::Material::Fluid(static_cast<float>(p.bulk), static_cast<float>(p.gamma),
+                                            static_cast<float>(p.viscosity), static_cast<float>(p.J0));
+    if (!(p.mass > 0) || !(p.volume > 0)) throw UsageError{"--positions " + positions + ": --mass and --volume must be > 0"};
+    m.mass = static_cast<float>(p.mass);
+    m.volume = static_cast<float>(p.volume);
+    m.positions = ckmpm::ReadRawPositions(positions);
+    m.velocities = p.velocities;
+    return m;
+}
+
+int Run(int argc, char** argv) {
+    ckmpm::Domain domain;
+    ckmpm::Config config;
+    bool have_domain = false;
+    long substeps = -1, frames = -1, autotune = 0;
+    std::string out;
+    std::vector<ckmpm::Model> models;
+    PendingModel pend;
+    for (int i = 1; i < argc; ++i) {
+        const char* opt = argv[i];
+        if (!std::strcmp(opt, "--help") || !std::strcmp(opt, "-h")) {
+            std::fputs(kUsage, stdout);
+            return 0;
+        }
+        if (i + 1 >= argc) throw UsageError{std::string(opt) + " needs a value"};
+        const char* val = argv[++i];
+        const std::string o(opt);
+        if (o == "--domain" || o == "--offset") {
+            const auto v = Numbers(opt, val, 3);
+            for (int k = 0; k < 3; ++k) (o == "--domain" ? domain.dim : domain.offset)[k] = static_cast<int32_t>(v[k]);
+            have_domain |= o == "--domain";
+        } else if (o == "--dx") config.dx = static_cast<float>(Number(opt, val));
+        else if (o == "--dt") config.dt = static_cast<float>(Number(opt, val));
+        else if (o == "--gravity") config.gravity = static_cast<float>(Number(opt, val));
+        else if (o == "--mass-clamp") config.mass_clamp = static_cast<float>(Number(opt, val));
+        else if (o == "--cfl") config.cfl = static_cast<float>(Number(opt, val));
+        else if (o == "--fps") config.fps = static_cast<uint32_t>(Number(opt, val));
+        else if (o == "--capacity") config.max_active_block_count = static_cast<uint32_t>(Number(opt, val));
+        else if (o == "--substeps") substeps = static_cast<long>(Number(opt, val));
+        else if (o == "--frames") frames = static_cast<long>(Number(opt, val));
+        else if (o == "--autotune") autotune = static_cast<long>(Number(opt, val));
+        else if (o == "--out") out = val;
+        else if (o == "--material") pend.material = val;
+        else if (o == "--lambda") pend.lambda = Number(opt, val);
+        else if (o == "--mu") pend.mu = Number(opt, val);
+        else if (o == "--E") pend.E = Number(opt, val);
+        else if (o == "--nu") pend.nu = Number(opt, val);
+        else if (o == "--friction") pend.friction = Number(opt, val);
+        else if (o == "--bulk") pend.bulk = Number(opt, val);
+        else if (o == "--gamma") pend.gamma = Number(opt, val);
+        else if (o == "--viscosity") pend.viscosity = Number(opt, val);
+        else if (o == "--J0") pend.J0 = Number(opt, val);
+        else if (o == "--mass") pend.mass = Number(opt, val);
+        else if (o == "--volume") pend.volume = Number(opt, val);
+        else if (o == "--velocity") {
+            const auto v = Numbers(opt, val, 3);
+            pend.velocities.assign({static_cast<float>(v[0]), static_cast<float>(v[1]), static_cast<float>(v[2])});
+        } else if (o == "--velocities") pend.velocities = ckmpm::ReadRawPositions(val);
+        else if (o == "--positions") {
+            models.push_back(Close(pend, val));
+            pend = PendingModel{};
+        } else throw UsageError{"unknown option " + o};
+    }
+    if (!have_domain || !(config.dx > 0)) throw UsageError{"--domain and --dx are required"};
+    if (models.empty()) throw UsageError{"no model: give --positions FILE after each model's options"};
+    if ((substeps < 0) == (frames < 0)) throw UsageError{"give exactly one of --substeps and --frames"};
+
+    ckmpm::Engine engine(domain, config);
+    engine.InitializeParticle(models);
+    engine.InitialSetup();
+    if (autotune > 0) engine.Autotune(static_cast<uint32_t>(autotune));
+    engine.Synchronize();
+    uint64_t particles = 0;
+    for (const ckmpm::Model& m : models) particles += m.count();
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t before = engine.Counters().substeps;
+    if (substeps >= 0) engine.Substeps(static_cast<uint32_t>(substeps));
+    else engine.Simulate(static_cast<uint32_t>(frames));
+    engine.Synchronize();
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const ckmpm_counters c = engine.Counters();
+    const uint64_t ran = c.substeps - before;
+    if (!out.empty())
+        for (size_t i = 0; i < models.size(); ++i)
+            ckmpm::WriteRawPositions(out + std::to_string(i) + ".bin", engine.Positions(static_cast<int32_t>(i)));
+    std::printf(
+        "{\"particles\": %llu, \"models\": %zu, \"substeps\": %llu, \"seconds\": %.6f, "
+        "\"particle_substeps_per_s\": %.6e, \"blocks\": [%u, %u, %u], \"max_speed\": %.6e, \"flags\": %u}\n",
+        static_cast<unsigned long long>(particles), models.size(), static_cast<unsigned long long>(ran), sec,
+        sec > 0 ? static_cast<double>(particles) * static_cast<double>(ran) / sec : 0.0, c.active_blocks,
+        c.neighbor_blocks, c.exterior_blocks, static_cast<double>(c.max_speed), c.flags);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        return Run(argc, argv);
+    } catch (const UsageError& e) {
+        std::fprintf(stderr, "ckmpm_scene: %s\n%s", e.what.c_str(), kUsage);
+        return 2;
+    } catch (const ckmpm::Error& e) {
+        std::fprintf(stderr, "ckmpm_scene: %s (CKMPM_ERR %d)\n", e.what(), e.code());
+        return 1;
+    }
+}
