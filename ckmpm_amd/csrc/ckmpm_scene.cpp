@@ -16,8 +16,11 @@
 //
 // Every --positions closes one model with the material options given before it.  The
 // final positions of model i go to <out>i.bin in the same raw format (storage order).
+// With --conservation DIR the per-substep conservation metric of model i goes to
+// DIR/conservationMetric_model_<i>.bin, the reference's file (mpm_engine.cuh:494-603).
 // One JSON line on stdout reports the run.  Exit status 0 on success, 2 on a usage error,
 // 1 on a library error (its CKMPM_ERR_* code and message on stderr).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +44,7 @@ const char kUsage[] =
     "  --capacity B        block capacity of each layout (65536)\n"
     "  --autotune K        first pick each model's G2P2G workgroup size (advances the scene)\n"
     "  --out PREFIX        write PREFIX<i>.bin: final positions of model i, float32 xyz\n"
+    "  --conservation DIR  write DIR/conservationMetric_model_<i>.bin: 13 floats per substep\n"
     "a model (options before the --positions that closes it):\n"
     "  --material fc|nh|dp|fluid\n"
     "  --lambda L --mu M   or  --E E --nu NU   (solids)\n"
@@ -119,7 +123,7 @@ int Run(int argc, char** argv) {
     ckmpm::Config config;
     bool have_domain = false;
     long substeps = -1, frames = -1, autotune = 0;
-    std::string out;
+    std::string out, cons_dir;
     std::vector<ckmpm::Model> models;
     PendingModel pend;
     for (int i = 1; i < argc; ++i) {
@@ -146,6 +150,7 @@ int Run(int argc, char** argv) {
         else if (o == "--frames") frames = static_cast<long>(Number(opt, val));
         else if (o == "--autotune") autotune = static_cast<long>(Number(opt, val));
         else if (o == "--out") out = val;
+        else if (o == "--conservation") cons_dir = val;
         else if (o == "--material") pend.material = val;
         else if (o == "--lambda") pend.lambda = Number(opt, val);
         else if (o == "--mu") pend.mu = Number(opt, val);
@@ -178,12 +183,39 @@ int Run(int argc, char** argv) {
     engine.Synchronize();
     uint64_t particles = 0;
     for (const ckmpm::Model& m : models) particles += m.count();
+    // conservation records: appended to the files after the substeps, or after every frame
+    // (the frame loop's substep count depends on the CFL dt)
+    std::vector<std::FILE*> cons;
+    for (size_t i = 0; !cons_dir.empty() && i < models.size(); ++i) {
+        const std::string path = cons_dir + "/conservationMetric_model_" + std::to_string(i) + ".bin";
+        cons.push_back(std::fopen(path.c_str(), "wb"));
+        if (!cons.back()) throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot create " + path);
+    }
+    const uint32_t cons_cap = substeps >= 0 ? static_cast<uint32_t>(std::max(substeps, 1L)) : 1u << 20;
+    auto flush_cons = [&]() {
+        for (size_t i = 0; i < cons.size(); ++i) {
+            const std::vector<float> r = engine.Conservation(static_cast<int32_t>(i));
+            if (!r.empty() && std::fwrite(r.data(), 4, r.size(), cons[i]) != r.size())
+                throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot write the conservation records");
+        }
+        engine.SetConservation(cons_cap);  // clears the log
+    };
+    if (!cons.empty()) engine.SetConservation(cons_cap);
     const auto t0 = std::chrono::steady_clock::now();
     const uint64_t before = engine.Counters().substeps;
-    if (substeps >= 0) engine.Substeps(static_cast<uint32_t>(substeps));
-    else engine.Simulate(static_cast<uint32_t>(frames));
+    if (substeps >= 0) {
+        engine.Substeps(static_cast<uint32_t>(substeps));
+    } else {
+        for (long f = 0; f < frames; ++f) {
+            engine.Simulate(1);
+            if (!cons.empty()) flush_cons();
+        }
+    }
     engine.Synchronize();
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (substeps >= 0 && !cons.empty()) flush_cons();
+    for (std::FILE* f : cons)
+        if (std::fclose(f) != 0) throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot write the conservation records");
     const ckmpm_counters c = engine.Counters();
     const uint64_t ran = c.substeps - before;
     if (!out.empty())

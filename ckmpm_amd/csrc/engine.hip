@@ -409,12 +409,18 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
 int ckmpm_substep(ckmpm_engine* e, double dt, double next_dt) {
     CK_TRY(CheckReady(e));
     CK_TRY(AdjointCheckTape(e));
+    CK_TRY(ConservationCheck(e));
     PhaseGrid(e, dt);
+    ConservationAfterGrid(e);
     // recorded after the grid phase, which leaves every recorded input unchanged and
     // provides the grid velocities of the substep
     CK_TRY(AdjointBeforeSubstep(e, dt, next_dt));
     PhaseParticles(e, dt, next_dt);
     AdjointAfterSubstep(e);
+    // simulated time as the frame loop keeps it (mpm_engine.cuh:870-871: the timestamp
+    // advances by the next substep's dt)
+    e->cur_dt = next_dt;
+    e->time += next_dt;
     const hipError_t r = hipGetLastError();
     if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
     return CKMPM_OK;
@@ -428,7 +434,9 @@ int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
         const double frame_end = e->time + spf;
         while (e->time < frame_end) {
             CK_TRY(AdjointCheckTape(e));
+            CK_TRY(ConservationCheck(e));
             PhaseGrid(e, e->cur_dt);
+            ConservationAfterGrid(e);
             Status st;
             LayoutCounts lc;
             CK_TRY(ReadStatus(e, st, lc));
@@ -657,6 +665,7 @@ int ckmpm_destroy(ckmpm_engine* e) {
     if (!e) return CKMPM_ERR_INVALID;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     AdjointDestroy(e);
+    ConservationDestroy(e);
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
     for (ModelBuf& m : e->models)
         for (hipEvent_t ev : m.tune_ev)

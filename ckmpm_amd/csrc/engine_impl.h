@@ -48,6 +48,7 @@ inline int AttrCount(int kind) {
 using namespace ckmpm;
 
 struct AdjointState;  // adjoint.hip
+struct ConservationLog;  // conservation.hip
 
 struct ckmpm_engine {
     DomainDev dom{};
@@ -92,6 +93,8 @@ struct ckmpm_engine {
     size_t ev_used = 0;
     // adjoint support (null unless ckmpm_adjoint_enable was called)
     AdjointState* adj = nullptr;
+    // per-substep conservation metric (null unless ckmpm_set_conservation enabled it)
+    ConservationLog* cons = nullptr;
 };
 
 namespace ckmpm {
@@ -130,6 +133,13 @@ int AdjointCheckTape(ckmpm_engine* e);
 // identity arrays of a model for the next substep (null, null without differentiation)
 void AdjointIdBuffers(ckmpm_engine* e, size_t model, const uint32_t** in, uint32_t** out);
 void AdjointDestroy(ckmpm_engine* e);
+
+// conservation metric hooks (conservation.hip); no-ops unless ckmpm_set_conservation
+// enabled it.  Check before a substep starts (error when the log is full); record after
+// its grid update, at the simulated time e->time.
+int ConservationCheck(ckmpm_engine* e);
+void ConservationAfterGrid(ckmpm_engine* e);
+void ConservationDestroy(ckmpm_engine* e);
 
 #define CK_TRY(x)                   \
     do {                            \

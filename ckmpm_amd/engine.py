@@ -145,6 +145,21 @@ class MPMEngine:
         self._check(self._lib.ckmpm_get_phase_times(self._h, ms, ctypes.byref(n)), "get_phase_times")
         return tuple(ms), int(n.value)
 
+    def set_conservation(self, capacity: int):
+        """Record the per-substep conservation metric (up to `capacity` substeps; 0: off)."""
+        self._check(self._lib.ckmpm_set_conservation(self._h, int(capacity)), "ckmpm_set_conservation")
+
+    def conservation(self, model: int) -> np.ndarray:
+        """(n, 13) float32 records of the reference's conservationMetric file: time,
+        Eulerian linear / angular momentum (zero), Lagrangian linear / angular momentum."""
+        n = ctypes.c_uint64()
+        self._check(self._lib.ckmpm_get_conservation(self._h, model, None, 0, ctypes.byref(n)), "get_conservation")
+        out = np.zeros((n.value, 13), np.float32)
+        if n.value:
+            self._check(self._lib.ckmpm_get_conservation(self._h, model, _ptr(out, _f32p), n.value, ctypes.byref(n)),
+                        "ckmpm_get_conservation")
+        return out
+
     def close(self):
         if self._h:
             self._lib.ckmpm_destroy(self._h)

@@ -108,7 +108,9 @@ int ckmpm_initial_setup(ckmpm_engine* eng);
 /* One iteration of the Simulate substep loop (mpm_engine.cuh:519-876) with
  * caller-chosen dt (advection) and next_dt (stress deposit): grid update with dt,
  * G2P2G, rebin.  Enqueued on the engine's stream; returns without a host sync.
- * The reference's first iteration has dt = 0 (mpm_engine.cuh:479). */
+ * The reference's first iteration has dt = 0 (mpm_engine.cuh:479).  The engine's
+ * simulated time then advances by next_dt, as the frame loop's timestamp does
+ * (mpm_engine.cuh:870-871). */
 int ckmpm_substep(ckmpm_engine* eng, double dt, double next_dt);
 
 /* Simulate's frame loop with the reference dt rules (mpm_engine.cuh:514-876):
@@ -148,6 +150,24 @@ int ckmpm_import_state(ckmpm_engine* eng, int32_t model, const float* soa);
 int ckmpm_export_grid_m(ckmpm_engine* eng, float* grid, uint64_t capacity_blocks);
 /* grid velocity of the current layout, [block][vx,vy,vz][node 128] */
 int ckmpm_export_grid_v(ckmpm_engine* eng, float* grid, uint64_t capacity_blocks);
+
+/* Simulate(config, path, collectConservationMetric = true) (mpm_engine.cuh:494-603;
+ * kernel CollectConservationMetric, mpm_algorithm.cuh:1083-1357): after the grid update
+ * of every ckmpm_substep / ckmpm_run_frames substep, per model, the G2P velocity v_p of
+ * each particle on the updated grid at its position x_p, summed into the Lagrangian
+ * linear momentum sum v_p and angular momentum sum x_p x v_p (per unit mass, as the
+ * reference).  Recorded on the device without a host sync (the reference syncs each
+ * substep); batched runs (ckmpm_batch_run) record nothing.  capacity = substeps kept; 0
+ * turns recording off.  Enabling clears the log; a substep that finds it full fails with
+ * CKMPM_ERR_CAPACITY before it starts. */
+int ckmpm_set_conservation(ckmpm_engine* eng, uint32_t capacity);
+/* The model's records, n of them, in the layout of the reference's
+ * conservationMetric_model_<i>.bin (mpm_engine.cuh:597-602): 13 floats per substep =
+ * simulated time, Eulerian linear and angular momentum (zero: the reference never fills
+ * them), Lagrangian linear and angular momentum.  records holds capacity substeps
+ * (13 floats each); capacity < n is an error; records = NULL only sets n.  Waits for
+ * the engine's stream. */
+int ckmpm_get_conservation(ckmpm_engine* eng, int32_t model, float* records, uint64_t capacity, uint64_t* n);
 
 /* Per-phase device time, measured with hipEvents on the engine's stream, like the
  * reference's MPMBenchmark registry (mpm_benchmark.h:12-79; phases "G2P2G (Cuda)",

@@ -227,6 +227,18 @@ class Engine {
         Call(ckmpm_get_positions(e_, model, xyz.data(), n, &n), "ckmpm_get_positions");
         return xyz;
     }
+    /* Simulate(..., collectConservationMetric = true): record the per-substep metric of
+     * every model for up to `capacity` substeps (0: off); enabling clears the log */
+    void SetConservation(uint32_t capacity) { Call(ckmpm_set_conservation(e_, capacity), "ckmpm_set_conservation"); }
+    /* the model's records, 13 floats per substep, as the reference's
+     * conservationMetric_model_<i>.bin holds them */
+    std::vector<float> Conservation(int32_t model) {
+        uint64_t n = 0;
+        Call(ckmpm_get_conservation(e_, model, nullptr, 0, &n), "ckmpm_get_conservation");
+        std::vector<float> r(13 * n);
+        if (n) Call(ckmpm_get_conservation(e_, model, r.data(), n, &n), "ckmpm_get_conservation");
+        return r;
+    }
     ckmpm_engine* handle() const noexcept { return e_; }
 
    private:

@@ -547,6 +547,44 @@ inline void StoreState(Model& m, uint32_t t, const double* x, const double* F, d
     }
 }
 
+// ---------------------------------------------------------------- conservation metric
+// CollectConservationMetric (mpm_algorithm.cuh:1083-1357), launched after the grid
+// update of each substep (mpm_engine.cuh:570-603): for each particle of model mi at its
+// current position x, the G2P velocity v = 0.5 sum_g sum_i w v_i on the updated grid
+// (:1270-1331), summed as linear sum v and angular sum x cross v (:1334-1348), per unit
+// mass.  out = {linear xyz, angular xyz}; the Eulerian entries the reference leaves
+// zero are not part of it.
+void Conservation(Engine& e, int mi, double* out) {
+    const Layout& L = e.cur;
+    const Model& m = e.models[mi];
+    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    for (uint32_t j = 0; j < L.A; ++j)
+        for (uint32_t t = m.off[j]; t < m.off[j + 1]; ++t) {
+            const uint32_t s = m.perm[t] & kSlotMask;
+            double x[3], v[3] = {0, 0, 0};
+            for (int a = 0; a < 3; ++a) x[a] = m.st[a * static_cast<size_t>(m.n) + s];
+            for (int g = 0; g < 2; ++g) {
+                Stencil st;
+                MakeStencil(e, x, g, st);
+                for (int o = 0; o < 8; ++o) {
+                    const int d3[3] = {(o >> 2) & 1, (o >> 1) & 1, o & 1};
+                    const double w = st.w[0][d3[0]] * st.w[1][d3[1]] * st.w[2][d3[2]];
+                    int local;
+                    const uint32_t bi = NodeBlock(e, L, st.c[0] + d3[0], st.c[1] + d3[1], st.c[2] + d3[2], local);
+                    if (bi == kEmpty) continue;
+                    for (int a = 0; a < 3; ++a) v[a] += w * e.V[static_cast<size_t>(bi) * 384 + 128 * a + g * 64 + local];
+                }
+            }
+            for (int a = 0; a < 3; ++a) v[a] *= 0.5;
+            out[0] += v[0];
+            out[1] += v[1];
+            out[2] += v[2];
+            out[3] += x[1] * v[2] - x[2] * v[1];
+            out[4] += x[2] * v[0] - x[0] * v[2];
+            out[5] += x[0] * v[1] - x[1] * v[0];
+        }
+}
+
 // ---------------------------------------------------------------- G2P2G
 // mpm_algorithm.cuh:1365-1845 per particle block, in list order, particles in
 // processing order (perm), FP64 throughout:
@@ -938,6 +976,17 @@ uint32_t orc_rebin(orc_engine* h, const float* const* xyz_per_model) {
 uint32_t orc_substep(orc_engine* h, double dt, double next_dt) {
     orc_physics(h, dt, next_dt);
     return orc_rebin(h, nullptr);
+}
+
+// Conservation metric of model mi on the current grid velocities and particle state:
+// call after orc_physics (which leaves both unchanged) and before orc_rebin.
+void orc_conservation(orc_engine* h, int mi, double* out) {
+    Conservation(*reinterpret_cast<Engine*>(h), mi, out);
+}
+// Replace the grid velocities of the current layout: Nb blocks x [vx,vy,vz][128]
+void orc_set_grid_v(orc_engine* h, const double* V) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    e->V.assign(V, V + static_cast<size_t>(e->cur.Nb) * 384);
 }
 
 uint32_t orc_flags(orc_engine* h) { return reinterpret_cast<Engine*>(h)->flags; }

@@ -25,6 +25,11 @@ KATs or fixtures for this path, so the pins are mathematical):
   mirror symmetry and zero net x-momentum, and a sign error in the stress term is
   caught by the energy/force direction test.
 * Rebin (canonical order): brute-force stable sort and permutation invariants.
+* Conservation metric (mpm_algorithm.cuh:1083-1357): on an imposed affine grid velocity
+  field a + w x node, G2P is exact (partition of unity, cancelling first moments), so
+  the sums equal N a + w x sum x and sum x x (a + w x x) in closed form; with a = 0 and
+  w = 0 the metric is zero, and a wrong sign, a dropped grid or a missing 1/2 fails
+  the closed form (tests/test_conservation.py).
 """
 from __future__ import annotations
 
@@ -89,6 +94,8 @@ def lib():
         L.orc_set_state_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, _f64p]
         L.orc_get_setup_order.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p]
         L.orc_set_grid_m_f32.argtypes = [ctypes.c_void_p, _f32p]
+        L.orc_conservation.argtypes = [ctypes.c_void_p, ctypes.c_int, _f64p]
+        L.orc_set_grid_v.argtypes = [ctypes.c_void_p, _f64p]
         L.orc_svd.argtypes = [_f64p, _f64p, _f64p, _f64p]
         L.orc_stress.argtypes = [ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p, ctypes.c_double, _f64p, _f64p, _f64p]
         L.orc_weight0.restype = ctypes.c_double
@@ -250,6 +257,19 @@ class Oracle:
         g = np.zeros(nb * 384)
         lib().orc_get_grid_v(self.h, _p(g, _f64p))
         return g.reshape(nb, 3, 128)
+
+    def set_grid_v(self, V: np.ndarray):
+        nb = self.counts(0)[1]
+        g = np.ascontiguousarray(V, np.float64).reshape(-1)
+        assert g.shape[0] == nb * 384
+        lib().orc_set_grid_v(self.h, _p(g, _f64p))
+
+    def conservation(self, m: int) -> np.ndarray:
+        """Lagrangian linear and angular momentum per unit mass of model m on the current
+        grid velocities (after physics(), before rebin()): 6 values."""
+        out = np.zeros(6)
+        lib().orc_conservation(self.h, m, _p(out, _f64p))
+        return out
 
     # lockstep import
     def set_layout(self, which, keys, A, Nb, Ex):

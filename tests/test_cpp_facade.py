@@ -125,11 +125,12 @@ def test_driver_matches_python_engine(case, tmp_path):
     sc = dict(_scenes())[case]
     steps = 12
     print(f"driver case {case}, {sc.n_particles} particles, {steps} substeps", flush=True)
-    r = subprocess.run(_command(sc, tmp_path) + ["--substeps", str(steps)], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run(_command(sc, tmp_path) + ["--substeps", str(steps), "--conservation", str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert f'"substeps": {steps}' in r.stdout and '"flags": 0' in r.stdout, r.stdout
     eng = engine_from_scene(sc)
+    eng.set_conservation(steps)
     for k in range(steps):
         eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
     eng.Synchronize()
@@ -139,6 +140,14 @@ def test_driver_matches_python_engine(case, tmp_path):
         assert got.shape == want.shape
         # FP32 global atomics of the P2G flush add in any order: allow a few ulps
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-5 * sc.dx)
+        # the reference's conservation file: 13 floats per substep
+        rec = np.fromfile(tmp_path / f"conservationMetric_model_{i}.bin", np.float32).reshape(-1, 13)
+        ref = eng.conservation(i)
+        assert rec.shape == (steps, 13)
+        # times: the driver steps with the FP32 Config::dt, this engine with the FP64 scene dt
+        np.testing.assert_allclose(rec[:, 0], ref[:, 0], rtol=1e-6)
+        assert (rec[:, 1:7] == 0).all()
+        np.testing.assert_allclose(rec[:, 7:], ref[:, 7:], rtol=0, atol=1e-5 * max(np.abs(ref[:, 7:]).max(), 1e-3))
     eng.close()
 
 
