@@ -18,6 +18,10 @@
 // final positions of model i go to <out>i.bin in the same raw format (storage order).
 // With --conservation DIR the per-substep conservation metric of model i goes to
 // DIR/conservationMetric_model_<i>.bin, the reference's file (mpm_engine.cuh:494-603).
+// With --export DIR and --frames, the positions of model i after frame f go to
+// DIR/model_<i>_particle_frame_<f>.txt, raw float32 xyz, as the reference's
+// exportParticlePositionAsBinary files (mpm_engine.cuh:437-447, written at the end of
+// every frame, :880-883); its partio .bgeo copies are not written (partio is absent).
 // One JSON line on stdout reports the run.  Exit status 0 on success, 2 on a usage error,
 // 1 on a library error (its CKMPM_ERR_* code and message on stderr).
 #include <algorithm>
@@ -45,6 +49,8 @@ const char kUsage[] =
     "  --autotune K        first pick each model's G2P2G workgroup size (advances the scene)\n"
     "  --out PREFIX        write PREFIX<i>.bin: final positions of model i, float32 xyz\n"
     "  --conservation DIR  write DIR/conservationMetric_model_<i>.bin: 13 floats per substep\n"
+    "  --export DIR        with --frames: after frame f write DIR/model_<i>_particle_frame_<f>.txt\n"
+    "                      (positions, float32 xyz, as the reference's binary export)\n"
     "a model (options before the --positions that closes it):\n"
     "  --material fc|nh|dp|fluid\n"
     "  --lambda L --mu M   or  --E E --nu NU   (solids)\n"
@@ -123,7 +129,7 @@ int Run(int argc, char** argv) {
     ckmpm::Config config;
     bool have_domain = false;
     long substeps = -1, frames = -1, autotune = 0;
-    std::string out, cons_dir;
+    std::string out, cons_dir, export_dir;
     std::vector<ckmpm::Model> models;
     PendingModel pend;
     for (int i = 1; i < argc; ++i) {
@@ -151,6 +157,7 @@ int Run(int argc, char** argv) {
         else if (o == "--autotune") autotune = static_cast<long>(Number(opt, val));
         else if (o == "--out") out = val;
         else if (o == "--conservation") cons_dir = val;
+        else if (o == "--export") export_dir = val;
         else if (o == "--material") pend.material = val;
         else if (o == "--lambda") pend.lambda = Number(opt, val);
         else if (o == "--mu") pend.mu = Number(opt, val);
@@ -175,6 +182,7 @@ int Run(int argc, char** argv) {
     if (!have_domain || !(config.dx > 0)) throw UsageError{"--domain and --dx are required"};
     if (models.empty()) throw UsageError{"no model: give --positions FILE after each model's options"};
     if ((substeps < 0) == (frames < 0)) throw UsageError{"give exactly one of --substeps and --frames"};
+    if (!export_dir.empty() && frames < 0) throw UsageError{"--export writes frames: use it with --frames"};
 
     ckmpm::Engine engine(domain, config);
     engine.InitializeParticle(models);
@@ -209,6 +217,10 @@ int Run(int argc, char** argv) {
         for (long f = 0; f < frames; ++f) {
             engine.Simulate(1);
             if (!cons.empty()) flush_cons();
+            for (size_t i = 0; !export_dir.empty() && i < models.size(); ++i)
+                ckmpm::WriteRawPositions(export_dir + "/model_" + std::to_string(i) + "_particle_frame_" +
+                                             std::to_string(f) + ".txt",
+                                         engine.Positions(static_cast<int32_t>(i)));
         }
     }
     engine.Synchronize();

@@ -70,11 +70,14 @@ def test_driver_usage_and_argument_errors(tmp_path):
         (["--domain", "8,8,8", "--dx", "0.03", "--material", "fc", "--mass", "1", "--volume", "1",
           "--positions", "p.bin"], "a solid needs"),
         (["--bogus", "1"], "unknown option"),
+        (["--domain", "8,8,8", "--dx", "0.03", "--material", "fluid", "--bulk", "1", "--mass", "1", "--volume", "1",
+          "--positions", "ok.bin", "--substeps", "1", "--export", "."], "use it with --frames"),
     ]
+    (tmp_path / "ok.bin").write_bytes(b"\0" * 24)   # two particles at the origin
+    (tmp_path / "odd.bin").write_bytes(b"\0" * 20)  # not whole xyz triples
     for args, msg in bad:
         r = subprocess.run([DRIVER, *args], capture_output=True, text=True, cwd=tmp_path)
         assert r.returncode == 2 and msg in r.stderr, (args, r.stderr)
-    (tmp_path / "odd.bin").write_bytes(b"\0" * 20)
     r = subprocess.run([DRIVER, "--domain", "8,8,8", "--dx", "0.03", "--material", "fluid", "--bulk", "1e4",
                         "--gamma", "7", "--mass", "1", "--volume", "1", "--positions", "odd.bin", "--substeps", "1"],
                        capture_output=True, text=True, cwd=tmp_path)
@@ -158,8 +161,12 @@ def test_driver_frame_loop_matches_simulate(tmp_path):
 
     sc = scenes.colliding_spheres()
     print("driver frame loop, colliding spheres, 1 frame", flush=True)
-    r = subprocess.run(_command(sc, tmp_path) + ["--frames", "1"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run(_command(sc, tmp_path) + ["--frames", "1", "--export", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    for i in range(len(sc.models)):  # the per-frame export of the last frame is the final state
+        np.testing.assert_array_equal(np.fromfile(tmp_path / f"model_{i}_particle_frame_0.txt", np.float32),
+                                      np.fromfile(tmp_path / f"out{i}.bin", np.float32))
     eng = engine_from_scene(sc)
     eng.Simulate(1)
     eng.Synchronize()
