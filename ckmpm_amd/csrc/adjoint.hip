@@ -1067,7 +1067,7 @@ int ckmpm_adjoint_rewind(ckmpm_engine* e, const float* vel, uint64_t nvel, const
     // (the kernel turns its order argument into the identity ordering: give it a copy in
     // the move-code buffer, which the next G2P2G overwrites)
     CK_HIP(e, hipMemcpyAsync(m.code, A->setup_order[0], sizeof(uint32_t) * m.n, hipMemcpyDeviceToDevice, st));
-    LaunchInitState(e->dom, m.kind, A->x0[0], A->host_io, A->host_io + m.n, m.pp_attr, m.code, m.n, m.mat.J0,
+    LaunchInitState(e->dom, m.kind, A->x0[0], A->host_io, A->host_io + m.n, m.pp_attr, m.code, m.n, InitAttr(m.mat),
                     m.st[e->sc], st);
     // grid mass / momentum of the new velocities on the setup layout
     const LayoutBuf& L = e->L[e->cur];

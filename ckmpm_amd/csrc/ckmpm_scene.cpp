@@ -52,10 +52,12 @@ const char kUsage[] =
     "  --export DIR        with --frames: after frame f write DIR/model_<i>_particle_frame_<f>.txt\n"
     "                      (positions, float32 xyz, as the reference's binary export)\n"
     "a model (options before the --positions that closes it):\n"
-    "  --material fc|nh|dp|fluid\n"
+    "  --material fc|nh|dp|fluid|vm|nacc\n"
     "  --lambda L --mu M   or  --E E --nu NU   (solids)\n"
     "  --friction DEG      Drucker-Prager friction angle (30)\n"
     "  --bulk K --gamma G --viscosity V --J0 J   (fluid)\n"
+    "  --yield S           von Mises yield stress\n"
+    "  --alpha0 A --beta B --xi X --M M --hardening 0|1   (Cam-clay; hardening 1)\n"
     "  --mass M --volume V particle mass and volume\n"
     "  --velocity VX,VY,VZ one initial velocity for all particles (0,0,0)\n"
     "  --velocities FILE   per-particle initial velocities, float32 xyz\n"
@@ -92,14 +94,16 @@ struct PendingModel {
     std::string material;
     double lambda = -1, mu = -1, E = -1, nu = -1, friction = 30;
     double bulk = 0, gamma = 0, viscosity = 0, J0 = 1;
+    double yield = 0, alpha0 = 0, beta = 0, xi = 0, M = 0, hardening = 1;
     double mass = 0, volume = 0;
     std::vector<float> velocities{0.0f, 0.0f, 0.0f};
 };
 
 ckmpm::Model Close(const PendingModel& p, const std::string& positions) {
     ckmpm::Model m;
-    if (p.material != "fc" && p.material != "nh" && p.material != "dp" && p.material != "fluid")
-        throw UsageError{"--positions " + positions + ": --material must be fc, nh, dp or fluid"};
+    if (p.material != "fc" && p.material != "nh" && p.material != "dp" && p.material != "fluid" && p.material != "vm" &&
+        p.material != "nacc")
+        throw UsageError{"--positions " + positions + ": --material must be fc, nh, dp, fluid, vm or nacc"};
     float lambda = static_cast<float>(p.lambda), mu = static_cast<float>(p.mu);
     if (p.material != "fluid") {
         if (p.E > 0) {
@@ -113,6 +117,10 @@ ckmpm::Model Close(const PendingModel& p, const std::string& positions) {
     if (p.material == "fc") m.material = ckmpm::Material::FixedCorotated(lambda, mu);
     else if (p.material == "nh") m.material = ckmpm::Material::NeoHookean(lambda, mu);
     else if (p.material == "dp") m.material = ckmpm::Material::DruckerPrager(lambda, mu, static_cast<float>(p.friction));
+    else if (p.material == "vm") m.material = ckmpm::Material::VonMises(lambda, mu, static_cast<float>(p.yield));
+    else if (p.material == "nacc")
+        m.material = ckmpm::Material::CamClay(lambda, mu, static_cast<float>(p.alpha0), static_cast<float>(p.beta),
+                                              static_cast<float>(p.xi), static_cast<float>(p.M), p.hardening != 0);
     else
         m.material = ckmpm::Material::Fluid(static_cast<float>(p.bulk), static_cast<float>(p.gamma),
                                             static_cast<float>(p.viscosity), static_cast<float>(p.J0));
@@ -168,6 +176,12 @@ int Run(int argc, char** argv) {
         else if (o == "--gamma") pend.gamma = Number(opt, val);
         else if (o == "--viscosity") pend.viscosity = Number(opt, val);
         else if (o == "--J0") pend.J0 = Number(opt, val);
+        else if (o == "--yield") pend.yield = Number(opt, val);
+        else if (o == "--alpha0") pend.alpha0 = Number(opt, val);
+        else if (o == "--beta") pend.beta = Number(opt, val);
+        else if (o == "--xi") pend.xi = Number(opt, val);
+        else if (o == "--M") pend.M = Number(opt, val);
+        else if (o == "--hardening") pend.hardening = Number(opt, val);
         else if (o == "--mass") pend.mass = Number(opt, val);
         else if (o == "--volume") pend.volume = Number(opt, val);
         else if (o == "--velocity") {

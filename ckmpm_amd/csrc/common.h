@@ -17,7 +17,7 @@ constexpr int kVChannels = 3;        // vx, vy, vz
 constexpr int kMFloats = kMChannels * kNodesPerBlock;  // 512 floats = 2 KiB per block
 constexpr int kVFloats = kVChannels * kNodesPerBlock;  // 384 floats = 1.5 KiB per block
 
-enum Kind : int { kFixedCorotated = 0, kNeoHookean = 1, kDruckerPrager = 2, kFluid = 3 };
+enum Kind : int { kFixedCorotated = 0, kNeoHookean = 1, kDruckerPrager = 2, kFluid = 3, kVonMises = 4, kCamClay = 5 };
 
 enum Flag : uint32_t {
     kFlagOutOfWindow = 1u << 0,
@@ -60,8 +60,15 @@ struct MaterialDev {
     float lambda, mu;
     float bulk, gamma, viscosity, J0;
     float sin_phi;
+    float yield;                // von Mises yield stress
+    float beta, xi, msq, alpha0;  // Cam-clay: beta, hardening factor, M^2, initial logJp
+    int hardening;              // Cam-clay hardening on
     float mass, volume;
 };
+
+// initial value of the material's scalar state attribute: J0 of a fluid, the initial
+// logJp (alpha0) of a Cam-clay model (mpm_algorithm.cuh:1043-1049)
+__host__ __device__ inline float InitAttr(const MaterialDev& m) { return m.kind == kCamClay ? m.alpha0 : m.J0; }
 
 __host__ __device__ inline uint32_t KeyOf(const DomainDev& d, int bx, int by, int bz) {
     return static_cast<uint32_t>((bx * d.dim[1] + by) * d.dim[2] + bz);

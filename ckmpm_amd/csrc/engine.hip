@@ -271,7 +271,7 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     if (e->models.size() >= static_cast<size_t>(kMaxModels)) return Fail(e, CKMPM_ERR_INVALID, "too many models");
     if (n == 0 || n >= (1ull << 31)) return Fail(e, CKMPM_ERR_INVALID, "particle count");
     if (nvel != 1 && nvel != n) return Fail(e, CKMPM_ERR_INVALID, "velocity count must be 1 or n");
-    if (mat->kind < 0 || mat->kind > 3) return Fail(e, CKMPM_ERR_INVALID, "material kind");
+    if (mat->kind < 0 || mat->kind > 5) return Fail(e, CKMPM_ERR_INVALID, "material kind");
     if ((lambda_p == nullptr) != (mu_p == nullptr)) return Fail(e, CKMPM_ERR_INVALID, "lambda_p and mu_p go together");
     if (lambda_p && mat->kind == kFluid) return Fail(e, CKMPM_ERR_INVALID, "per-particle Lame parameters need a solid");
     ModelBuf m;
@@ -292,6 +292,12 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     m.mat.viscosity = mat->viscosity;
     m.mat.J0 = mat->J0;
     m.mat.sin_phi = static_cast<float>(std::sin(static_cast<double>(mat->friction_angle_deg) * 3.14159265358979323846 / 180.0));
+    m.mat.yield = mat->yield_stress;
+    m.mat.beta = mat->beta;
+    m.mat.xi = mat->hardening_factor;
+    m.mat.msq = mat->m * mat->m;
+    m.mat.alpha0 = mat->alpha0;
+    m.mat.hardening = mat->hardening_on != 0;
     m.mat.mass = mass;
     m.mat.volume = volume;
     m.x0.assign(xyz, xyz + 3 * n);
@@ -358,7 +364,7 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
             CK_HIP(e, hipMemcpyAsync(lam, m.lam0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
             CK_HIP(e, hipMemcpyAsync(mu, m.mu0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
         }
-        LaunchInitState(e->dom, m.kind, xyz, lam, mu, m.pp_attr, vals, m.n, m.mat.J0, m.st[0], s);
+        LaunchInitState(e->dom, m.kind, xyz, lam, mu, m.pp_attr, vals, m.n, InitAttr(m.mat), m.st[0], s);
         CK_HIP(e, hipStreamSynchronize(s));
     }
     ExclusiveScan(e->bitmap, e->word_off, nullptr, e->nwords, e->nwords, e->total, true, e->scan_tmp, s);

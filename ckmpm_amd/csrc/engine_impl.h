@@ -38,7 +38,7 @@ struct LayoutBuf {
 };
 
 inline int AttrCount(int kind) {
-    return kind == kDruckerPrager ? 13 : (kind == kFluid ? 4 : 12);
+    return (kind == kDruckerPrager || kind == kCamClay) ? 13 : (kind == kFluid ? 4 : 12);
 }
 
 

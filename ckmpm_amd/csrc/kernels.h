@@ -45,7 +45,7 @@ void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t*
 // out = identity ordering with move 13.
 // pp_attr: attribute index of per-particle (lambda, mu), 0 when the material's are used.
 void LaunchInitState(const DomainDev& d, int kind, const float* xyz, const float* lam_p, const float* mu_p, int pp_attr,
-                     uint32_t* order, uint32_t n, float J0, float* st, hipStream_t s);
+                     uint32_t* order, uint32_t n, float init, float* st, hipStream_t s);
 void LaunchExportPositions(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, const uint32_t* off,
                            const float* st, uint32_t n, float* xyz, uint32_t grid, hipStream_t s);
 void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uint32_t n, uint32_t nvel, float mass,

@@ -284,7 +284,7 @@ __global__ void SetupOffsetsKernel(const uint32_t* sorted, uint32_t n, const uin
 // ordering becomes the identity with move 13 (no move), like InitializeAdvectionBucket
 // (mpm_algorithm.cuh:949-964).
 __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const float* lam_p, const float* mu_p,
-                                int pp_attr, uint32_t* order, uint32_t n, float J0, float* st) {
+                                int pp_attr, uint32_t* order, uint32_t n, float init, float* st) {
     const size_t N = n;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const uint32_t src = order[p];
@@ -300,11 +300,12 @@ __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const f
             st[(pp_attr + 1) * N + p] = mu_p[src];
         }
         if (kind == kFluid) {
-            st[3 * N + p] = J0 - 1.0f;  // the fluid state is q = J - 1
+            st[3 * N + p] = init - 1.0f;  // the fluid state is q = J - 1 (init = J0)
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) st[(3 + k) * N + p] = 0.0f;  // E = F - I = 0
             if (kind == kDruckerPrager) st[12 * N + p] = 0.0f;
+            if (kind == kCamClay) st[12 * N + p] = init;  // logJp = alpha0
         }
     }
 }
@@ -875,7 +876,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                 } else {
 #pragma unroll
                     for (int k = 0; k < 9; ++k) F[k] = a.st_in[(3 + k) * N + s];
-                    if (KIND == kDruckerPrager) logJp = a.st_in[12 * N + s];
+                    if (KIND == kDruckerPrager || KIND == kCamClay) logJp = a.st_in[12 * N + s];
                 }
                 float lam = a.mat.lambda, mu = a.mat.mu;
                 if (a.pp_attr) {
@@ -975,7 +976,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                     for (int k = 0; k < 9; ++k) F[k] += a.dt * (C[k] + CE[k]);
                     if (KIND == kFixedCorotated) StressFixedCorotated(lam, mu, F, tau);
                     else if (KIND == kNeoHookean) StressNeoHookean(lam, mu, F, tau);
-                    else StressDruckerPrager(lam, mu, a.mat.sin_phi, F, logJp, tau);
+                    else if (KIND == kDruckerPrager) StressDruckerPrager(lam, mu, a.mat.sin_phi, F, logJp, tau);
+                    else if (KIND == kVonMises) StressVonMises(lam, mu, a.mat.yield, F, tau);
+                    else StressCamClay(a.mat, lam, mu, F, logJp, tau);
                 }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) a.st_out[k * N + t] = x[k];
@@ -984,7 +987,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                 } else {
 #pragma unroll
                     for (int k = 0; k < 9; ++k) a.st_out[(3 + k) * N + t] = F[k];
-                    if (KIND == kDruckerPrager) a.st_out[12 * N + t] = logJp;
+                    if (KIND == kDruckerPrager || KIND == kCamClay) a.st_out[12 * N + t] = logJp;
                 }
                 if (a.pp_attr) {
                     a.st_out[a.pp_attr * N + t] = lam;
@@ -1473,8 +1476,8 @@ void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t*
     SetupOffsetsKernel<<<Blocks(a_cap + 1, kThreads), kThreads, 0, s>>>(sorted_keys, n, akeys, lc, off);
 }
 void LaunchInitState(const DomainDev& d, int kind, const float* xyz, const float* lam_p, const float* mu_p, int pp_attr,
-                     uint32_t* order, uint32_t n, float J0, float* st, hipStream_t s) {
-    InitStateKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, kind, xyz, lam_p, mu_p, pp_attr, order, n, J0, st);
+                     uint32_t* order, uint32_t n, float init, float* st, hipStream_t s) {
+    InitStateKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, kind, xyz, lam_p, mu_p, pp_attr, order, n, init, st);
 }
 void LaunchExportPositions(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, const uint32_t* off,
                            const float* st, uint32_t n, float* xyz, uint32_t grid, hipStream_t s) {
@@ -1513,6 +1516,8 @@ void LaunchG2P2GT(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
         case kFixedCorotated: G2P2GKernel<kFixedCorotated, T><<<grid, T, 0, s>>>(a); break;
         case kNeoHookean: G2P2GKernel<kNeoHookean, T><<<grid, T, 0, s>>>(a); break;
         case kDruckerPrager: G2P2GKernel<kDruckerPrager, T><<<grid, T, 0, s>>>(a); break;
+        case kVonMises: G2P2GKernel<kVonMises, T><<<grid, T, 0, s>>>(a); break;
+        case kCamClay: G2P2GKernel<kCamClay, T><<<grid, T, 0, s>>>(a); break;
         default: G2P2GKernel<kFluid, T><<<grid, T, 0, s>>>(a); break;
     }
 }

@@ -152,6 +152,171 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
         }
 }
 
+// Hencky strain of F = I + E from its signed SVD: eps_i = log s_i (log1p of the
+// precise s_i - 1), with |s_i| clamped at 1e-4 as in the Drucker-Prager model (the
+// reference's logf of a non-positive singular value is NaN).
+__device__ __forceinline__ void HenckyStrain(const Svd3& sv, const float (&d)[3], float (&eps)[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) eps[a] = sv.s[a] >= 1e-4f ? log1pf(d[a]) : logf(fmaxf(fabsf(sv.s[a]), 1e-4f));
+}
+
+// E = U diag(ns) V^T - I
+__device__ __forceinline__ void RebuildStrain(const Svd3& sv, const float (&ns)[3], float (&E)[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            E[i * 3 + j] = sv.U[i * 3 + 0] * ns[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * ns[1] * sv.V[j * 3 + 1] +
+                           sv.U[i * 3 + 2] * ns[2] * sv.V[j * 3 + 2] - (i == j ? 1.0f : 0.0f);
+}
+
+// tau = U diag(k) U^T (symmetric)
+__device__ __forceinline__ void PrincipalToTau(const Svd3& sv, const float (&k)[3], float (&tau)[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            const float v = sv.U[i * 3 + 0] * k[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * k[1] * sv.U[j * 3 + 1] +
+                            sv.U[i * 3 + 2] * k[2] * sv.U[j * 3 + 2];
+            tau[i * 3 + j] = v;
+            tau[j * 3 + i] = v;
+        }
+}
+
+// von Mises plasticity over the StVK-Hencky law: return map mpm_algorithm.cuh:450-487,
+// stress mpm_material.cuh:479-519.  Yield when |dev eps| exceeds yield / (2 mu); the
+// return moves dev eps radially back onto that radius (trace kept) and rebuilds
+// F = U exp(eps) V^T.  The reference then takes the SVD of the projected F again; its
+// singular values are exp(eps) with the same U, so tau = U diag(2 mu eps + lambda tr eps)
+// U^T directly.
+__device__ __forceinline__ void StressVonMises(float lambda, float mu, float yield, float (&E)[9], float (&tau)[9]) {
+    Svd3 sv;
+    float d[3], eps[3];
+    SignedSvdStrain(E, sv, d);
+    HenckyStrain(sv, d, eps);
+    const float tr = eps[0] + eps[1] + eps[2];
+    float eh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) eh[a] = eps[a] - tr * (1.0f / 3.0f);
+    const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+    const float over = ehn - yield / (2.0f * mu);
+    if (over > 0.0f && ehn > 0.0f) {
+        const float r = over / ehn;
+        float ns[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            eps[a] -= r * eh[a];
+            ns[a] = expf(eps[a]);
+        }
+        RebuildStrain(sv, ns, E);
+    }
+    float k[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) k[a] = 2.0f * mu * eps[a] + lambda * tr;
+    PrincipalToTau(sv, k, tau);
+}
+
+// Non-associated Cam-clay, mpm_material.cuh:521-671 (called from
+// mpm_algorithm.cuh:329-400).  kappa = 2 mu / 3 + lambda; the yield surface in (p, q)
+// is (1 + 2 beta) q^2 + M^2 (p - p_min)(p - p0) = 0 with p0 = kappa (1e-5 +
+// sinh(xi max(-logJp, 0))), p_min = -beta p0.
+//  * trial state from the SVD of F: J, s_hat = mu J^(-2/3) dev(s^2), p = -kappa/2 (J^2 - 1);
+//  * p outside [p_min, p0]: F is projected to the isotropic stretch of the cap pressure,
+//    J_new = sqrt(1 - 2 p / kappa), and logJp += log(J / J_new) when hardening;
+//  * else, when y >= 1e-4, s_hat is scaled onto the yield surface at the trial p
+//    (s_i^2 = s_hat_i c + tr(s^2)/3) and, when hardening and p lies inside the band, logJp
+//    grows by log(J / J_fake), J_fake from the point where the ray from the ellipse
+//    centre through (p, q) meets the surface.  The reference performs this update inside
+//    a 9-fold loop over matrix entries (mpm_material.cuh:612-640), so it adds 9 times;
+//    kept, for identical results;
+//  * tau = mu J_trial^(-2/3) dev(F F^T) + kappa/2 (J^2 - 1) I (the deviatoric factor
+//    uses the trial J, as the reference does).
+// The reference rebuilds a projected F as U diag(s) V (mpm_material.cuh:574-581,
+// 599-609); this uses U diag(s) V^T.  The two differ by a rotation on the right, which
+// changes neither F F^T nor the singular values, hence neither the stress nor any later
+// state the stress depends on.
+// Small strains are formed from d = s - 1 (J - 1 and s^2 - mean s^2 without cancellation).
+__device__ __forceinline__ void StressCamClay(const MaterialDev& m, float lambda, float mu, float (&E)[9], float& logJp,
+                                              float (&tau)[9]) {
+    const float kappa = 2.0f * mu / 3.0f + lambda;
+    const float beta = m.beta, msq = m.msq;
+    Svd3 sv;
+    float d[3];
+    SignedSvdStrain(E, sv, d);
+    const float p0 = kappa * (1e-5f + sinhf(m.xi * fmaxf(-logJp, 0.0f)));
+    const float pmin = -beta * p0;
+    // J - 1 and s_i^2 - 1 from d
+    const float jm1 = d[0] + d[1] + d[2] + d[0] * d[1] + d[0] * d[2] + d[1] * d[2] + d[0] * d[1] * d[2];
+    const float Je = 1.0f + jm1;
+    float bd[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bd[a] = d[a] * (2.0f + d[a]);
+    const float bmean = (bd[0] + bd[1] + bd[2]) * (1.0f / 3.0f);
+    const float devf = mu * expf(-2.0f / 3.0f * log1pf(jm1));  // mu J^(-2/3)
+    float sh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sh[a] = devf * (bd[a] - bmean);
+    const float ptr = -0.5f * kappa * jm1 * (2.0f + jm1);  // -kappa/2 (J^2 - 1)
+    const float ysh = 1.5f * (1.0f + 2.0f * beta);
+    const float yph = msq * (ptr - pmin) * (ptr - p0);
+    const float sn2 = sh[0] * sh[0] + sh[1] * sh[1] + sh[2] * sh[2];
+    const float y = ysh * sn2 + yph;
+    // stretches after the return (b = s^2 - 1 kept precise when nothing is projected)
+    float b1[3] = {bd[0], bd[1], bd[2]};
+    float j2m1 = jm1 * (2.0f + jm1);  // J^2 - 1
+    bool rebuild = false;
+    float ns[3];
+    if (ptr > p0 || ptr < pmin) {
+        const float p = ptr > p0 ? p0 : pmin;
+        const float jn2 = 1.0f - 2.0f * p / kappa;
+        const float jn = sqrtf(jn2);
+        const float sn = cbrtf(jn);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ns[a] = sn;
+            b1[a] = sn * sn - 1.0f;
+        }
+        j2m1 = jn2 - 1.0f;
+        rebuild = true;
+        if (m.hardening) logJp += logf(Je / jn);
+    } else if (y >= 1e-4f) {
+        const float c = expf(2.0f / 3.0f * log1pf(jm1)) / mu * sqrtf(-yph / ysh) / sqrtf(sn2);
+        const float tr3 = 1.0f + bmean;  // tr(s^2) / 3
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ns[a] = sqrtf(sh[a] * c + tr3);
+            b1[a] = sh[a] * c + bmean;
+        }
+        const float jn = ns[0] * ns[1] * ns[2];
+        j2m1 = jn * jn - 1.0f;
+        rebuild = true;
+        if (m.hardening && p0 > 1e-4f && ptr < p0 - 1e-4f && ptr > 1e-4f + pmin) {
+            const float pc = (1.0f - beta) * p0 * 0.5f;
+            const float qt = sqrtf(1.5f * sn2);
+            float d0 = pc - ptr, d1 = -qt;
+            const float dn = sqrtf(d0 * d0 + d1 * d1);
+            d0 /= dn;
+            d1 /= dn;
+            const float A = msq * d0 * d0 + (1.0f + 2.0f * beta) * d1 * d1;
+            const float B = msq * d0 * (2.0f * pc - p0 - pmin);
+            const float C = msq * (pc - pmin) * (pc - p0);
+            const float disc = sqrtf(B * B - 4.0f * A * C);
+            const float p1 = pc + (-B + disc) / (2.0f * A) * d0;
+            const float p2 = pc + (-B - disc) / (2.0f * A) * d0;
+            const float pf = (ptr - pc) * (p1 - pc) > 0.0f ? p1 : p2;
+            const float jf = sqrtf(fabsf(1.0f - 2.0f * pf / kappa));
+            if (jf > 1e-4f) logJp += 9.0f * logf(Je / jf);
+        }
+    }
+    if (rebuild) RebuildStrain(sv, ns, E);
+    const float b1m = (b1[0] + b1[1] + b1[2]) * (1.0f / 3.0f);
+    const float hyd = 0.5f * kappa * j2m1;
+    float k[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) k[a] = devf * (b1[a] - b1m) + hyd;
+    PrincipalToTau(sv, k, tau);
+}
+
 // Weakly compressible fluid, mpm_algorithm.cuh:539-547 and mpm_material.cuh:466-477:
 // J <- J (1 + dt tr C); p = K (J^-gamma - 1); tau = J (nu (C + C^T) - p I).
 // The state holds q = J - 1 (the reference stores J): near J = 1 an FP32 J keeps only

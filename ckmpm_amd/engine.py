@@ -16,9 +16,9 @@ import numpy as np
 
 from . import native
 from .native import CkmpmError, _f32, _ptr, _f32p, _u32p
-from .scenes import DRUCKER_PRAGER, FIXED_COROTATED, FLUID, NEO_HOOKEAN, Scene
+from .scenes import CAM_CLAY, DRUCKER_PRAGER, FIXED_COROTATED, FLUID, NEO_HOOKEAN, VON_MISES, Scene
 
-NATTR = {FIXED_COROTATED: 12, NEO_HOOKEAN: 12, DRUCKER_PRAGER: 13, FLUID: 4}
+NATTR = {FIXED_COROTATED: 12, NEO_HOOKEAN: 12, DRUCKER_PRAGER: 13, FLUID: 4, VON_MISES: 12, CAM_CLAY: 13}
 
 
 @dataclasses.dataclass
@@ -51,6 +51,12 @@ class MPMMaterial:
     viscosity: float = 0.0
     J0: float = 1.0
     friction_angle_deg: float = 30.0
+    yield_stress: float = 0.0       # VonMisesMaterialParameter (mpm_material.cuh:236-253)
+    alpha0: float = 0.0             # NonAssociatedCamClayMaterialParameter (mpm_material.cuh:189-214)
+    beta: float = 0.0
+    hardening_factor: float = 0.0
+    m: float = 0.0
+    hardening_on: bool = True
 
 
 @dataclasses.dataclass
@@ -92,7 +98,8 @@ class MPMEngine:
             v = _f32(m.velocities).reshape(-1, 3)
             mat = m.material
             md = native.MaterialDesc(mat.kind, mat.lam, mat.mu, mat.bulk, mat.gamma, mat.viscosity, mat.J0,
-                                     mat.friction_angle_deg)
+                                     mat.friction_angle_deg, mat.yield_stress, mat.alpha0, mat.beta,
+                                     mat.hardening_factor, mat.m, int(mat.hardening_on))
             lp = None if m.lam_p is None else _f32(m.lam_p)
             mp = None if m.mu_p is None else _f32(m.mu_p)
             idx = ctypes.c_int32()
@@ -361,8 +368,9 @@ def engine_from_scene(scene: Scene, capacity: Optional[int] = None, adjoint_segm
                              max_active_block_count=capacity or scene.block_capacity()))
     models = []
     for m in scene.models:
-        mat = MPMMaterial(m.material.kind, m.material.lam, m.material.mu, m.material.bulk, m.material.gamma,
-                          m.material.viscosity, m.material.J0, m.material.friction_deg)
+        q = m.material
+        mat = MPMMaterial(q.kind, q.lam, q.mu, q.bulk, q.gamma, q.viscosity, q.J0, q.friction_deg, q.yield_stress,
+                          q.alpha0, q.beta, q.hardening_factor, q.m, q.hardening_on)
         models.append(MPMModel(m.mass, m.volume, m.x, m.v, mat, m.lam_p, m.mu_p))
     eng.InitializeParticle(models)
     if adjoint_segment:

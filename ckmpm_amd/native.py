@@ -54,7 +54,9 @@ class SimDesc(ctypes.Structure):
 class MaterialDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("lam", ctypes.c_float), ("mu", ctypes.c_float), ("bulk", ctypes.c_float),
                 ("gamma", ctypes.c_float), ("viscosity", ctypes.c_float), ("J0", ctypes.c_float),
-                ("friction_angle_deg", ctypes.c_float)]
+                ("friction_angle_deg", ctypes.c_float), ("yield_stress", ctypes.c_float), ("alpha0", ctypes.c_float),
+                ("beta", ctypes.c_float), ("hardening_factor", ctypes.c_float), ("m", ctypes.c_float),
+                ("hardening_on", ctypes.c_int32)]
 
 
 class HashInfo(ctypes.Structure):

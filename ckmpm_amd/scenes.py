@@ -28,7 +28,7 @@ from typing import List, Optional
 
 import numpy as np
 
-FIXED_COROTATED, NEO_HOOKEAN, DRUCKER_PRAGER, FLUID = 0, 1, 2, 3
+FIXED_COROTATED, NEO_HOOKEAN, DRUCKER_PRAGER, FLUID, VON_MISES, CAM_CLAY = 0, 1, 2, 3, 4, 5
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -75,9 +75,16 @@ class Material:
     viscosity: float = 0.0
     J0: float = 1.0
     friction_deg: float = 30.0
+    yield_stress: float = 0.0       # von Mises
+    alpha0: float = 0.0             # Cam-clay: initial logJp, beta, hardening factor, M, hardening
+    beta: float = 0.0
+    hardening_factor: float = 0.0
+    m: float = 0.0
+    hardening_on: bool = True
 
     def params(self) -> List[float]:
-        return [self.lam, self.mu, self.bulk, self.gamma, self.viscosity, self.J0, self.friction_deg]
+        return [self.lam, self.mu, self.bulk, self.gamma, self.viscosity, self.J0, self.friction_deg,
+                self.yield_stress, self.alpha0, self.beta, self.hardening_factor, self.m, float(self.hardening_on)]
 
 
 @dataclasses.dataclass

@@ -43,12 +43,14 @@ enum {
 #define CKMPM_FLAG_MISSING_BLOCK 0x10u
 
 /* constitutive models: mpm_material.cuh:10-18 (FixedCorotated, DruckerPragerStvkhencky,
- * Fluid) plus Neo-Hookean, which the reference lacks */
+ * Fluid, VonMises, NonAssociatedCamClay) plus Neo-Hookean, which the reference lacks */
 enum {
     CKMPM_FIXED_COROTATED = 0,
     CKMPM_NEO_HOOKEAN = 1,
     CKMPM_DRUCKER_PRAGER = 2,
-    CKMPM_FLUID = 3
+    CKMPM_FLUID = 3,
+    CKMPM_VON_MISES = 4,
+    CKMPM_CAM_CLAY = 5 /* NonAssociatedCamClay */
 };
 
 /* MPMDomain<MPMDomainRange<Dx,Dy,Dz>, MPMDomainOffset<Ox,Oy,Oz>> (mpm_domain.h:9-61):
@@ -76,6 +78,11 @@ typedef struct {
     float bulk, gamma;        /* fluid: pressure = bulk (J^-gamma - 1) */
     float viscosity, J0;      /* fluid */
     float friction_angle_deg; /* DP; the reference hard-codes 30 (mpm_material.cuh:369) */
+    float yield_stress;       /* von Mises (VonMisesMaterialParameter, mpm_material.cuh:236-253) */
+    /* Cam-clay (NonAssociatedCamClayMaterialParameter, mpm_material.cuh:189-214): initial
+     * logJp, beta (p_min = -beta p0), hardening factor xi, slope M, hardening on/off */
+    float alpha0, beta, hardening_factor, m;
+    int32_t hardening_on;
 } ckmpm_material_desc;
 
 typedef struct {

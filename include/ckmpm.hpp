@@ -93,6 +93,24 @@ struct Material {
         m.d.friction_angle_deg = friction_angle_deg;
         return m;
     }
+    /* MPMMaterial<kVonMises>{mu, lambda, yieldStress}: StVK-Hencky with von Mises return */
+    static Material VonMises(float lambda, float mu, float yield_stress) {
+        Material m = Solid(CKMPM_VON_MISES, lambda, mu);
+        m.d.yield_stress = yield_stress;
+        return m;
+    }
+    /* MPMMaterial<kNonAssociatedCamClay>{mu, lambda, alpha0, beta, hardeningFactor, M,
+     * hardeningOn} */
+    static Material CamClay(float lambda, float mu, float alpha0, float beta, float hardening_factor, float m_slope,
+                            bool hardening_on = true) {
+        Material m = Solid(CKMPM_CAM_CLAY, lambda, mu);
+        m.d.alpha0 = alpha0;
+        m.d.beta = beta;
+        m.d.hardening_factor = hardening_factor;
+        m.d.m = m_slope;
+        m.d.hardening_on = hardening_on ? 1 : 0;
+        return m;
+    }
     /* pressure = bulk (J^-gamma - 1), viscous stress viscosity (C + C^T) */
     static Material Fluid(float bulk, float gamma, float viscosity, float J0 = 1.0f) {
         Material m;

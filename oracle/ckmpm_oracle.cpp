@@ -47,7 +47,7 @@ constexpr int kMoveShift = 27;
 constexpr uint32_t kSlotMask = (1u << kMoveShift) - 1u;
 constexpr uint32_t kNoMove = 13u << kMoveShift;
 
-enum Kind { kFixedCorotated = 0, kNeoHookean = 1, kDruckerPrager = 2, kFluid = 3 };
+enum Kind { kFixedCorotated = 0, kNeoHookean = 1, kDruckerPrager = 2, kFluid = 3, kVonMises = 4, kCamClay = 5 };
 
 enum Flags : uint32_t {
     kFlagOutOfWindow = 1u << 0,   // particle left its 8^3-node P2G window
@@ -245,7 +245,26 @@ void SignedSvd(const double* F, double* U, double* s, double* V) {
 struct Material {
     int kind = kFixedCorotated;
     double lambda = 0, mu = 0, bulk = 0, gamma = 0, viscosity = 0, J0 = 1, sin_phi = 0.5;
+    double yield = 0;                                  // von Mises
+    double alpha0 = 0, beta = 0, xi = 0, M = 0;        // Cam-clay
+    bool hardening = true;
 };
+inline void MaterialFromParams(int kind, const double* params, Material& m) {
+    m.kind = kind;
+    m.lambda = params[0];
+    m.mu = params[1];
+    m.bulk = params[2];
+    m.gamma = params[3];
+    m.viscosity = params[4];
+    m.J0 = params[5];
+    m.sin_phi = std::sin(params[6] * kPi / 180.0);
+    m.yield = params[7];
+    m.alpha0 = params[8];
+    m.beta = params[9];
+    m.xi = params[10];
+    m.M = params[11];
+    m.hardening = params[12] != 0.0;
+}
 
 // FC: mpm_material.cuh:318-357.  tau = U diag(2 mu s(s-1) + lambda J(J-1)) U^T.
 void StressFixedCorotated(double lambda, double mu, const double* F, double* tau) {
@@ -326,6 +345,132 @@ void StressDruckerPrager(double lambda, double mu, double sin_phi, double* F, do
     MatMul(P, Ft, tau);
 }
 
+// tau = U diag(k) U^T
+inline void PrincipalTau(const double* U, const double* k, double* tau) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0;
+            for (int c = 0; c < 3; ++c) v += U[i * 3 + c] * k[c] * U[j * 3 + c];
+            tau[i * 3 + j] = v;
+        }
+}
+// F = U diag(s) V^T
+inline void Compose(const double* U, const double* s, const double* V, double* F) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0;
+            for (int c = 0; c < 3; ++c) v += U[i * 3 + c] * s[c] * V[j * 3 + c];
+            F[i * 3 + j] = v;
+        }
+}
+
+// von Mises over StVK-Hencky: return map mpm_algorithm.cuh:450-487 (eps = log s,
+// yield when |dev eps| > yield / (2 mu), radial return of dev eps, F = U exp(eps) V^T),
+// stress mpm_material.cuh:479-519 on the projected F: tau = U diag(phat) V^T F^T with
+// phat = (2 mu log s + lambda tr log s) / s, from a second SVD as the reference takes it.
+// log |s| clamped at 1e-4 as in the Drucker-Prager model (the reference's logf of a
+// non-positive singular value is NaN).
+void StressVonMises(double lambda, double mu, double yield, double* F, double* tau) {
+    double U[9], s[3], V[9];
+    SignedSvd(F, U, s, V);
+    double eps[3];
+    for (int a = 0; a < 3; ++a) eps[a] = std::log(std::max(std::fabs(s[a]), 1e-4));
+    const double tr = eps[0] + eps[1] + eps[2];
+    double eh[3];
+    for (int a = 0; a < 3; ++a) eh[a] = eps[a] - tr / 3.0;
+    const double ehn = std::sqrt(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+    const double over = ehn - yield / (2.0 * mu);
+    if (over > 0.0) {
+        double ns[3];
+        for (int a = 0; a < 3; ++a) ns[a] = std::exp(eps[a] - over * eh[a] / ehn);
+        Compose(U, ns, V, F);
+    }
+    SignedSvd(F, U, s, V);
+    double lg[3];
+    for (int a = 0; a < 3; ++a) lg[a] = std::log(std::max(std::fabs(s[a]), 1e-4));
+    const double ltr = lg[0] + lg[1] + lg[2];
+    double ph[3];
+    for (int a = 0; a < 3; ++a) ph[a] = (2.0 * mu * lg[a] + lambda * ltr) / s[a];
+    double P[9], Ft[9];
+    Compose(U, ph, V, P);
+    Transpose(F, Ft);
+    MatMul(P, Ft, tau);
+}
+
+// Non-associated Cam-clay: mpm_material.cuh:521-671 (UpdateForce mpm_algorithm.cuh:329-400).
+// kappa = 2 mu / 3 + lambda, p0 = kappa (1e-5 + sinh(xi max(-logJp, 0))), p_min = -beta p0.
+// Trial: J = s0 s1 s2, s_hat = mu J^(-2/3) (s^2 - tr(s^2)/3), p = -kappa/2 (J - 1/J) J,
+// y = 3/2 (1 + 2 beta) |s_hat|^2 + M^2 (p - p_min)(p - p0).
+//  * p > p0 or p < p_min: s = J_new^(1/3) with J_new = sqrt(1 - 2 p_cap / kappa);
+//    logJp += log(J / J_new) when hardening (:555-583);
+//  * else if y >= 1e-4: s_i = sqrt(s_hat_i c + tr(s^2)/3), c = J^(2/3) / mu
+//    sqrt(-M^2 (p - p_min)(p - p0) / (3/2 (1 + 2 beta))) / |s_hat| (:585-610); with
+//    hardening and p inside (p_min + 1e-4, p0 - 1e-4), logJp += log(J / J_fake) nine times
+//    (the update sits in the reference's 9-iteration loop, :612-640), J_fake from the
+//    point where the ray from (p_c, 0), p_c = (1 - beta) p0 / 2, through (p, -q),
+//    q = sqrt(3/2) |s_hat|, meets the yield ellipse;
+//  * tau = mu J_trial^(-2/3) dev(F F^T) + kappa/2 (J^2 - 1) I (:645-669).
+// A projected F is U diag(s) V^T (the reference writes U diag(s) V; the two differ by a
+// right rotation, which leaves F F^T, the singular values and so every stress unchanged).
+void StressCamClay(const Material& m, double lambda, double mu, double* F, double& logJp, double* tau) {
+    const double kappa = 2.0 * mu / 3.0 + lambda;
+    const double beta = m.beta, msq = m.M * m.M;
+    double U[9], s[3], V[9];
+    SignedSvd(F, U, s, V);
+    const double p0 = kappa * (1e-5 + std::sinh(m.xi * std::max(-logJp, 0.0)));
+    const double pmin = -beta * p0;
+    const double Je = s[0] * s[1] * s[2];
+    const double bh[3] = {s[0] * s[0], s[1] * s[1], s[2] * s[2]};
+    const double tr3 = (bh[0] + bh[1] + bh[2]) / 3.0;
+    const double devf = mu * std::pow(Je, -2.0 / 3.0);
+    double sh[3];
+    for (int a = 0; a < 3; ++a) sh[a] = devf * (bh[a] - tr3);
+    const double ptr = -(kappa * 0.5 * (Je - 1.0 / Je)) * Je;
+    const double ysh = 1.5 * (1.0 + 2.0 * beta);
+    const double yph = msq * (ptr - pmin) * (ptr - p0);
+    const double sn2 = sh[0] * sh[0] + sh[1] * sh[1] + sh[2] * sh[2];
+    const double y = ysh * sn2 + yph;
+    double ns[3] = {s[0], s[1], s[2]};
+    if (ptr > p0 || ptr < pmin) {
+        const double p = ptr > p0 ? p0 : pmin;
+        const double jn = std::sqrt(-2.0 * p / kappa + 1.0);
+        ns[0] = ns[1] = ns[2] = std::cbrt(jn);
+        Compose(U, ns, V, F);
+        if (m.hardening) logJp += std::log(Je / jn);
+    } else if (y >= 1e-4) {
+        const double c = std::pow(Je, 2.0 / 3.0) / mu * std::sqrt(-yph / ysh) / std::sqrt(sn2);
+        for (int a = 0; a < 3; ++a) ns[a] = std::sqrt(sh[a] * c + tr3);
+        Compose(U, ns, V, F);
+        for (int rep = 0; rep < 9; ++rep)
+            if (m.hardening && p0 > 1e-4 && ptr < p0 - 1e-4 && ptr > 1e-4 + pmin) {
+                const double pc = (1.0 - beta) * p0 / 2.0;
+                const double qt = std::sqrt(1.5 * sn2);
+                double d0 = pc - ptr, d1 = -qt;
+                const double dn = std::sqrt(d0 * d0 + d1 * d1);
+                d0 /= dn;
+                d1 /= dn;
+                const double A = msq * d0 * d0 + (1.0 + 2.0 * beta) * d1 * d1;
+                const double B = msq * d0 * (2.0 * pc - p0 - pmin);
+                const double C = msq * (pc - pmin) * (pc - p0);
+                const double disc = B * B - 4.0 * A * C;
+                const double p1 = pc + (-B + std::sqrt(disc)) / (2.0 * A) * d0;
+                const double p2 = pc + (-B - std::sqrt(disc)) / (2.0 * A) * d0;
+                const double pf = (ptr - pc) * (p1 - pc) > 0 ? p1 : p2;
+                const double jsq = -2.0 * pf / kappa + 1.0;
+                const double jf = std::sqrt(std::fabs(jsq));
+                if (jf > 1e-4) logJp += std::log(Je / jf);
+            }
+    }
+    const double J = ns[0] * ns[1] * ns[2];
+    double Ft[9], b[9];
+    Transpose(F, Ft);
+    MatMul(F, Ft, b);
+    const double btr = (b[0] + b[4] + b[8]) / 3.0;
+    const double hyd = kappa * 0.5 * (J * J - 1.0);
+    for (int i = 0; i < 9; ++i) tau[i] = devf * (b[i] - ((i % 4 == 0) ? btr : 0.0));
+    for (int i = 0; i < 3; ++i) tau[i * 4] += hyd;
+}
+
 // Weakly compressible fluid: mpm_algorithm.cuh:527-569 and mpm_material.cuh:466-477.
 // J <- J (1 + dt tr C); p = K (J^-gamma - 1); tau = J (nu (C + C^T) - p I).
 void StressFluid(const Material& m, const double* C, double J, double* tau) {
@@ -364,7 +509,9 @@ inline int AttrCount(int kind) {
     switch (kind) {
         case kFixedCorotated:
         case kNeoHookean: return 12;
-        case kDruckerPrager: return 13;
+        case kDruckerPrager:
+        case kCamClay: return 13;
+        case kVonMises: return 12;
         default: return 4;
     }
 }
@@ -533,7 +680,7 @@ inline void LoadState(const Model& m, uint32_t s, double* x, double* F, double& 
         J = m.st[3 * n + s];
     } else {
         for (int k = 0; k < 9; ++k) F[k] = m.st[(3 + k) * n + s];
-        if (m.mat.kind == kDruckerPrager) logJp = m.st[12 * n + s];
+        if (m.mat.kind == kDruckerPrager || m.mat.kind == kCamClay) logJp = m.st[12 * n + s];
     }
 }
 inline void StoreState(Model& m, uint32_t t, const double* x, const double* F, double J, double logJp) {
@@ -543,7 +690,7 @@ inline void StoreState(Model& m, uint32_t t, const double* x, const double* F, d
         m.st_next[3 * n + t] = J;
     } else {
         for (int k = 0; k < 9; ++k) m.st_next[(3 + k) * n + t] = F[k];
-        if (m.mat.kind == kDruckerPrager) m.st_next[12 * n + t] = logJp;
+        if (m.mat.kind == kDruckerPrager || m.mat.kind == kCamClay) m.st_next[12 * n + t] = logJp;
     }
 }
 
@@ -658,8 +805,12 @@ void Physics(Engine& e, double dt, double next_dt) {
                         StressFixedCorotated(lam, mu, F, tau);
                     else if (m.mat.kind == kNeoHookean)
                         StressNeoHookean(lam, mu, F, tau);
-                    else
+                    else if (m.mat.kind == kDruckerPrager)
                         StressDruckerPrager(lam, mu, m.mat.sin_phi, F, logJp, tau);
+                    else if (m.mat.kind == kVonMises)
+                        StressVonMises(lam, mu, m.mat.yield, F, tau);
+                    else
+                        StressCamClay(m.mat, lam, mu, F, logJp, tau);
                 }
                 StoreState(m, t, x, F, J, logJp);
 
@@ -827,14 +978,7 @@ int orc_add_model(orc_engine* h, int kind, const double* params, double mass, do
                   uint32_t n, const float* vel, uint32_t nvel, const float* lam_p, const float* mu_p) {
     Engine* e = reinterpret_cast<Engine*>(h);
     Model m;
-    m.mat.kind = kind;
-    m.mat.lambda = params[0];
-    m.mat.mu = params[1];
-    m.mat.bulk = params[2];
-    m.mat.gamma = params[3];
-    m.mat.viscosity = params[4];
-    m.mat.J0 = params[5];
-    m.mat.sin_phi = std::sin(params[6] * kPi / 180.0);
+    MaterialFromParams(kind, params, m.mat);
     m.mass = mass;
     m.volume = volume;
     m.n = n;
@@ -846,8 +990,8 @@ int orc_add_model(orc_engine* h, int kind, const double* params, double mass, do
         m.pp_attr = m.nattr;
         m.nattr += 2;
     }
-    // initial state in input order: x, F = I (FC/NH/DP), logJp = 0 (DP), J = J0 (fluid)
-    // (CopyParticleArrayToParticleBuffer, mpm_algorithm.cuh:973-1081)
+    // initial state in input order: x, F = I (solids), logJp = 0 (DP) or alpha0 (Cam-clay),
+    // J = J0 (fluid) (CopyParticleArrayToParticleBuffer, mpm_algorithm.cuh:973-1081)
     m.st.assign(static_cast<size_t>(m.nattr) * n, 0.0);
     for (uint32_t p = 0; p < n; ++p)
         for (int a = 0; a < 3; ++a) m.st[a * static_cast<size_t>(n) + p] = xyz[3 * static_cast<size_t>(p) + a];
@@ -861,6 +1005,8 @@ int orc_add_model(orc_engine* h, int kind, const double* params, double mass, do
     } else {
         for (int k = 0; k < 9; k += 4)
             for (uint32_t p = 0; p < n; ++p) m.st[(3 + k) * static_cast<size_t>(n) + p] = 1.0;
+        if (kind == kCamClay)
+            for (uint32_t p = 0; p < n; ++p) m.st[12 * static_cast<size_t>(n) + p] = m.mat.alpha0;
     }
     e->models.push_back(std::move(m));
     return static_cast<int>(e->models.size()) - 1;
@@ -1084,20 +1230,15 @@ void orc_svd(const double* F, double* U, double* s, double* V) { SignedSvd(F, U,
 void orc_stress(int kind, const double* params, const double* F_in, double logJp_in, const double* C, double J,
                 double* F_out, double* logJp_out, double* tau) {
     Material m;
-    m.kind = kind;
-    m.lambda = params[0];
-    m.mu = params[1];
-    m.bulk = params[2];
-    m.gamma = params[3];
-    m.viscosity = params[4];
-    m.J0 = params[5];
-    m.sin_phi = std::sin(params[6] * kPi / 180.0);
+    MaterialFromParams(kind, params, m);
     double F[9];
     std::memcpy(F, F_in, sizeof(F));
     double lj = logJp_in;
     if (kind == kFixedCorotated) StressFixedCorotated(m.lambda, m.mu, F, tau);
     else if (kind == kNeoHookean) StressNeoHookean(m.lambda, m.mu, F, tau);
     else if (kind == kDruckerPrager) StressDruckerPrager(m.lambda, m.mu, m.sin_phi, F, lj, tau);
+    else if (kind == kVonMises) StressVonMises(m.lambda, m.mu, m.yield, F, tau);
+    else if (kind == kCamClay) StressCamClay(m, m.lambda, m.mu, F, lj, tau);
     else StressFluid(m, C, J, tau);
     std::memcpy(F_out, F, sizeof(F));
     *logJp_out = lj;

@@ -128,8 +128,17 @@ def svd(F: np.ndarray):
     return U.reshape(3, 3), s, V.reshape(3, 3)
 
 
+def _params13(params) -> np.ndarray:
+    """lam, mu, bulk, gamma, viscosity, J0, friction (deg), yield, alpha0, beta,
+    hardening factor, M, hardening on; missing trailing entries: 0 (hardening on: 1)."""
+    pr = np.array([0, 0, 0, 0, 0, 1, 30, 0, 0, 0, 0, 0, 1], np.float64)
+    p = np.asarray(params, np.float64)
+    pr[: p.shape[0]] = p
+    return pr
+
+
 def stress(kind: int, params, F, logJp=0.0, C=None, J=1.0):
-    pr = np.ascontiguousarray(params, np.float64)
+    pr = _params13(params)
     F = np.ascontiguousarray(F, np.float64).reshape(9)
     C = np.zeros(9) if C is None else np.ascontiguousarray(C, np.float64).reshape(9)
     Fo, lj, tau = np.zeros(9), np.zeros(1), np.zeros(9)
@@ -167,7 +176,7 @@ class Oracle:
     def add_model(self, kind, params, mass, volume, x, v, lam_p=None, mu_p=None) -> int:
         x = np.ascontiguousarray(x, np.float32)
         v = np.ascontiguousarray(v, np.float32)
-        pr = np.ascontiguousarray(params, np.float64)
+        pr = _params13(params)
         lp = None if lam_p is None else np.ascontiguousarray(lam_p, np.float32)
         mp = None if mu_p is None else np.ascontiguousarray(mu_p, np.float32)
         idx = lib().orc_add_model(self.h, int(kind), _p(pr, _f64p), float(mass), float(volume), _p(x, _f32p),

@@ -6,7 +6,8 @@ ragged file; the driver prints its usage and rejects bad command lines before it
 touches a device.
 GPU: the driver, given scenes as command lines and positions as .bin assets, ends with
 the same particle positions as the Python engine on the same scenes (fixed-corotated,
-Neo-Hookean, Drucker-Prager and fluid models; substep and frame loops).
+Neo-Hookean, Drucker-Prager, fluid, von Mises and Cam-clay models; substep and frame
+loops).
 """
 import os
 import subprocess
@@ -66,7 +67,7 @@ def test_driver_usage_and_argument_errors(tmp_path):
         (["--domain", "8,8"], "needs 3 comma-separated numbers"),
         (["--domain", "8,8,8", "--dx", "0.03"], "no model"),
         (["--domain", "8,8,8", "--dx", "x"], "needs a number"),
-        (["--domain", "8,8,8", "--dx", "0.03", "--material", "wood", "--positions", "p.bin"], "must be fc, nh"),
+        (["--domain", "8,8,8", "--dx", "0.03", "--material", "wood", "--positions", "p.bin"], "must be fc, nh, dp"),
         (["--domain", "8,8,8", "--dx", "0.03", "--material", "fc", "--mass", "1", "--volume", "1",
           "--positions", "p.bin"], "a solid needs"),
         (["--bogus", "1"], "unknown option"),
@@ -85,7 +86,7 @@ def test_driver_usage_and_argument_errors(tmp_path):
 
 
 # ------------------------------------------------------------------------------- GPU
-_KIND = {0: "fc", 1: "nh", 2: "dp", 3: "fluid"}
+_KIND = {0: "fc", 1: "nh", 2: "dp", 3: "fluid", 4: "vm", 5: "nacc"}
 
 
 def _command(sc, tmp_path):
@@ -99,7 +100,9 @@ def _command(sc, tmp_path):
             args += ["--bulk", repr(mat.bulk), "--gamma", repr(mat.gamma), "--viscosity", repr(mat.viscosity),
                      "--J0", repr(mat.J0)]
         else:
-            args += ["--lambda", repr(mat.lam), "--mu", repr(mat.mu), "--friction", repr(mat.friction_deg)]
+            args += ["--lambda", repr(mat.lam), "--mu", repr(mat.mu), "--friction", repr(mat.friction_deg),
+                     "--yield", repr(mat.yield_stress), "--alpha0", repr(mat.alpha0), "--beta", repr(mat.beta),
+                     "--xi", repr(mat.hardening_factor), "--M", repr(mat.m), "--hardening", str(int(mat.hardening_on))]
         v = np.asarray(m.v, np.float32).reshape(-1, 3)
         if v.shape[0] == 1:
             args += ["--velocity", ",".join(repr(float(c)) for c in v[0])]
@@ -116,12 +119,15 @@ def _scenes():
 
     nh = scenes.colliding_spheres(seed=3)
     nh.models[1].material.kind = scenes.NEO_HOOKEAN
+    from tests.test_gpu_parity import _box_scene  # boxes with random velocities: plastic flow
+
     return [("spheres_fc", scenes.colliding_spheres()), ("spheres_fc_nh", nh),
+            ("box_vm", _box_scene(scenes.VON_MISES, seed=30)), ("box_nacc", _box_scene(scenes.CAM_CLAY, seed=31)),
             ("fluid_sand", scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8))]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["spheres_fc", "spheres_fc_nh", "fluid_sand"])
+@pytest.mark.parametrize("case", ["spheres_fc", "spheres_fc_nh", "box_vm", "box_nacc", "fluid_sand"])
 def test_driver_matches_python_engine(case, tmp_path):
     from ckmpm_amd.engine import engine_from_scene
 

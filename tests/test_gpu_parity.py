@@ -148,16 +148,24 @@ def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5,
     x = scenes._stratified_cells(rng, g.astype(np.float64), dx, ppc_side).astype(np.float32)
     v = (speed * rng.normal(3 * x.shape[0]).reshape(-1, 3)).astype(np.float32)
     vol = dx ** 3 / ppc_side ** 3
+    lam, mu = scenes.lame(2e5, 0.3)
     if kind == scenes.FLUID:
         mat = scenes.Material(kind, bulk=1e4, gamma=7.0, viscosity=0.1, J0=j0)
+    elif kind == scenes.VON_MISES:
+        # yields within the run: |dev eps| reaches ~1e-3 against yield / (2 mu) = 2e-4
+        mat = scenes.Material(kind, lam, mu, yield_stress=30.0)
+    elif kind == scenes.CAM_CLAY:
+        # p0 = kappa (1e-5 + sinh(0.8 * 0.001)) ~ 135 against |p| up to ~ kappa 1e-3 ~ 170:
+        # elastic, cap and shear-return particles in the same run
+        mat = scenes.Material(kind, lam, mu, alpha0=-0.001, beta=0.5, hardening_factor=0.8, m=2.36)
     else:
-        lam, mu = scenes.lame(2e5, 0.3)
         mat = scenes.Material(kind, lam, mu, friction_deg=30.0)
     m = scenes.ModelSpec(mat, 1000 * vol, vol, x, v)
     return scenes.Scene(f"box{kind}", (dim, dim, dim), (0, 0, 0), dx, gravity, dt, 10, [m])
 
 
-@pytest.mark.parametrize("kind", [scenes.FIXED_COROTATED, scenes.NEO_HOOKEAN, scenes.DRUCKER_PRAGER, scenes.FLUID])
+@pytest.mark.parametrize("kind", [scenes.FIXED_COROTATED, scenes.NEO_HOOKEAN, scenes.DRUCKER_PRAGER, scenes.FLUID,
+                                  scenes.VON_MISES, scenes.CAM_CLAY])
 def test_material_box_lockstep(kind):
     sc = _box_scene(kind, seed=10 + kind)
     _lockstep(sc, 6, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
