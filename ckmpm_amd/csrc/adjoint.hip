@@ -960,6 +960,7 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
     if (e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "enable the adjoint before ckmpm_initial_setup");
     if (e->adj) return Fail(e, CKMPM_ERR_INVALID, "adjoint already enabled");
     if (e->models.empty()) return Fail(e, CKMPM_ERR_INVALID, "add the models first");
+    if (e->bc_custom) return Fail(e, CKMPM_ERR_INVALID, "the adjoint differentiates the default boundary only");
     for (const ModelBuf& m : e->models) {
         if (m.kind != kFixedCorotated && m.kind != kNeoHookean)
             return Fail(e, CKMPM_ERR_INVALID, "the adjoint supports fixed-corotated and Neo-Hookean models");

@@ -49,6 +49,10 @@ const char kUsage[] =
     "  --autotune K        first pick each model's G2P2G workgroup size (advances the scene)\n"
     "  --out PREFIX        write PREFIX<i>.bin: final positions of model i, float32 xyz\n"
     "  --conservation DIR  write DIR/conservationMetric_model_<i>.bin: 13 floats per substep\n"
+    "  --walls sticky|slip|none   grid walls within 8 cells of the faces (sticky)\n"
+    "  --region fixed,X0,Y0,Z0,X1,Y1,Z1 | velocity,<box>,VX,VY,VZ | rotation,<box>,WX,WY,WZ,CX,CY,CZ\n"
+    "                      grid region on cells [X0,X1) x [Y0,Y1) x [Z0,Z1), applied in order\n"
+    "                      before the walls; any --walls / --region replaces the default rule\n"
     "  --export DIR        with --frames: after frame f write DIR/model_<i>_particle_frame_<f>.txt\n"
     "                      (positions, float32 xyz, as the reference's binary export)\n"
     "a model (options before the --positions that closes it):\n"
@@ -140,6 +144,8 @@ int Run(int argc, char** argv) {
     std::string out, cons_dir, export_dir;
     std::vector<ckmpm::Model> models;
     PendingModel pend;
+    ckmpm::Boundary boundary;
+    bool custom_boundary = false;
     for (int i = 1; i < argc; ++i) {
         const char* opt = argv[i];
         if (!std::strcmp(opt, "--help") || !std::strcmp(opt, "-h")) {
@@ -166,6 +172,31 @@ int Run(int argc, char** argv) {
         else if (o == "--out") out = val;
         else if (o == "--conservation") cons_dir = val;
         else if (o == "--export") export_dir = val;
+        else if (o == "--walls") {
+            const std::string w(val);
+            if (w == "sticky") boundary.d.walls = CKMPM_WALLS_STICKY;
+            else if (w == "slip") boundary.d.walls = CKMPM_WALLS_SLIP;
+            else if (w == "none") boundary.d.walls = CKMPM_WALLS_NONE;
+            else throw UsageError{"--walls must be sticky, slip or none"};
+            custom_boundary = true;
+        } else if (o == "--region") {
+            const std::string spec(val);
+            const size_t comma = spec.find(',');
+            const std::string kind = spec.substr(0, comma);
+            const std::string rest = comma == std::string::npos ? "" : spec.substr(comma + 1);
+            const size_t want = kind == "fixed" ? 6 : kind == "velocity" ? 9 : kind == "rotation" ? 12 : 0;
+            if (!want) throw UsageError{"--region kind must be fixed, velocity or rotation"};
+            const auto v = Numbers(opt, rest.c_str(), want);
+            const std::array<int32_t, 3> lo{static_cast<int32_t>(v[0]), static_cast<int32_t>(v[1]), static_cast<int32_t>(v[2])};
+            const std::array<int32_t, 3> hi{static_cast<int32_t>(v[3]), static_cast<int32_t>(v[4]), static_cast<int32_t>(v[5])};
+            auto f3 = [&](size_t i) {
+                return std::array<float, 3>{static_cast<float>(v[i]), static_cast<float>(v[i + 1]), static_cast<float>(v[i + 2])};
+            };
+            if (kind == "fixed") boundary.Fixed(lo, hi);
+            else if (kind == "velocity") boundary.Velocity(lo, hi, f3(6));
+            else boundary.Rotation(lo, hi, f3(6), f3(9));
+            custom_boundary = true;
+        }
         else if (o == "--material") pend.material = val;
         else if (o == "--lambda") pend.lambda = Number(opt, val);
         else if (o == "--mu") pend.mu = Number(opt, val);
@@ -200,6 +231,7 @@ int Run(int argc, char** argv) {
 
     ckmpm::Engine engine(domain, config);
     engine.InitializeParticle(models);
+    if (custom_boundary) engine.SetBoundary(boundary);
     engine.InitialSetup();
     if (autotune > 0) engine.Autotune(static_cast<uint32_t>(autotune));
     engine.Synchronize();

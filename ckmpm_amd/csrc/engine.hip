@@ -83,6 +83,7 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
     g.st = e->status;
     g.nmodels = static_cast<int>(e->models.size());
     for (int i = 0; i < g.nmodels; ++i) g.blkmax[i] = e->models[i].blkmax;
+    g.bc = e->bc;
     // 2 blocks x 4 chains per workgroup and pass: 0.19 A workgroups cover Nb (about 1.2 A)
     LaunchGridUpdate(g, GridFor(e->a_hint * 3u / 16u + 8u), s);
     Mark(e);
@@ -235,6 +236,7 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
     e->nwords = static_cast<uint32_t>((nblocks + 31) / 32);
     check(Alloc(e, &e->V, static_cast<size_t>(e->cap) * kVFloats));
     check(Alloc(e, &e->status, 1));
+    check(Alloc(e, &e->bc, 1));
     e->emit_tiles = (e->nwords + kEmitTileWords - 1) / kEmitTileWords;
     check(Alloc(e, &e->bitmap, static_cast<size_t>(e->emit_tiles) * kEmitTileWords));
     check(Alloc(e, &e->tile_cnt, 2 * static_cast<size_t>(e->emit_tiles)));
@@ -253,6 +255,7 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
                              e->stream);
         (void)hipMemsetAsync(e->tile_cnt, 0, 2 * static_cast<size_t>(e->emit_tiles) * sizeof(uint32_t), e->stream);
         (void)hipMemsetAsync(e->status, 0, sizeof(Status), e->stream);
+        (void)hipMemsetAsync(e->bc, 0, sizeof(BoundaryDev), e->stream);
         if (hipStreamSynchronize(e->stream) != hipSuccess) rc = Fail(e, CKMPM_ERR_HIP, "init");
     }
     if (rc != CKMPM_OK) {
@@ -429,6 +432,27 @@ int ckmpm_substep(ckmpm_engine* e, double dt, double next_dt) {
     e->time += next_dt;
     const hipError_t r = hipGetLastError();
     if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
+    return CKMPM_OK;
+}
+
+int ckmpm_set_boundary(ckmpm_engine* e, const ckmpm_boundary_desc* b) {
+    if (!e) return CKMPM_ERR_INVALID;
+    BoundaryDev h{};
+    if (b && e->adj)
+        return Fail(e, CKMPM_ERR_INVALID, "the adjoint differentiates the default boundary only");
+    if (b) {
+        if (b->walls < CKMPM_WALLS_STICKY || b->walls > CKMPM_WALLS_NONE)
+            return Fail(e, CKMPM_ERR_INVALID, "walls must be CKMPM_WALLS_STICKY, _SLIP or _NONE");
+        if (b->nregions > CKMPM_MAX_REGIONS) return Fail(e, CKMPM_ERR_INVALID, "more than CKMPM_MAX_REGIONS regions");
+        for (uint32_t r = 0; r < b->nregions; ++r)
+            if (b->regions[r].kind < CKMPM_REGION_FIXED || b->regions[r].kind > CKMPM_REGION_ROTATION)
+                return Fail(e, CKMPM_ERR_INVALID, "region " + std::to_string(r) + ": unknown kind");
+        h.custom = 1;
+        h.d = *b;
+    }
+    CK_HIP(e, hipMemcpyAsync(e->bc, &h, sizeof(h), hipMemcpyHostToDevice, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    e->bc_custom = b != nullptr;
     return CKMPM_OK;
 }
 

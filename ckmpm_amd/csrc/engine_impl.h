@@ -67,6 +67,8 @@ struct ckmpm_engine {
     int mprev = 0;        // M[mprev] holds the previous layout's mass / momentum
     float* V = nullptr;
     Status* status = nullptr;
+    BoundaryDev* bc = nullptr;   // grid boundary description (custom = 0: the default rule)
+    bool bc_custom = false;      // host copy of bc->custom
     uint32_t* bitmap = nullptr;
     uint32_t nwords = 0;
     uint32_t* word_off = nullptr;

@@ -560,10 +560,58 @@ __device__ __forceinline__ void MaxBits(uint32_t* p, float v) {
 // momentum load chains of those blocks are issued together, so a wave has kGridIlp
 // independent latency chains in flight instead of one.
 constexpr int kGridIlp = 4;
+// A scene's boundary description (the reference's ProcessGridCellVelocity hook,
+// mpm_algorithm.cuh:795-799, as data): regions in order on the node's integer cell, then
+// the walls within 8 cells of the domain faces.
+__device__ __noinline__ void ApplyBoundary(const ckmpm_boundary_desc& bc, const DomainDev& dom, const int (&cell)[3],
+                                           float (&v)[3]) {
+    for (uint32_t r = 0; r < bc.nregions && r < CKMPM_MAX_REGIONS; ++r) {
+        const ckmpm_region& g = bc.regions[r];
+        bool in = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) in &= cell[k] >= g.lo[k] && cell[k] < g.hi[k];
+        if (!in) continue;
+        if (g.kind == CKMPM_REGION_FIXED) {
+            v[0] = v[1] = v[2] = 0.0f;
+        } else if (g.kind == CKMPM_REGION_VELOCITY) {
+            v[0] = g.value[0];
+            v[1] = g.value[1];
+            v[2] = g.value[2];
+        } else {
+            float q[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) q[k] = static_cast<float>(cell[k]) * dom.dx - g.center[k];
+            v[0] = g.value[1] * q[2] - g.value[2] * q[1];
+            v[1] = g.value[2] * q[0] - g.value[0] * q[2];
+            v[2] = g.value[0] * q[1] - g.value[1] * q[0];
+        }
+    }
+    if (bc.walls == CKMPM_WALLS_NONE) return;
+    float nrm[3];
+    bool on = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        nrm[k] = cell[k] < 8 ? 1.0f : (cell[k] >= 4 * dom.dim[k] - 8 ? -1.0f : 0.0f);
+        on |= nrm[k] != 0.0f;
+    }
+    if (!on) return;
+    if (bc.walls == CKMPM_WALLS_STICKY) {
+        v[0] = v[1] = v[2] = 0.0f;
+        return;
+    }
+    const float inv = rsqrtf(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nrm[k] *= inv;
+    const float vn = nrm[0] * v[0] + nrm[1] * v[1] + nrm[2] * v[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] -= vn * nrm[k];
+}
+
 __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
     const uint32_t n = threadIdx.x & 127u;
     const uint32_t stride = gridDim.x * 2;
+    const bool custom = a.bc->custom != 0;
     float vmax = 0.0f;
     for (uint32_t i0 = blockIdx.x * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
         uint32_t key[kGridIlp], io[kGridIlp];
@@ -606,7 +654,14 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) v[k] = p[u][k] * im;
                 v[1] += a.gravity * a.dt;
-                if (oob) v[0] = v[1] = v[2] = 0.0f;
+                if (custom) {
+                    // node (x, y, z) of either sub-grid: cell 4 b + (x, y, z), as the hook sees it
+                    const int cell[3] = {4 * b[0] + static_cast<int>((n >> 4) & 3u), 4 * b[1] + static_cast<int>((n >> 2) & 3u),
+                                         4 * b[2] + static_cast<int>(n & 3u)};
+                    ApplyBoundary(a.bc->d, a.dom, cell, v);
+                } else if (oob) {
+                    v[0] = v[1] = v[2] = 0.0f;
+                }
                 v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
             }
             float* dst = a.V + static_cast<size_t>(i) * kVFloats;

@@ -152,6 +152,25 @@ class MPMEngine:
         self._check(self._lib.ckmpm_get_phase_times(self._h, ms, ctypes.byref(n)), "get_phase_times")
         return tuple(ms), int(n.value)
 
+    def set_boundary(self, boundary):
+        """Grid boundary (scenes.Boundary; None: the default sticky walls)."""
+        if boundary is None:
+            self._check(self._lib.ckmpm_set_boundary(self._h, None), "ckmpm_set_boundary")
+            return
+        if len(boundary.regions) > native.MAX_REGIONS:
+            raise CkmpmError(native.ERR_INVALID, "ckmpm_set_boundary", "too many regions")
+        d = native.BoundaryDesc()
+        d.walls = boundary.walls
+        d.nregions = len(boundary.regions)
+        for i, r in enumerate(boundary.regions):
+            g = d.regions[i]
+            g.lo[:] = [int(c) for c in r.lo]
+            g.hi[:] = [int(c) for c in r.hi]
+            g.kind = r.kind
+            g.value[:] = [float(c) for c in r.value]
+            g.center[:] = [float(c) for c in r.center]
+        self._check(self._lib.ckmpm_set_boundary(self._h, ctypes.byref(d)), "ckmpm_set_boundary")
+
     def set_conservation(self, capacity: int):
         """Record the per-substep conservation metric (up to `capacity` substeps; 0: off)."""
         self._check(self._lib.ckmpm_set_conservation(self._h, int(capacity)), "ckmpm_set_conservation")
@@ -373,6 +392,8 @@ def engine_from_scene(scene: Scene, capacity: Optional[int] = None, adjoint_segm
                           q.alpha0, q.beta, q.hardening_factor, q.m, q.hardening_on)
         models.append(MPMModel(m.mass, m.volume, m.x, m.v, mat, m.lam_p, m.mu_p))
     eng.InitializeParticle(models)
+    if scene.boundary is not None:
+        eng.set_boundary(scene.boundary)
     if adjoint_segment:
         eng.adjoint_enable(adjoint_segment)
     eng.InitialSetup()

@@ -28,7 +28,7 @@ EXPORTS = [
     "ckmpm_synchronize", "ckmpm_get_counters", "ckmpm_clear_flags", "ckmpm_get_positions", "ckmpm_model_info",
     "ckmpm_export_layout", "ckmpm_export_ordering", "ckmpm_export_state", "ckmpm_import_state",
     "ckmpm_export_grid_m", "ckmpm_export_grid_v", "ckmpm_set_profiling", "ckmpm_get_phase_times",
-    "ckmpm_set_conservation", "ckmpm_get_conservation",
+    "ckmpm_set_conservation", "ckmpm_get_conservation", "ckmpm_set_boundary",
     "ckmpm_destroy", "ckmpm_last_error", "ckmpm_autotune", "ckmpm_set_tuning", "ckmpm_get_tuning",
     "ckmpm_hash_create", "ckmpm_hash_build", "ckmpm_hash_build_device", "ckmpm_hash_find",
     "ckmpm_hash_find_device", "ckmpm_hash_synchronize", "ckmpm_hash_info", "ckmpm_hash_export",
@@ -57,6 +57,18 @@ class MaterialDesc(ctypes.Structure):
                 ("friction_angle_deg", ctypes.c_float), ("yield_stress", ctypes.c_float), ("alpha0", ctypes.c_float),
                 ("beta", ctypes.c_float), ("hardening_factor", ctypes.c_float), ("m", ctypes.c_float),
                 ("hardening_on", ctypes.c_int32)]
+
+
+class Region(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_int32 * 3), ("hi", ctypes.c_int32 * 3), ("kind", ctypes.c_int32),
+                ("value", ctypes.c_float * 3), ("center", ctypes.c_float * 3)]
+
+
+MAX_REGIONS = 16
+
+
+class BoundaryDesc(ctypes.Structure):
+    _fields_ = [("walls", ctypes.c_int32), ("nregions", ctypes.c_uint32), ("regions", Region * MAX_REGIONS)]
 
 
 class HashInfo(ctypes.Structure):
@@ -111,6 +123,7 @@ def load():
     L.ckmpm_export_grid_v.argtypes = [_vp, _f32p, ctypes.c_uint64]
     L.ckmpm_set_profiling.argtypes = [_vp, ctypes.c_int32]
     L.ckmpm_set_conservation.argtypes = [_vp, ctypes.c_uint32]
+    L.ckmpm_set_boundary.argtypes = [_vp, ctypes.POINTER(BoundaryDesc)]
     L.ckmpm_get_conservation.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_float), ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_uint64)]
     L.ckmpm_get_phase_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]

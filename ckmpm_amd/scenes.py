@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List, Optional
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -87,6 +87,29 @@ class Material:
                 self.yield_stress, self.alpha0, self.beta, self.hardening_factor, self.m, float(self.hardening_on)]
 
 
+# grid boundary (ckmpm_set_boundary): walls and region kinds
+WALLS_STICKY, WALLS_SLIP, WALLS_NONE = 0, 1, 2
+REGION_FIXED, REGION_VELOCITY, REGION_ROTATION = 0, 1, 2
+
+
+@dataclasses.dataclass
+class Region:
+    """Cells lo <= c < hi; v = 0 (fixed), value (velocity) or value x (c dx - center)."""
+    lo: Sequence[int]
+    hi: Sequence[int]
+    kind: int
+    value: Sequence[float] = (0.0, 0.0, 0.0)
+    center: Sequence[float] = (0.0, 0.0, 0.0)
+
+
+@dataclasses.dataclass
+class Boundary:
+    """The reference's ProcessGridCellVelocity scene hook as data: regions in order, then
+    the walls (within 8 cells of the domain faces)."""
+    walls: int = WALLS_STICKY
+    regions: List[Region] = dataclasses.field(default_factory=list)
+
+
 @dataclasses.dataclass
 class ModelSpec:
     material: Material
@@ -110,6 +133,7 @@ class Scene:
     models: List[ModelSpec]
     mass_clamp: float = 0.0
     capacity: int = 0        # block-list capacity (0: derived from the particle count)
+    boundary: Optional[Boundary] = None  # None: the default sticky walls
 
     @property
     def n_particles(self) -> int:

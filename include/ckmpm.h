@@ -125,6 +125,45 @@ int ckmpm_substep(ckmpm_engine* eng, double dt, double next_dt);
  * non-finite velocity.  Runs `frames` frames. */
 int ckmpm_run_frames(ckmpm_engine* eng, uint32_t frames);
 
+/* Grid boundary conditions.  The reference's grid update zeroes the velocity of every grid
+ * node within 2 blocks (8 cells) of a domain face (mpm_algorithm.cuh:759-762,800-805)
+ * unless the scene's config supplies ProcessGridCellVelocity(cell, velocity, frame),
+ * which then replaces that rule (:795-799).  The scenes' hooks are regions and walls:
+ * a fixed box and slip walls (tests/fire_hydrant/test_fire_hydrant.cuh:148-185), slip
+ * walls (tests/fluid_with_two_longs/test_fluid_with_two_longs.cuh:133-161), rotating
+ * clamps and sticky walls (tests/twisting_bar/test_twisting_bar.cuh:126-168).  This
+ * description expresses them as data.  For every grid node with mass above the clamp,
+ * after v = p / m + g dt: the regions are applied in order to the node's cell (integer
+ * cell coordinates, both sub-grids alike, as the hook receives them), then the walls. */
+enum {
+    CKMPM_WALLS_STICKY = 0, /* v = 0 within 8 cells of a face (the reference's default) */
+    CKMPM_WALLS_SLIP = 1,   /* there, remove the component along n / |n|, n = sum of the
+                               inward normals of the faces within 8 cells */
+    CKMPM_WALLS_NONE = 2
+};
+enum {
+    CKMPM_REGION_FIXED = 0,    /* v = 0 */
+    CKMPM_REGION_VELOCITY = 1, /* v = value */
+    CKMPM_REGION_ROTATION = 2  /* v = value x (cell dx - center): rotation with angular velocity value */
+};
+#define CKMPM_MAX_REGIONS 16
+typedef struct {
+    int32_t lo[3], hi[3]; /* cells lo <= c < hi */
+    int32_t kind;         /* CKMPM_REGION_* */
+    float value[3];
+    float center[3];
+} ckmpm_region;
+typedef struct {
+    int32_t walls; /* CKMPM_WALLS_* */
+    uint32_t nregions;
+    ckmpm_region regions[CKMPM_MAX_REGIONS];
+} ckmpm_boundary_desc;
+/* Replaces the boundary description (NULL: the reference's default, sticky walls and no
+ * regions).  Takes effect from the next substep, also in captured batch graphs; not to
+ * be called while the engine's stream is being captured.  An engine with the adjoint
+ * enabled keeps the default (its grid backward differentiates that rule only). */
+int ckmpm_set_boundary(ckmpm_engine* eng, const ckmpm_boundary_desc* boundary);
+
 /* Waits for the stream; returns the first error encoded in the device flags. */
 int ckmpm_synchronize(ckmpm_engine* eng);
 int ckmpm_get_counters(ckmpm_engine* eng, ckmpm_counters* out);

@@ -174,6 +174,39 @@ inline void WriteRawPositions(const std::string& path, const std::vector<float>&
     if (std::fclose(f) != 0 || !ok) throw Error(CKMPM_ERR_INVALID, "cannot write " + path);
 }
 
+/* The scene's ProcessGridCellVelocity hook as data (ckmpm_boundary_desc): regions in
+ * order, then walls within 8 cells of the domain faces. */
+struct Boundary {
+    ckmpm_boundary_desc d{};
+    explicit Boundary(int32_t walls = CKMPM_WALLS_STICKY) { d.walls = walls; }
+    Boundary& Fixed(std::array<int32_t, 3> lo, std::array<int32_t, 3> hi) {
+        return Add(lo, hi, CKMPM_REGION_FIXED, {0, 0, 0}, {0, 0, 0});
+    }
+    Boundary& Velocity(std::array<int32_t, 3> lo, std::array<int32_t, 3> hi, std::array<float, 3> v) {
+        return Add(lo, hi, CKMPM_REGION_VELOCITY, v, {0, 0, 0});
+    }
+    /* v = omega x (cell dx - center) */
+    Boundary& Rotation(std::array<int32_t, 3> lo, std::array<int32_t, 3> hi, std::array<float, 3> omega,
+                       std::array<float, 3> center) {
+        return Add(lo, hi, CKMPM_REGION_ROTATION, omega, center);
+    }
+
+   private:
+    Boundary& Add(std::array<int32_t, 3> lo, std::array<int32_t, 3> hi, int32_t kind, std::array<float, 3> value,
+                  std::array<float, 3> center) {
+        if (d.nregions >= CKMPM_MAX_REGIONS) throw Error(CKMPM_ERR_INVALID, "more than CKMPM_MAX_REGIONS regions");
+        ckmpm_region& r = d.regions[d.nregions++];
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = lo[k];
+            r.hi[k] = hi[k];
+            r.value[k] = value[k];
+            r.center[k] = center[k];
+        }
+        r.kind = kind;
+        return *this;
+    }
+};
+
 /* MPMEngine: Initialize at construction, then InitializeParticle, InitialSetup and
  * Substep / Simulate. */
 class Engine {
@@ -234,6 +267,9 @@ class Engine {
         Call(ckmpm_get_counters(e_, &c), "ckmpm_get_counters");
         return c;
     }
+    /* grid boundary (replaces the default sticky walls, as a scene hook does) */
+    void SetBoundary(const Boundary& b) { Call(ckmpm_set_boundary(e_, &b.d), "ckmpm_set_boundary"); }
+    void ResetBoundary() { Call(ckmpm_set_boundary(e_, nullptr), "ckmpm_set_boundary"); }
     /* per-model G2P2G workgroup size chosen by timing real substeps (the scene advances) */
     void Autotune(uint32_t substeps = 3) { Call(ckmpm_autotune(e_, substeps), "ckmpm_autotune"); }
     /* RetrieveParticleBuffer + ExportParticleToModel: global positions, storage order */

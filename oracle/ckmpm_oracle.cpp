@@ -530,6 +530,15 @@ struct Engine {
     uint32_t flags = 0;
     double max_v2 = 0;
     bool physics_pending = false;
+    // boundary description (the scene hook ProcessGridCellVelocity as data); custom =
+    // false keeps the default sticky 2-block margin
+    struct Region {
+        int lo[3], hi[3], kind;
+        double value[3], center[3];
+    };
+    bool bc_custom = false;
+    int walls = 0;  // 0 sticky, 1 slip, 2 none
+    std::vector<Region> regions;
 };
 
 // ------------------------------------------------------------------ layouts
@@ -639,6 +648,50 @@ void Rasterize(Engine& e) {
 // edge, NaN -> inf, max |v|^2.  Restated as a read of the previous layout's M
 // through its key table (this also performs CopySelectedGridBlocks,
 // mpm_algorithm.cuh:1920-1945, which only moves blocks between layouts).
+// The scene hook of the reference's grid update (mpm_algorithm.cuh:795-799) as data: the
+// regions in order on the node's integer cell (fixed: v = 0, velocity: v = value,
+// rotation: v = value x (cell dx - center)), then the walls within 8 cells of a face
+// (sticky: v = 0; slip: v -= (v . n) n with n the normalized sum of the inward face
+// normals), as the fire-hydrant, two-longs and twisting-bar hooks do
+// (tests/fire_hydrant/test_fire_hydrant.cuh:148-185,
+// tests/fluid_with_two_longs/test_fluid_with_two_longs.cuh:133-161,
+// tests/twisting_bar/test_twisting_bar.cuh:126-168).
+void ApplyBoundary(const Engine& e, const int* cell, double* v) {
+    for (const Engine::Region& g : e.regions) {
+        bool in = true;
+        for (int k = 0; k < 3; ++k) in = in && cell[k] >= g.lo[k] && cell[k] < g.hi[k];
+        if (!in) continue;
+        if (g.kind == 0) {
+            v[0] = v[1] = v[2] = 0.0;
+        } else if (g.kind == 1) {
+            for (int k = 0; k < 3; ++k) v[k] = g.value[k];
+        } else {
+            double q[3];
+            for (int k = 0; k < 3; ++k) q[k] = cell[k] * e.dx - g.center[k];
+            const double w[3] = {g.value[0], g.value[1], g.value[2]};
+            v[0] = w[1] * q[2] - w[2] * q[1];
+            v[1] = w[2] * q[0] - w[0] * q[2];
+            v[2] = w[0] * q[1] - w[1] * q[0];
+        }
+    }
+    if (e.walls == 2) return;
+    double n[3];
+    bool on = false;
+    for (int k = 0; k < 3; ++k) {
+        n[k] = cell[k] < 8 ? 1.0 : (cell[k] >= 4 * e.dom.dim[k] - 8 ? -1.0 : 0.0);
+        on = on || n[k] != 0.0;
+    }
+    if (!on) return;
+    if (e.walls == 0) {
+        v[0] = v[1] = v[2] = 0.0;
+        return;
+    }
+    const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    double vn = 0.0;
+    for (int k = 0; k < 3; ++k) vn += v[k] * n[k] / nn;
+    for (int k = 0; k < 3; ++k) v[k] -= vn * n[k] / nn;
+}
+
 void GridUpdate(Engine& e, double dt) {
     const Layout& L = e.cur;
     e.V.assign(static_cast<size_t>(L.Nb) * 384, 0.0);
@@ -659,7 +712,12 @@ void GridUpdate(Engine& e, double dt) {
             if (mass > e.mass_clamp) {
                 for (int a = 0; a < 3; ++a) v[a] = src[128 * (a + 1) + n] / mass;
                 v[1] += e.gravity * dt;
-                if (oob) v[0] = v[1] = v[2] = 0.0;
+                if (e.bc_custom) {
+                    const int cell[3] = {4 * b[0] + ((n >> 4) & 3), 4 * b[1] + ((n >> 2) & 3), 4 * b[2] + (n & 3)};
+                    ApplyBoundary(e, cell, v);
+                } else if (oob) {
+                    v[0] = v[1] = v[2] = 0.0;
+                }
                 v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
             }
             for (int a = 0; a < 3; ++a) dst[128 * a + n] = v[a];
@@ -1122,6 +1180,28 @@ uint32_t orc_rebin(orc_engine* h, const float* const* xyz_per_model) {
 uint32_t orc_substep(orc_engine* h, double dt, double next_dt) {
     orc_physics(h, dt, next_dt);
     return orc_rebin(h, nullptr);
+}
+
+// Boundary description: custom = 0 restores the default; region r has cells box[6 r ..
+// 6 r + 5] = lo xyz, hi xyz, kind kinds[r], value vals[6 r .. 6 r + 2], center
+// vals[6 r + 3 .. 6 r + 5].
+void orc_set_boundary(orc_engine* h, int custom, int walls, uint32_t nregions, const int* box, const int* kinds,
+                      const double* vals) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    e->bc_custom = custom != 0;
+    e->walls = walls;
+    e->regions.clear();
+    for (uint32_t r = 0; r < nregions; ++r) {
+        Engine::Region g;
+        for (int k = 0; k < 3; ++k) {
+            g.lo[k] = box[6 * r + k];
+            g.hi[k] = box[6 * r + 3 + k];
+            g.value[k] = vals[6 * r + k];
+            g.center[k] = vals[6 * r + 3 + k];
+        }
+        g.kind = kinds[r];
+        e->regions.push_back(g);
+    }
 }
 
 // Conservation metric of model mi on the current grid velocities and particle state:

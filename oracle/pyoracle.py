@@ -96,6 +96,8 @@ def lib():
         L.orc_set_grid_m_f32.argtypes = [ctypes.c_void_p, _f32p]
         L.orc_conservation.argtypes = [ctypes.c_void_p, ctypes.c_int, _f64p]
         L.orc_set_grid_v.argtypes = [ctypes.c_void_p, _f64p]
+        L.orc_set_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _i32p, _i32p,
+                                       _f64p]
         L.orc_svd.argtypes = [_f64p, _f64p, _f64p, _f64p]
         L.orc_stress.argtypes = [ctypes.c_int, _f64p, _f64p, ctypes.c_double, _f64p, ctypes.c_double, _f64p, _f64p, _f64p]
         L.orc_weight0.restype = ctypes.c_double
@@ -171,7 +173,22 @@ class Oracle:
         o = cls(scene.dim, scene.dx, scene.gravity, scene.mass_clamp, scene.block_capacity(), scene.offset)
         for m in scene.models:
             o.add_model(m.material.kind, m.material.params(), m.mass, m.volume, m.x, m.v, m.lam_p, m.mu_p)
+        if getattr(scene, "boundary", None) is not None:
+            o.set_boundary(scene.boundary)
         return o
+
+    def set_boundary(self, b):
+        """Boundary with .walls and .regions (each .lo, .hi, .kind, .value, .center)."""
+        n = len(b.regions)
+        box = np.zeros(max(6 * n, 1), np.int32)
+        kinds = np.zeros(max(n, 1), np.int32)
+        vals = np.zeros(max(6 * n, 1), np.float64)
+        for i, r in enumerate(b.regions):
+            box[6 * i:6 * i + 6] = list(r.lo) + list(r.hi)
+            kinds[i] = r.kind
+            # the engine holds value and center in FP32: use the same numbers
+            vals[6 * i:6 * i + 6] = np.float32(list(r.value) + list(r.center))
+        lib().orc_set_boundary(self.h, 1, int(b.walls), n, _p(box, _i32p), _p(kinds, _i32p), _p(vals, _f64p))
 
     def add_model(self, kind, params, mass, volume, x, v, lam_p=None, mu_p=None) -> int:
         x = np.ascontiguousarray(x, np.float32)

@@ -71,6 +71,9 @@ def test_driver_usage_and_argument_errors(tmp_path):
         (["--domain", "8,8,8", "--dx", "0.03", "--material", "fc", "--mass", "1", "--volume", "1",
           "--positions", "p.bin"], "a solid needs"),
         (["--bogus", "1"], "unknown option"),
+        (["--walls", "soft"], "--walls must be"),
+        (["--region", "fixed,1,2,3"], "needs 6 comma-separated numbers"),
+        (["--region", "spin,1,2,3,4,5,6"], "--region kind must be"),
         (["--domain", "8,8,8", "--dx", "0.03", "--material", "fluid", "--bulk", "1", "--mass", "1", "--volume", "1",
           "--positions", "ok.bin", "--substeps", "1", "--export", "."], "use it with --frames"),
     ]
@@ -111,6 +114,13 @@ def _command(sc, tmp_path):
             args += ["--velocities", str(tmp_path / f"v{i}.bin")]
         np.asarray(m.x, np.float32).tofile(tmp_path / f"x{i}.bin")
         args += ["--positions", str(tmp_path / f"x{i}.bin")]
+    b = sc.boundary
+    if b is not None:
+        args += ["--walls", ["sticky", "slip", "none"][b.walls]]
+        for r in b.regions:
+            kind = ["fixed", "velocity", "rotation"][r.kind]
+            vals = list(r.lo) + list(r.hi) + ([] if r.kind == 0 else list(r.value)) + (list(r.center) if r.kind == 2 else [])
+            args += ["--region", kind + "," + ",".join(repr(float(x)) if i >= 6 else str(int(x)) for i, x in enumerate(vals))]
     return args + ["--out", str(tmp_path / "out")]
 
 
@@ -119,15 +129,20 @@ def _scenes():
 
     nh = scenes.colliding_spheres(seed=3)
     nh.models[1].material.kind = scenes.NEO_HOOKEAN
+    from tests.test_boundary import _corner_scene, _regions
     from tests.test_gpu_parity import _box_scene  # boxes with random velocities: plastic flow
+
+    corner = _corner_scene(None)
+    corner.boundary = scenes.Boundary(scenes.WALLS_SLIP, _regions(corner.dx))
 
     return [("spheres_fc", scenes.colliding_spheres()), ("spheres_fc_nh", nh),
             ("box_vm", _box_scene(scenes.VON_MISES, seed=30)), ("box_nacc", _box_scene(scenes.CAM_CLAY, seed=31)),
+            ("corner_boundary", corner),
             ("fluid_sand", scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8))]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["spheres_fc", "spheres_fc_nh", "box_vm", "box_nacc", "fluid_sand"])
+@pytest.mark.parametrize("case", ["spheres_fc", "spheres_fc_nh", "box_vm", "box_nacc", "corner_boundary", "fluid_sand"])
 def test_driver_matches_python_engine(case, tmp_path):
     from ckmpm_amd.engine import engine_from_scene
 
