@@ -84,6 +84,7 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
     g.nmodels = static_cast<int>(e->models.size());
     for (int i = 0; i < g.nmodels; ++i) g.blkmax[i] = e->models[i].blkmax;
     g.bc = e->bc;
+    g.custom = e->bc_custom ? 1 : 0;
     // 2 blocks x 4 chains per workgroup and pass: 0.19 A workgroups cover Nb (about 1.2 A)
     LaunchGridUpdate(g, GridFor(e->a_hint * 3u / 16u + 8u), s);
     Mark(e);
@@ -753,7 +754,9 @@ int BatchFail(ckmpm_batch* b, int code, const std::string& msg) {
         if (_r != hipSuccess) return BatchFail((b), CKMPM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(_r)); \
     } while (0)
 
-int EngineRoles(const ckmpm_engine* e) { return e->cur | (e->sc << 1) | (e->mprev << 2); }
+// buffer roles, and the grid-update instantiation (default or custom boundary), that a
+// captured graph bakes in
+int EngineRoles(const ckmpm_engine* e) { return e->cur | (e->sc << 1) | (e->mprev << 2) | (e->bc_custom << 3); }
 
 void EagerSubstep(ckmpm_engine* e) {
     const double dt = e->substeps == 0 ? 0.0 : e->dt_default;

@@ -69,8 +69,8 @@ void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
 
 // ---------------------------------------------------------------- substep
-// Grid boundary description in device memory (ckmpm_set_boundary); custom = 0 selects
-// the reference's default rule (sticky 2-block margin) without reading the rest.
+// Grid boundary description in device memory (ckmpm_set_boundary); custom = 0 stands for
+// the reference's default rule (sticky 2-block margin).
 struct BoundaryDev {
     int custom;
     ckmpm_boundary_desc d;
@@ -91,7 +91,8 @@ struct GridUpdateArgs {
     // per-block P2G deposit bounds of the previous layout's G2P2G, reduced into st->dep_bits
     int nmodels;
     const float* blkmax[kMaxModels];
-    const BoundaryDev* bc;       // boundary description (never null)
+    const BoundaryDev* bc;       // boundary description (read when custom)
+    int custom;                  // host copy of bc->custom: selects the kernel instantiation
 };
 void LaunchBeginSubstep(Status* st, hipStream_t s);
 void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s);

@@ -563,7 +563,7 @@ constexpr int kGridIlp = 4;
 // A scene's boundary description (the reference's ProcessGridCellVelocity hook,
 // mpm_algorithm.cuh:795-799, as data): regions in order on the node's integer cell, then
 // the walls within 8 cells of the domain faces.
-__device__ __noinline__ void ApplyBoundary(const ckmpm_boundary_desc& bc, const DomainDev& dom, const int (&cell)[3],
+__device__ __forceinline__ void ApplyBoundary(const ckmpm_boundary_desc& bc, const DomainDev& dom, const int (&cell)[3],
                                            float (&v)[3]) {
     for (uint32_t r = 0; r < bc.nregions && r < CKMPM_MAX_REGIONS; ++r) {
         const ckmpm_region& g = bc.regions[r];
@@ -607,11 +607,13 @@ __device__ __noinline__ void ApplyBoundary(const ckmpm_boundary_desc& bc, const 
     for (int k = 0; k < 3; ++k) v[k] -= vn * nrm[k];
 }
 
+// kCustom: the scene's boundary description replaces the default sticky margin (a
+// separate instantiation, so that the default path keeps its registers and code size)
+template <bool kCustom>
 __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
     const uint32_t n = threadIdx.x & 127u;
     const uint32_t stride = gridDim.x * 2;
-    const bool custom = a.bc->custom != 0;
     float vmax = 0.0f;
     for (uint32_t i0 = blockIdx.x * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
         uint32_t key[kGridIlp], io[kGridIlp];
@@ -654,7 +656,7 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) v[k] = p[u][k] * im;
                 v[1] += a.gravity * a.dt;
-                if (custom) {
+                if (kCustom) {
                     // node (x, y, z) of either sub-grid: cell 4 b + (x, y, z), as the hook sees it
                     const int cell[3] = {4 * b[0] + static_cast<int>((n >> 4) & 3u), 4 * b[1] + static_cast<int>((n >> 2) & 3u),
                                          4 * b[2] + static_cast<int>(n & 3u)};
@@ -1563,7 +1565,8 @@ void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word
 }
 void LaunchBeginSubstep(Status* st, hipStream_t s) { BeginSubstepKernel<<<1, 64, 0, s>>>(st); }
 void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s) {
-    GridUpdateKernel<<<grid, kThreads, 0, s>>>(a);
+    if (a.custom) GridUpdateKernel<true><<<grid, kThreads, 0, s>>>(a);
+    else GridUpdateKernel<false><<<grid, kThreads, 0, s>>>(a);
 }
 template <int T>
 void LaunchG2P2GT(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
