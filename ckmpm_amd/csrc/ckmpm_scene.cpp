@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -239,17 +240,20 @@ int Run(int argc, char** argv) {
     for (const ckmpm::Model& m : models) particles += m.count();
     // conservation records: appended to the files after the substeps, or after every frame
     // (the frame loop's substep count depends on the CFL dt)
-    std::vector<std::FILE*> cons;
+    struct Closer {
+        void operator()(std::FILE* f) const { std::fclose(f); }
+    };
+    std::vector<std::unique_ptr<std::FILE, Closer>> cons;
     for (size_t i = 0; !cons_dir.empty() && i < models.size(); ++i) {
         const std::string path = cons_dir + "/conservationMetric_model_" + std::to_string(i) + ".bin";
-        cons.push_back(std::fopen(path.c_str(), "wb"));
+        cons.emplace_back(std::fopen(path.c_str(), "wb"));
         if (!cons.back()) throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot create " + path);
     }
     const uint32_t cons_cap = substeps >= 0 ? static_cast<uint32_t>(std::max(substeps, 1L)) : 1u << 20;
     auto flush_cons = [&]() {
         for (size_t i = 0; i < cons.size(); ++i) {
             const std::vector<float> r = engine.Conservation(static_cast<int32_t>(i));
-            if (!r.empty() && std::fwrite(r.data(), 4, r.size(), cons[i]) != r.size())
+            if (!r.empty() && std::fwrite(r.data(), 4, r.size(), cons[i].get()) != r.size())
                 throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot write the conservation records");
         }
         engine.SetConservation(cons_cap);  // clears the log
@@ -272,8 +276,8 @@ int Run(int argc, char** argv) {
     engine.Synchronize();
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (substeps >= 0 && !cons.empty()) flush_cons();
-    for (std::FILE* f : cons)
-        if (std::fclose(f) != 0) throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot write the conservation records");
+    for (auto& f : cons)
+        if (std::fclose(f.release()) != 0) throw ckmpm::Error(CKMPM_ERR_INVALID, "cannot write the conservation records");
     const ckmpm_counters c = engine.Counters();
     const uint64_t ran = c.substeps - before;
     if (!out.empty())
