@@ -1381,6 +1381,8 @@ __global__ __launch_bounds__(kThreads) void RebinBaseKernel(DomainDev d, const u
     for (uint32_t e0 = (blockIdx.x * waves + (threadIdx.x >> 6)) * 2; e0 < Ex; e0 += step) {
         const uint32_t e = e0 + half;
         const bool valid = e < Ex && occ[e] != 0u;
+        // the extension blocks at the end of the list are mostly unoccupied: whole waves skip
+        if (!__any(valid)) continue;
         uint32_t js = kEmpty;
         if (valid && dd < static_cast<uint32_t>(kDirections)) {
             int bx, by, bz, ox, oy, oz;
@@ -1409,8 +1411,8 @@ __global__ __launch_bounds__(kThreads) void RebinBaseKernel(DomainDev d, const u
 
 // Particle permutation of the next layout for every model: a wavefront per (model,
 // block) pair, kScatterUnroll particles per lane in flight (a block holds a few hundred
-// particles per model, so most pairs take one pass of dependent loads).
-constexpr int kScatterUnroll = 4;
+// particles per model, so most pairs take one pass of dependent loads: 512 per pass).
+constexpr int kScatterUnroll = 8;
 __global__ __launch_bounds__(kThreads) void RebinScatterKernel(const LayoutCounts* lc, RebinModels rm) {
     const uint32_t A = lc->A;
     const uint32_t pairs = A * static_cast<uint32_t>(rm.nmodels);
