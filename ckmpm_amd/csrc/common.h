@@ -38,14 +38,17 @@ struct LayoutCounts {
 // Device-resident engine status.
 constexpr int kMaxModels = 8;
 
+// The per-substep maxima are double-buffered by substep parity (slot = substep & 1): the
+// grid update of substep k reduces into slot k & 1 and clears slot (k + 1) & 1 for the
+// next substep, so no separate reset launch is needed.  The G2P2G of substep k reads
+// slot k & 1; the host reads the slot of the last grid update.
 struct Status {
     uint32_t flags;
-    uint32_t p2g_global;  // particles deposited through the FP32 global-atomic path
-    uint32_t vmax2_bits;  // max |v|^2 of the last grid update (non-negative float bits)
-    uint32_t pad;
+    uint32_t p2g_global;     // particles deposited through the FP32 global-atomic path
+    uint32_t vmax2_bits[2];  // max |v|^2 of the grid update (non-negative float bits)
     // per model: largest P2G deposit bound of the previous substep (float bits), reduced
     // by the grid update from the per-block maxima; sets the fixed-point scale of P2G
-    uint32_t dep_bits[kMaxModels];
+    uint32_t dep_bits[2][kMaxModels];
 };
 
 struct DomainDev {

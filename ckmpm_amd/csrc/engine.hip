@@ -66,7 +66,6 @@ void Mark(ckmpm_engine* e) {
 void PhaseGrid(ckmpm_engine* e, double dt) {
     hipStream_t s = e->stream;
     Mark(e);
-    LaunchBeginSubstep(e->status, s);
     GridUpdateArgs g{};
     g.dom = e->dom;
     g.keys = e->L[e->cur].keys;
@@ -85,6 +84,7 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
     for (int i = 0; i < g.nmodels; ++i) g.blkmax[i] = e->models[i].blkmax;
     g.bc = e->bc;
     g.custom = e->bc_custom ? 1 : 0;
+    g.slot = static_cast<int>(e->substeps & 1u);
     // 2 blocks x 4 chains per workgroup and pass: 0.19 A workgroups cover Nb (about 1.2 A)
     LaunchGridUpdate(g, GridFor(e->a_hint * 3u / 16u + 8u), s);
     Mark(e);
@@ -123,6 +123,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         a.next_dt = static_cast<float>(next_dt);
         a.st = e->status;
         a.model = static_cast<int>(mi);
+        a.slot = static_cast<int>(e->substeps & 1u);
         a.blkmax = m.blkmax;
         AdjointIdBuffers(e, mi, &a.ids_in, &a.ids_out);
         if (e->tuning) (void)hipEventRecord(m.tune_ev[0], s);
@@ -471,7 +472,7 @@ int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
             Status st;
             LayoutCounts lc;
             CK_TRY(ReadStatus(e, st, lc));
-            const float vmax = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits));
+            const float vmax = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits[e->substeps & 1u]));
             if (std::isinf(vmax) || (st.flags & kFlagNonFinite))
                 return Fail(e, CKMPM_ERR_NONFINITE, "velocity reached infinity");
             double next_dt = std::min(e->dt_default, frame_end - e->time);
@@ -556,7 +557,8 @@ int ckmpm_get_counters(ckmpm_engine* e, ckmpm_counters* out) {
     out->neighbor_blocks = lc.Nb;
     out->exterior_blocks = lc.Ex;
     out->flags = st.flags;
-    out->max_speed = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits));
+    // the slot of the last grid update (substep count - 1; both slots are zero before any)
+    out->max_speed = std::sqrt(__builtin_bit_cast(float, st.vmax2_bits[(e->substeps + 1u) & 1u]));
     out->p2g_global_deposits = st.p2g_global;
     out->substeps = e->substeps;
     return CKMPM_OK;
@@ -756,7 +758,9 @@ int BatchFail(ckmpm_batch* b, int code, const std::string& msg) {
 
 // buffer roles, and the grid-update instantiation (default or custom boundary), that a
 // captured graph bakes in
-int EngineRoles(const ckmpm_engine* e) { return e->cur | (e->sc << 1) | (e->mprev << 2) | (e->bc_custom << 3); }
+int EngineRoles(const ckmpm_engine* e) {
+    return e->cur | (e->sc << 1) | (e->mprev << 2) | (e->bc_custom << 3) | static_cast<int>((e->substeps & 1u) << 4);
+}
 
 void EagerSubstep(ckmpm_engine* e) {
     const double dt = e->substeps == 0 ? 0.0 : e->dt_default;

@@ -93,8 +93,8 @@ struct GridUpdateArgs {
     const float* blkmax[kMaxModels];
     const BoundaryDev* bc;       // boundary description (read when custom)
     int custom;                  // host copy of bc->custom: selects the kernel instantiation
+    int slot;                    // substep parity: the Status slot reduced into (the other is cleared)
 };
-void LaunchBeginSubstep(Status* st, hipStream_t s);
 void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s);
 
 struct G2P2GArgs {
@@ -116,6 +116,7 @@ struct G2P2GArgs {
     float dt, next_dt;
     Status* st;
     int model;      // index of st->dep_bits
+    int slot;       // substep parity: the Status slot of this substep
     float* blkmax;  // [A] largest deposit bound per particle block (for the next substep)
     // particle identities (differentiation): ids_out[t] = ids_in[slot of t]; null: none
     const uint32_t* ids_in;

@@ -536,11 +536,6 @@ __global__ void SetCountKernel(LayoutCounts* lc, int field, const uint32_t* tota
 }
 
 // ============================================================== grid update
-__global__ void BeginSubstepKernel(Status* st) {
-    if (blockIdx.x != 0) return;
-    if (threadIdx.x == 0) st->vmax2_bits = 0u;
-    if (threadIdx.x < static_cast<uint32_t>(kMaxModels)) st->dep_bits[threadIdx.x] = 0u;
-}
 
 // atomicMax of a non-negative float (as bits) that skips the atomic when the stored
 // value is already at least v: the stored value only grows during a launch, so a
@@ -614,6 +609,11 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
     const uint32_t n = threadIdx.x & 127u;
     const uint32_t stride = gridDim.x * 2;
+    // clear the other parity slot for the next substep (nothing else touches it now)
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) a.st->vmax2_bits[a.slot ^ 1] = 0u;
+        if (threadIdx.x < static_cast<uint32_t>(kMaxModels)) a.st->dep_bits[a.slot ^ 1][threadIdx.x] = 0u;
+    }
     float vmax = 0.0f;
     for (uint32_t i0 = blockIdx.x * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
         uint32_t key[kGridIlp], io[kGridIlp];
@@ -710,7 +710,7 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     if (threadIdx.x <= static_cast<uint32_t>(a.nmodels)) {
         float mx = 0.0f;
         for (int w = 0; w < kThreads / 64; ++w) mx = fmaxf(mx, sMax[threadIdx.x][w]);
-        MaxBits(threadIdx.x == 0 ? &a.st->vmax2_bits : &a.st->dep_bits[threadIdx.x - 1], mx);
+        MaxBits(threadIdx.x == 0 ? &a.st->vmax2_bits[a.slot] : &a.st->dep_bits[a.slot][threadIdx.x - 1], mx);
     }
 }
 
@@ -845,8 +845,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     // 2^(mexp-1) go to LDS
     int mexp = kMinExp;
     {
-        const float hint = __uint_as_float(a.st->dep_bits[a.model]);
-        const float vmax = sqrtf(__uint_as_float(a.st->vmax2_bits));
+        const float hint = __uint_as_float(a.st->dep_bits[a.slot][a.model]);
+        const float vmax = sqrtf(__uint_as_float(a.st->vmax2_bits[a.slot]));
         float ref = fmaxf(32.0f * hint, 8.0f * a.mat.mass * vmax);
         if (!(ref <= 1.0e37f)) ref = 1.0e37f;  // a non-finite grid is already flagged
         if (ref > 0.0f) mexp = max(kMinExp, static_cast<int>((__float_as_uint(ref) >> 23) & 0xFFu) - 127 + 2);
@@ -1565,7 +1565,6 @@ void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word
                                                                     st);
     SetCountKernel<<<1, 64, 0, s>>>(lc, 0, total, cap, st);
 }
-void LaunchBeginSubstep(Status* st, hipStream_t s) { BeginSubstepKernel<<<1, 64, 0, s>>>(st); }
 void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s) {
     if (a.custom) GridUpdateKernel<true><<<grid, kThreads, 0, s>>>(a);
     else GridUpdateKernel<false><<<grid, kThreads, 0, s>>>(a);
