@@ -554,7 +554,7 @@ __device__ __forceinline__ void MaxBits(uint32_t* p, float v) {
 // Each thread serves one node of kGridIlp blocks at a time: the key -> table -> mass /
 // momentum load chains of those blocks are issued together, so a wave has kGridIlp
 // independent latency chains in flight instead of one.
-constexpr int kGridIlp = 4;
+constexpr int kGridIlp = 8;
 // A scene's boundary description (the reference's ProcessGridCellVelocity hook,
 // mpm_algorithm.cuh:795-799, as data): regions in order on the node's integer cell, then
 // the walls within 8 cells of the domain faces.
