@@ -94,6 +94,30 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
 // particle blocks from which the rebin's layout extension runs on a side stream
 constexpr uint32_t kForkBlocks = 8192;
 
+// Whether work of this substep may fork to the side stream: large layouts only (the fork
+// and join cost ~13 us, a loss for small scenes) and never under graph capture (a
+// two-stream fork captured into the batch's graphs crashed in the HIP runtime).  The
+// side stream is created on first use, so engines that never fork (small or batched
+// scenes) hold one stream and keep their hardware-queue assignment.
+bool SideStreamReady(ckmpm_engine* e, hipStream_t s) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(s, &cap);
+    bool fork = cap == hipStreamCaptureStatusNone && e->a_hint >= kForkBlocks && !e->side_failed;
+    if (fork && !e->side) {
+        if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&e->join_ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();  // no side stream: stay on one
+            if (e->side) (void)hipStreamDestroy(e->side);
+            if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
+            e->side = nullptr;
+            e->fork_ev = nullptr;
+            e->side_failed = true;
+        }
+    }
+    return fork && e->side && e->fork_ev && e->join_ev;
+}
+
 // G2P2G(dt, next_dt) for every model, then the rebin into the next layout.
 void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
     hipStream_t s = e->stream;
@@ -102,6 +126,10 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
     float* Mout = e->M[1 - e->mprev];
     const int sc = e->sc, sn = 1 - e->sc;
     const uint32_t grid = GridFor(e->a_hint + e->a_hint / 4 + 8);
+    // The models' launches run one after the other on the engine stream.  Running the
+    // second model's on the side stream, concurrently with the first (the launches are
+    // independent; their grid deposits are commutative atomic adds), measured slower on
+    // the 8M fluid + sand scene: 0.614 against 0.607 ms.
     for (size_t mi = 0; mi < e->models.size(); ++mi) {
         ModelBuf& m = e->models[mi];
         G2P2GArgs a{};
@@ -152,28 +180,9 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
     LaunchRebinScan(C.keys, C.lc, rm, e->tsum, e->newidx, P.keys, P.table, P.lc, e->cap, e->status, s);
     // Two independent chains from here: the particle permutation (RebinBase, Scatter:
     // current layout, orderings) and the extension of the next layout (its neighbour and
-    // exterior blocks, bitmap, tile counts).  For large layouts the extension runs on a
-    // side stream: the overlap saves ~24 us per substep at 17k particle blocks while the
-    // fork and join cost ~13 us, a loss for small scenes.  Never under graph capture (a
-    // two-stream fork captured into the batch's graphs crashed in the HIP runtime); the
-    // side stream is created on first use, so engines that never fork (small or batched
-    // scenes) hold one stream and keep their hardware-queue assignment.
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    (void)hipStreamIsCapturing(s, &cap);
-    bool fork = cap == hipStreamCaptureStatusNone && e->a_hint >= kForkBlocks && !e->side_failed;
-    if (fork && !e->side) {
-        if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&e->join_ev, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();  // no side stream: stay on one
-            if (e->side) (void)hipStreamDestroy(e->side);
-            if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
-            e->side = nullptr;
-            e->fork_ev = nullptr;
-            e->side_failed = true;
-        }
-    }
-    fork = fork && e->side && e->fork_ev && e->join_ev;
+    // exterior blocks, bitmap, tile counts).  For large layouts the extension runs on the
+    // side stream: the overlap saves ~24 us per substep at 17k particle blocks.
+    const bool fork = SideStreamReady(e, s);
     if (fork) {
         (void)hipEventRecord(e->fork_ev, s);
         (void)hipStreamWaitEvent(e->side, e->fork_ev, 0);
