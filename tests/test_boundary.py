@@ -123,3 +123,36 @@ def test_gpu_adjoint_refuses_custom_boundaries():
         eng.set_boundary(scenes.Boundary(scenes.WALLS_SLIP))
     eng.set_boundary(None)  # the default stays allowed
     eng.close()
+
+
+@pytest.mark.gpu
+def test_gpu_invalid_descriptions_are_rejected():
+    import ctypes
+
+    from ckmpm_amd import native
+    from ckmpm_amd.engine import engine_from_scene
+
+    sc = _corner_scene(None)
+    eng = engine_from_scene(sc)
+    lib = native.load()
+    bad = []
+    d = native.BoundaryDesc()
+    d.walls = 7
+    bad.append(d)
+    d = native.BoundaryDesc()
+    d.nregions = native.MAX_REGIONS + 1
+    bad.append(d)
+    d = native.BoundaryDesc()
+    d.nregions = 1
+    d.regions[0].kind = 9
+    bad.append(d)
+    for b in bad:
+        assert lib.ckmpm_set_boundary(eng._h, ctypes.byref(b)) == native.ERR_INVALID
+    # the engine still runs with its default boundary
+    eng.Substep(0.0, sc.dt)
+    eng.Synchronize()
+    eng.close()
+    # an unknown material kind is refused at add_model
+    sc.models[0].material.kind = 6
+    with pytest.raises(native.CkmpmError, match="material kind"):
+        engine_from_scene(sc)
