@@ -52,6 +52,26 @@ def cpu_baseline(workload: str, seconds: float = 12.0):
             "sample": f"{sample}; {n} substeps of the FP64 oracle in {el:.1f} s on one host core"}
 
 
+def copy_bandwidth(torch, mib: int = 1024, reps: int = 5) -> float:
+    """On-box device-to-device copy bandwidth in GB/s (read + write bytes of a plain
+    copy of `mib` MiB, best of `reps`, timed with events on torch's stream, which runs
+    the copy): the attainable HBM rate the roofline fraction can also be read against."""
+    n = mib << 18
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) * 1e-3)
+    del src, dst
+    return 2.0 * n * 4 / best / 1e9
+
+
 def _measured_traffic(path: str, workload: str):
     """HBM bytes per G2P2G launch pair for this workload, as measured with rocprofv3
     FETCH_SIZE + WRITE_SIZE and scaled by a plain copy's reading (None if absent)."""
@@ -383,6 +403,10 @@ def main():
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["bytes_per_launch_pair"]
         out["roofline"]["traffic_source"] = traffic["source"]
+    eng.close()
+    copy_gbs = copy_bandwidth(torch)
+    out["roofline"]["copy_gbs"] = copy_gbs
+    out["roofline"]["frac_of_copy"] = achieved / copy_gbs
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
