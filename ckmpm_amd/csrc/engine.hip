@@ -46,9 +46,11 @@ int ReadStatus(ckmpm_engine* e, Status& st, LayoutCounts& lc) {
 // the table unless insert_a), on stream s.
 void ExtendLayout(ckmpm_engine* e, LayoutBuf& L, bool insert_a, hipStream_t s) {
     if (insert_a) LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
+    // marking grids sized for the expected block count (the capacity before the first read)
+    const uint32_t hint = e->a_hint ? e->a_hint + e->a_hint / 4 + 64u : e->cap;
     for (int pass = 0; pass < 2; ++pass)
         LaunchExtensionPass(e->dom, L.keys, L.table, L.lc, e->cap, e->bitmap, e->emit_tiles, e->tile_cnt, pass,
-                            e->status, s);
+                            e->status, hint, s);
 }
 
 void Mark(ckmpm_engine* e) {
@@ -188,7 +190,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         (void)hipStreamWaitEvent(e->side, e->fork_ev, 0);
         ExtendLayout(e, P, false, e->side);
     }
-    LaunchRebinBase(e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap, s);
+    LaunchRebinBase(e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap, 2u * e->a_hint + 64u, s);
     // a wavefront per (model, block), 4 per workgroup
     LaunchRebinScatter(C.lc, rm, GridFor((e->a_hint + e->a_hint / 4) * static_cast<uint32_t>(rm.nmodels) / 4u + 8u), s);
     if (fork) {

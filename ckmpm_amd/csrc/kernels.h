@@ -62,9 +62,11 @@ void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, ui
 // after counts.A (pass 0, sets Nb) or counts.Nb (pass 1, sets Ex), registering them in
 // the table and clearing the bitmap.  tile_cnt holds 2 x emit_tiles counters, zero on
 // the first call (each pass clears the other's).
+// a_hint: expected particle-block count, sizes the marking grid (the kernel loops over
+// the device count, so any hint is correct; a close one saves launching idle workgroups)
 void LaunchExtensionPass(const DomainDev& d, uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap,
                          uint32_t* bitmap, uint32_t emit_tiles, uint32_t* tile_cnt, int pass, Status* st,
-                         hipStream_t s);
+                         uint32_t a_hint, hipStream_t s);
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
 
@@ -148,8 +150,10 @@ void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* 
 // Next layout's particle blocks (keys, table, A) and every model's block offsets.
 void LaunchRebinScan(const uint32_t* keys, const LayoutCounts* lc, const RebinModels& rm, const uint32_t* tsum, uint32_t* newidx, uint32_t* keys_next, uint32_t* table_next,
                      LayoutCounts* lc_next, uint32_t cap, Status* st, hipStream_t s);
+// ex_hint: expected exterior-block count, sizes the grid (any value is correct)
 void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap, hipStream_t s);
+                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap,
+                     uint32_t ex_hint, hipStream_t s);
 void LaunchRebinScatter(const LayoutCounts* lc, const RebinModels& rm, uint32_t grid, hipStream_t s);
 
 }  // namespace ckmpm

@@ -1551,9 +1551,9 @@ void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, ui
 }
 void LaunchExtensionPass(const DomainDev& d, uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap,
                          uint32_t* bitmap, uint32_t emit_tiles, uint32_t* tile_cnt, int pass, Status* st,
-                         hipStream_t s) {
+                         uint32_t a_hint, hipStream_t s) {
     const uint32_t per = pass == 0 ? 8 : 27;
-    MarkExtensionKernel<<<Blocks(static_cast<uint64_t>(cap) * per, kThreads), kThreads, 0, s>>>(
+    MarkExtensionKernel<<<Blocks(static_cast<uint64_t>(std::min(cap, a_hint)) * per, kThreads), kThreads, 0, s>>>(
         d, keys, lc, cap, table, bitmap, tile_cnt + pass * emit_tiles, pass == 0 ? 0 : -1);
     BitmapScanEmitKernel<<<emit_tiles, kThreads, 0, s>>>(bitmap, tile_cnt + pass * emit_tiles,
                                                          tile_cnt + (1 - pass) * emit_tiles, emit_tiles, keys, table,
@@ -1601,8 +1601,9 @@ void LaunchRebinScan(const uint32_t* keys, const LayoutCounts* lc, const RebinMo
                                                          cap, st);
 }
 void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap, hipStream_t s) {
-    RebinBaseKernel<<<Blocks(cap, 8), kThreads, 0, s>>>(d, keys, table, lc, occ, newidx, rm, cap);
+                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap,
+                     uint32_t ex_hint, hipStream_t s) {
+    RebinBaseKernel<<<Blocks(std::min(cap, ex_hint), 8), kThreads, 0, s>>>(d, keys, table, lc, occ, newidx, rm, cap);
 }
 void LaunchRebinScatter(const LayoutCounts* lc, const RebinModels& rm, uint32_t grid, hipStream_t s) {
     RebinScatterKernel<<<grid, kThreads, 0, s>>>(lc, rm);
