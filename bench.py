@@ -95,11 +95,12 @@ def bench_batch(args, torch):
     print(f"[bench] {len(scs)} scenes, {N} particles, set up in {time.perf_counter() - t:.1f} s", file=sys.stderr,
           flush=True)
     res = {}
-    # batched = one hipGraph per scene replayed on the scene's own stream (the streams
-    # overlap on the hardware queues); one fork/join graph over all scenes and plain
-    # interleaved streams are reported beside it
-    modes = (("batched", MPMBatch.ENGINE_GRAPHS), ("single_graph", MPMBatch.GRAPH), ("streams", MPMBatch.STREAMS),
-             ("sequential", MPMBatch.SEQUENTIAL))
+    # batched = fused launches: each kernel of a substep runs once for all scenes, its
+    # workgroups split by scene, two substeps per hipGraph replay; one hipGraph per scene
+    # on the scene's own stream, one fork/join graph over all scenes and plain interleaved
+    # streams are reported beside it
+    modes = (("batched", MPMBatch.FUSED), ("engine_graphs", MPMBatch.ENGINE_GRAPHS), ("single_graph", MPMBatch.GRAPH),
+             ("streams", MPMBatch.STREAMS), ("sequential", MPMBatch.SEQUENTIAL))
     for name, mode in modes:
         b = MPMBatch(engines)
         b.Run(args.warmup, mode)
@@ -122,7 +123,7 @@ def bench_batch(args, torch):
     ach = substep_bytes / (res["batched"]["ms_per_step"] * 1e-3) / 1e9
     kinds = [sc.models[0].material.kind for sc in scs]
     return {
-        "metric": "particle-substeps/s batched (64 scenes, per-scene hipGraphs on concurrent streams) vs sequential single-scene",
+        "metric": "particle-substeps/s batched (64 scenes, one fused launch per kernel and substep, hipGraph-replayed) vs sequential single-scene",
         "value": res["batched"]["value"],
         "unit": "particle-substeps/s",
         "n_gpus": 1,

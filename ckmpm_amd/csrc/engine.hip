@@ -42,15 +42,57 @@ int ReadStatus(ckmpm_engine* e, Status& st, LayoutCounts& lc) {
     return CKMPM_OK;
 }
 
+// Launches of a substep recorded for a fused batch instead of enqueued: per (stage,
+// kernel, variant), the argument structs and workgroup counts of the scenes in the order
+// they were recorded.  Stages follow the substep's dependency order (grid update, G2P2G,
+// the four rebin kernels, the two extension passes' mark and emit); launches of one stage
+// with different kernels or variants are independent of each other.
+struct FusedPhase {
+    int stage = 0, kernel = 0, variant = 0;
+    std::vector<uint8_t> args;    // FusedArgsSize(kernel) bytes per scene
+    std::vector<uint32_t> grids;  // workgroups per scene
+};
+struct FusedRec {
+    std::vector<FusedPhase> ph;
+    void Add(int stage, int kernel, int variant, const void* a, size_t size, uint32_t grid) {
+        FusedPhase* p = nullptr;
+        for (FusedPhase& q : ph)
+            if (q.stage == stage && q.kernel == kernel && q.variant == variant) p = &q;
+        if (!p) {
+            ph.emplace_back();
+            p = &ph.back();
+            p->stage = stage;
+            p->kernel = kernel;
+            p->variant = variant;
+        }
+        const uint8_t* b = static_cast<const uint8_t*>(a);
+        p->args.insert(p->args.end(), b, b + size);
+        p->grids.push_back(grid);
+    }
+};
+enum Stage : int { kStGrid, kStG2P2G, kStCount, kStScan, kStBase, kStScatter, kStMark0, kStEmit0, kStMark1, kStEmit1 };
+
+// Launch now on s, or record for a fused batch.
+template <class T>
+void Run(FusedRec* rec, int stage, int kernel, int variant, const T& a, uint32_t grid, hipStream_t s) {
+    if (rec) rec->Add(stage, kernel, variant, &a, sizeof(T), grid);
+    else LaunchOne(kernel, variant, &a, grid, s);
+}
+
 // Neighbour and exterior extension of layout L (its A keys already written, and in
 // the table unless insert_a), on stream s.
-void ExtendLayout(ckmpm_engine* e, LayoutBuf& L, bool insert_a, hipStream_t s) {
+void ExtendLayout(ckmpm_engine* e, LayoutBuf& L, bool insert_a, hipStream_t s, FusedRec* rec = nullptr) {
     if (insert_a) LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
     // marking grids sized for the expected block count (the capacity before the first read)
     const uint32_t hint = e->a_hint ? e->a_hint + e->a_hint / 4 + 64u : e->cap;
-    for (int pass = 0; pass < 2; ++pass)
-        LaunchExtensionPass(e->dom, L.keys, L.table, L.lc, e->cap, e->bitmap, e->emit_tiles, e->tile_cnt, pass,
-                            e->status, hint, s);
+    for (int pass = 0; pass < 2; ++pass) {
+        uint32_t* cnt = e->tile_cnt + pass * e->emit_tiles;
+        uint32_t* other = e->tile_cnt + (1 - pass) * e->emit_tiles;
+        const MarkArgs m{e->dom, L.keys, L.lc, e->cap, L.table, e->bitmap, cnt, pass == 0 ? 0 : -1};
+        Run(rec, kStMark0 + 2 * pass, kFkMark, 0, m, MarkGrid(e->cap, hint, pass), s);
+        const EmitArgs em{e->bitmap, cnt, other, e->emit_tiles, L.keys, L.table, L.lc, e->cap, pass, e->status};
+        Run(rec, kStEmit0 + 2 * pass, kFkEmit, 0, em, e->emit_tiles, s);
+    }
 }
 
 void Mark(ckmpm_engine* e) {
@@ -65,7 +107,7 @@ void Mark(ckmpm_engine* e) {
 
 // Grid update with dt: velocities of the current layout from the previous layout's
 // mass / momentum; clears the next P2G target.
-void PhaseGrid(ckmpm_engine* e, double dt) {
+void PhaseGrid(ckmpm_engine* e, double dt, FusedRec* rec = nullptr) {
     hipStream_t s = e->stream;
     Mark(e);
     GridUpdateArgs g{};
@@ -89,7 +131,7 @@ void PhaseGrid(ckmpm_engine* e, double dt) {
     g.slot = static_cast<int>(e->substeps & 1u);
     // 2 blocks x 8 chains per workgroup and pass: 0.094 A workgroups cover Nb (about 1.2 A).
     // 8 chains per thread instead of 4: 0.044 -> 0.038 ms on the 8M scene.
-    LaunchGridUpdate(g, GridFor(e->a_hint * 3u / 32u + 8u), s);
+    Run(rec, kStGrid, kFkGrid, g.custom, g, GridFor(e->a_hint * 3u / 32u + 8u), s);
     Mark(e);
 }
 
@@ -121,7 +163,7 @@ bool SideStreamReady(ckmpm_engine* e, hipStream_t s) {
 }
 
 // G2P2G(dt, next_dt) for every model, then the rebin into the next layout.
-void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
+void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = nullptr) {
     hipStream_t s = e->stream;
     LayoutBuf& C = e->L[e->cur];
     LayoutBuf& P = e->L[1 - e->cur];
@@ -157,9 +199,10 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         a.slot = static_cast<int>(e->substeps & 1u);
         a.blkmax = m.blkmax;
         AdjointIdBuffers(e, mi, &a.ids_in, &a.ids_out);
-        if (e->tuning) (void)hipEventRecord(m.tune_ev[0], s);
-        LaunchG2P2G(a, grid, m.threads, s);
-        if (e->tuning) (void)hipEventRecord(m.tune_ev[1], s);
+        const bool tune = e->tuning && !rec;
+        if (tune) (void)hipEventRecord(m.tune_ev[0], s);
+        Run(rec, kStG2P2G, kFkG2P2G, m.mat.kind * 2 + (m.threads == 512 ? 1 : 0), a, G2P2GGrid(grid), s);
+        if (tune) (void)hipEventRecord(m.tune_ev[1], s);
     }
     Mark(e);
     // ---- rebin: new layout in P (the previous layout's buffers are free now)
@@ -177,27 +220,31 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt) {
         rm.n[i] = m.n;
     }
     // exterior blocks are about 1.4 A; the kernels loop over any count
-    LaunchRebinCount(e->dom, C.keys, C.table, C.lc, rm, e->occ, e->tsum, P.keys, P.lc, P.table, e->cap,
-                     2u * e->a_hint + 64u, s);
-    LaunchRebinScan(C.keys, C.lc, rm, e->tsum, e->newidx, P.keys, P.table, P.lc, e->cap, e->status, s);
+    const RebinCountArgs ca{e->dom, C.keys, C.table, C.lc, rm, e->occ, e->tsum, P.keys, P.lc, P.table, e->cap};
+    Run(rec, kStCount, kFkRebinCount, 0, ca, RebinCountGrid(e->cap, 2u * e->a_hint + 64u), s);
+    const RebinScanArgs sa{C.keys, C.lc, rm, e->tsum, e->newidx, P.keys, P.table, P.lc, e->cap, e->status};
+    Run(rec, kStScan, kFkRebinScan, 0, sa, RebinScanGrid(e->cap), s);
     // Two independent chains from here: the particle permutation (RebinBase, Scatter:
     // current layout, orderings) and the extension of the next layout (its neighbour and
     // exterior blocks, bitmap, tile counts).  For large layouts the extension runs on the
     // side stream: the overlap saves ~24 us per substep at 17k particle blocks.
-    const bool fork = SideStreamReady(e, s);
+    const bool fork = !rec && SideStreamReady(e, s);
     if (fork) {
         (void)hipEventRecord(e->fork_ev, s);
         (void)hipStreamWaitEvent(e->side, e->fork_ev, 0);
         ExtendLayout(e, P, false, e->side);
     }
-    LaunchRebinBase(e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap, 2u * e->a_hint + 64u, s);
+    const RebinBaseArgs ba{e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap};
+    Run(rec, kStBase, kFkRebinBase, 0, ba, RebinBaseGrid(e->cap, 2u * e->a_hint + 64u), s);
     // a wavefront per (model, block), 4 per workgroup
-    LaunchRebinScatter(C.lc, rm, GridFor((e->a_hint + e->a_hint / 4) * static_cast<uint32_t>(rm.nmodels) / 4u + 8u), s);
+    const RebinScatterArgs xa{C.lc, rm};
+    Run(rec, kStScatter, kFkRebinScatter, 0, xa,
+        GridFor((e->a_hint + e->a_hint / 4) * static_cast<uint32_t>(rm.nmodels) / 4u + 8u), s);
     if (fork) {
         (void)hipEventRecord(e->join_ev, e->side);
         (void)hipStreamWaitEvent(s, e->join_ev, 0);
     } else {
-        ExtendLayout(e, P, false, s);
+        ExtendLayout(e, P, false, s, rec);
     }
     Mark(e);
     e->cur = 1 - e->cur;
@@ -739,6 +786,7 @@ struct BatchGraph {
     hipGraphExec_t exec = nullptr;
     std::vector<int> roles;         // engine buffer roles at capture
     std::vector<uint32_t> hints;    // block-count hints the launch sizes were chosen for
+    void* tables = nullptr;         // fused mode: argument tables and workgroup starts
 };
 
 struct ckmpm_batch {
@@ -749,6 +797,9 @@ struct ckmpm_batch {
     // the engines step together, so their buffer roles take one of two configurations
     // (even / odd substep count): one graph each
     BatchGraph g[2];
+    // mode 4: one graph per parity of fused launches (each kernel of a substep once for
+    // all scenes)
+    BatchGraph fz[2];
     // mode 3: one graph per engine and parity, replayed on the engine's own stream
     std::vector<BatchGraph> pe[2];
     uint64_t captures = 0;
@@ -780,11 +831,14 @@ void EagerSubstep(ckmpm_engine* e) {
     PhaseParticles(e, dt, e->dt_default);
 }
 
+// The graph's replays have completed (its tables are freed).
 void ReleaseGraph(BatchGraph& g) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.tables) (void)hipFree(g.tables);
     g.exec = nullptr;
     g.graph = nullptr;
+    g.tables = nullptr;
 }
 
 int Parity(const ckmpm_batch* b) { return static_cast<int>(b->eng[0]->substeps & 1u); }
@@ -852,6 +906,80 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
     return CKMPM_OK;
 }
 
+// Two substeps of every engine as fused launches: each (kernel, variant) of a substep
+// runs once for all scenes, its workgroups split by scene (LaunchFused), from argument
+// tables built here on the host and copied once.  The stages run in order on the capture
+// stream, so every engine's launches keep their order; the scenes' buffers are disjoint.
+int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
+    ReleaseGraph(g);
+    const size_t ne = b->eng.size();
+    g.roles.assign(ne, -1);
+    g.hints.assign(ne, 0);
+    FusedRec rec[2];
+    for (size_t i = 0; i < ne; ++i) {
+        g.roles[i] = EngineRoles(b->eng[i]);
+        g.hints[i] = b->eng[i]->a_hint;
+    }
+    for (int k = 0; k < 2; ++k)
+        for (ckmpm_engine* e : b->eng) {
+            const bool prof = e->profile;
+            e->profile = false;
+            PhaseGrid(e, e->dt_default, &rec[k]);
+            PhaseParticles(e, e->dt_default, e->dt_default, &rec[k]);
+            e->profile = prof;
+        }
+    for (ckmpm_engine* e : b->eng) e->substeps -= 2;  // recorded, not run
+    // host image of the tables: per launch, the argument structs then the workgroup starts
+    struct Launch {
+        int kernel, variant;
+        size_t args, begin;
+        uint32_t n, total;
+    };
+    std::vector<Launch> launches;
+    std::vector<uint8_t> img;
+    auto place = [&](const void* src, size_t bytes) {
+        const size_t off = (img.size() + 255) & ~static_cast<size_t>(255);
+        img.resize(off + bytes);
+        std::memcpy(img.data() + off, src, bytes);
+        return off;
+    };
+    for (int k = 0; k < 2; ++k) {
+        std::stable_sort(rec[k].ph.begin(), rec[k].ph.end(), [](const FusedPhase& x, const FusedPhase& y) {
+            if (x.stage != y.stage) return x.stage < y.stage;
+            if (x.kernel != y.kernel) return x.kernel < y.kernel;
+            return x.variant < y.variant;
+        });
+        for (const FusedPhase& p : rec[k].ph) {
+            std::vector<uint32_t> begin(p.grids.size() + 1, 0u);
+            uint64_t tot = 0;
+            for (size_t i = 0; i < p.grids.size(); ++i) {
+                begin[i] = static_cast<uint32_t>(tot);
+                tot += p.grids[i];
+            }
+            if (tot > 0x7FFFFFFFull / 1024u) return BatchFail(b, CKMPM_ERR_INVALID, "fused launch too large");
+            begin.back() = static_cast<uint32_t>(tot);
+            if (p.args.size() != FusedArgsSize(p.kernel) * p.grids.size())
+                return BatchFail(b, CKMPM_ERR_INVALID, "fused launch: argument table size");
+            Launch l{p.kernel, p.variant, 0, 0, static_cast<uint32_t>(p.grids.size()), static_cast<uint32_t>(tot)};
+            l.args = place(p.args.data(), p.args.size());
+            l.begin = place(begin.data(), begin.size() * sizeof(uint32_t));
+            launches.push_back(l);
+        }
+    }
+    BATCH_HIP(b, hipMalloc(&g.tables, img.size()));
+    BATCH_HIP(b, hipMemcpy(g.tables, img.data(), img.size(), hipMemcpyHostToDevice));
+    uint8_t* dev = static_cast<uint8_t*>(g.tables);
+    BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
+    for (const Launch& l : launches)
+        LaunchFused(l.kernel, l.variant, dev + l.args, reinterpret_cast<const uint32_t*>(dev + l.begin), l.n, l.total,
+                    b->cs);
+    const hipError_t rc = hipStreamEndCapture(b->cs, &g.graph);
+    if (rc != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(rc));
+    BATCH_HIP(b, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    ++b->captures;
+    return CKMPM_OK;
+}
+
 bool EngineGraphCurrent(const ckmpm_engine* e, const BatchGraph& g) {
     if (!g.exec || EngineRoles(e) != g.roles[0]) return false;
     const uint32_t h = g.hints[0], a = e->a_hint;
@@ -865,8 +993,11 @@ extern "C" {
 int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch** out) {
     if (!engines || !out || count == 0) return CKMPM_ERR_INVALID;
     *out = nullptr;
-    for (uint32_t i = 0; i < count; ++i)
+    for (uint32_t i = 0; i < count; ++i) {
         if (!engines[i] || !engines[i]->setup_done || engines[i]->adj) return CKMPM_ERR_INVALID;
+        for (uint32_t j = 0; j < i; ++j)
+            if (engines[j] == engines[i]) return CKMPM_ERR_INVALID;  // an engine appears once
+    }
     ckmpm_batch* b = new ckmpm_batch();
     b->eng.assign(engines, engines + count);
     b->join.resize(count, nullptr);
@@ -883,7 +1014,7 @@ int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch
 }
 
 int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode) {
-    if (!b || mode < 0 || mode > 3) return CKMPM_ERR_INVALID;
+    if (!b || mode < 0 || mode > 4) return CKMPM_ERR_INVALID;
     if (mode == 3) {
         uint32_t left = substeps;
         bool first = false;
@@ -929,7 +1060,7 @@ int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode) {
             for (ckmpm_engine* e : b->eng) EagerSubstep(e);
         return CKMPM_OK;
     }
-    // graph mode: a pending first substep (dt = 0) and an odd remainder run eagerly
+    // graph modes: a pending first substep (dt = 0) and an odd remainder run eagerly
     uint32_t left = substeps;
     bool first = false;
     for (ckmpm_engine* e : b->eng) first |= e->substeps == 0;
@@ -940,10 +1071,11 @@ int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode) {
     }
     const uint32_t pairs = left / 2;
     if (pairs) {
-        BatchGraph& g = b->g[Parity(b)];
+        BatchGraph& g = mode == 4 ? b->fz[Parity(b)] : b->g[Parity(b)];
         if (!GraphCurrent(b, g)) {
+            BATCH_HIP(b, hipStreamSynchronize(b->cs));
             for (ckmpm_engine* e : b->eng) BATCH_HIP(b, hipStreamSynchronize(e->stream));
-            const int rc = Capture(b, g);
+            const int rc = mode == 4 ? CaptureFused(b, g) : Capture(b, g);
             if (rc != CKMPM_OK) return rc;
         }
         // the replay stream runs after everything already queued on the engine streams
@@ -985,6 +1117,8 @@ int ckmpm_batch_destroy(ckmpm_batch* b) {
     if (b->cs) (void)hipStreamSynchronize(b->cs);
     ReleaseGraph(b->g[0]);
     ReleaseGraph(b->g[1]);
+    ReleaseGraph(b->fz[0]);
+    ReleaseGraph(b->fz[1]);
     for (int p = 0; p < 2; ++p)
         for (BatchGraph& g : b->pe[p]) ReleaseGraph(g);
     for (hipEvent_t ev : b->join)

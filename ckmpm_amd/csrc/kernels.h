@@ -57,16 +57,6 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
 // padded) and each extension pass has emit_tiles counters of newly set bits.
 constexpr uint32_t kEmitTileWords = 2048;
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
-// One extension pass: mark the candidates that are not in the table (pass 0: b + {0,1}^3
-// of the particle blocks, pass 1: b + {-1,0,1}^3), then emit them in ascending key order
-// after counts.A (pass 0, sets Nb) or counts.Nb (pass 1, sets Ex), registering them in
-// the table and clearing the bitmap.  tile_cnt holds 2 x emit_tiles counters, zero on
-// the first call (each pass clears the other's).
-// a_hint: expected particle-block count, sizes the marking grid (the kernel loops over
-// the device count, so any hint is correct; a close one saves launching idle workgroups)
-void LaunchExtensionPass(const DomainDev& d, uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap,
-                         uint32_t* bitmap, uint32_t emit_tiles, uint32_t* tile_cnt, int pass, Status* st,
-                         uint32_t a_hint, hipStream_t s);
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
 
@@ -97,7 +87,6 @@ struct GridUpdateArgs {
     int custom;                  // host copy of bc->custom: selects the kernel instantiation
     int slot;                    // substep parity: the Status slot reduced into (the other is cleared)
 };
-void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s);
 
 struct G2P2GArgs {
     DomainDev dom;
@@ -124,8 +113,6 @@ struct G2P2GArgs {
     const uint32_t* ids_in;
     uint32_t* ids_out;
 };
-// threads per workgroup: 256 or 512 (one particle block per workgroup either way)
-void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s);
 
 struct RebinModels {
     int nmodels;
@@ -141,19 +128,107 @@ struct RebinModels {
 // count tile sums: kTileSums words per tile of 32 exterior blocks
 constexpr int kTileSums = 1 + kMaxModels;
 size_t RebinTileSumWords(uint32_t cap);
-// Entering counts per exterior block of the current layout (occ, cnt_e, tile sums);
-// also clears the next layout's table entries.  ex_hint: expected exterior blocks.
-void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                      const RebinModels& rm, uint32_t* occ, uint32_t* tsum, const uint32_t* keys_next,
-                      const LayoutCounts* lc_next, uint32_t* table_next, uint32_t cap, uint32_t ex_hint,
-                      hipStream_t s);
-// Next layout's particle blocks (keys, table, A) and every model's block offsets.
-void LaunchRebinScan(const uint32_t* keys, const LayoutCounts* lc, const RebinModels& rm, const uint32_t* tsum, uint32_t* newidx, uint32_t* keys_next, uint32_t* table_next,
-                     LayoutCounts* lc_next, uint32_t cap, Status* st, hipStream_t s);
-// ex_hint: expected exterior-block count, sizes the grid (any value is correct)
-void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap,
-                     uint32_t ex_hint, hipStream_t s);
-void LaunchRebinScatter(const LayoutCounts* lc, const RebinModels& rm, uint32_t grid, hipStream_t s);
+
+// Rebin (RebinCount: entering counts per exterior block of the current layout, occ,
+// cnt_e, tile sums, and the next layout's table cleared; RebinScan: next layout's
+// particle blocks (keys, table, A) and every model's block offsets; RebinBase: first
+// destination of each (old block, move); RebinScatter: the next particle permutation)
+// and layout extension (Mark: candidates b + {0,1}^3 (lo = 0) or b + {-1,0,1}^3 (lo = -1)
+// not in the table, counted per emit tile; Emit: the marked blocks in ascending key
+// order after A (pass 0, sets Nb) or Nb (pass 1, sets Ex), registered in the table, the
+// bitmap cleared; tile_cnt holds 2 x emit_tiles counters, zero on the first pass, and
+// each pass clears the other's).
+struct RebinCountArgs {
+    DomainDev d;
+    const uint32_t* keys;
+    const uint32_t* table;
+    const LayoutCounts* lc;
+    RebinModels rm;
+    uint32_t* occ;
+    uint32_t* tsum;
+    const uint32_t* keys_next;
+    const LayoutCounts* lc_next;
+    uint32_t* table_next;
+    uint32_t cap;
+};
+struct RebinScanArgs {
+    const uint32_t* keys;
+    const LayoutCounts* lc;
+    RebinModels rm;
+    const uint32_t* tsum;
+    uint32_t* newidx;
+    uint32_t* keys_next;
+    uint32_t* table_next;
+    LayoutCounts* lc_next;
+    uint32_t cap;
+    Status* st;
+};
+struct RebinBaseArgs {
+    DomainDev d;
+    const uint32_t* keys;
+    const uint32_t* table;
+    const LayoutCounts* lc;
+    const uint32_t* occ;
+    const uint32_t* newidx;
+    RebinModels rm;
+    uint32_t cap;
+};
+struct RebinScatterArgs {
+    const LayoutCounts* lc;
+    RebinModels rm;
+};
+struct MarkArgs {
+    DomainDev d;
+    const uint32_t* keys;
+    const LayoutCounts* lc;
+    uint32_t cap;
+    const uint32_t* table;
+    uint32_t* bitmap;
+    uint32_t* tile_cnt;
+    int lo;
+};
+struct EmitArgs {
+    uint32_t* bitmap;
+    const uint32_t* tile_cnt;
+    uint32_t* tile_cnt_other;
+    uint32_t ntiles;
+    uint32_t* keys;
+    uint32_t* table;
+    LayoutCounts* lc;
+    uint32_t cap;
+    int pass;
+    Status* st;
+};
+
+// Kernels of one substep.  The grid-update and G2P2G entries have variants (the boundary
+// instantiation; material kind x workgroup size of 256 or 512 threads).
+enum FusedKernel : int {
+    kFkGrid = 0,      // variant: custom boundary
+    kFkG2P2G,         // variant: kind * 2 + (threads == 512)
+    kFkRebinCount,
+    kFkRebinScan,
+    kFkRebinBase,
+    kFkRebinScatter,
+    kFkMark,
+    kFkEmit,
+};
+size_t FusedArgsSize(int kernel);  // bytes of the kernel's argument struct
+// One launch of grid workgroups; args points to the kernel's argument struct.
+void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStream_t s);
+// A fused launch runs one phase for many scenes: table holds one argument struct per scene
+// (FusedArgsSize bytes each), begin[i] is the first workgroup of scene i (begin[0] = 0,
+// begin[n] = total), and workgroup w of the launch runs workgroup w - begin[i] of a
+// launch of begin[i+1] - begin[i] workgroups for scene i.  The scenes must not share
+// buffers.
+void LaunchFused(int kernel, int variant, const void* table, const uint32_t* begin, uint32_t n, uint32_t total,
+                 hipStream_t s);
+// Workgroup counts: RebinCount and RebinBase for about ex_hint exterior blocks, Mark for
+// about a_hint particle blocks (the kernels loop over the device counts, so any hint is
+// correct; a close one saves idle workgroups); RebinScan must cover the capacity.
+uint32_t RebinCountGrid(uint32_t cap, uint32_t ex_hint);
+uint32_t RebinScanGrid(uint32_t cap);
+uint32_t RebinBaseGrid(uint32_t cap, uint32_t ex_hint);
+uint32_t MarkGrid(uint32_t cap, uint32_t a_hint, int pass);
+uint32_t G2P2GGrid(uint32_t grid);
 
 }  // namespace ckmpm

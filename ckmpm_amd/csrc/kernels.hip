@@ -379,13 +379,20 @@ __global__ void TableInsertAKernel(const uint32_t* keys, const LayoutCounts* lc,
 // counts are aggregated per wavefront (its candidates span one to three tiles): one
 // atomic per distinct tile instead of one per bit, which would serialize on the few
 // counter addresses.
-__global__ void MarkExtensionKernel(DomainDev d, const uint32_t* keys, const LayoutCounts* lc, uint32_t cap,
-                                    const uint32_t* table, uint32_t* bitmap, uint32_t* tile_cnt, int lo) {
-    const uint32_t A = min(lc->A, cap);
+// wg / ng: workgroup index and workgroup count of the launch (the scene's part of a fused
+// launch); the same holds for every kernel body below.
+__device__ __forceinline__ void MarkExtensionBody(const MarkArgs& a, uint32_t wg, uint32_t ng) {
+    const DomainDev& d = a.d;
+    const uint32_t* keys = a.keys;
+    const uint32_t* table = a.table;
+    uint32_t* bitmap = a.bitmap;
+    uint32_t* tile_cnt = a.tile_cnt;
+    const int lo = a.lo;
+    const uint32_t A = min(a.lc->A, a.cap);
     const int span = 2 - lo;
     const int per = span * span * span;
     const uint32_t total = A * static_cast<uint32_t>(per);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    for (uint32_t q = wg * kThreads + threadIdx.x; q < total; q += ng * kThreads) {
         const uint32_t i = q / per;
         const int o = static_cast<int>(q % per);
         int bx, by, bz;
@@ -411,6 +418,7 @@ __global__ void MarkExtensionKernel(DomainDev d, const uint32_t* keys, const Lay
         }
     }
 }
+__global__ __launch_bounds__(kThreads) void MarkExtensionKernel(MarkArgs a) { MarkExtensionBody(a, blockIdx.x, gridDim.x); }
 
 // Reduce per-thread values over a workgroup into red[0, nq) (all threads see the result
 // after the call).  red must hold nq entries.
@@ -437,13 +445,19 @@ __device__ __forceinline__ void BlockSums(const uint32_t (&v)[NQ], int nq, uint3
 // writes the set bits in ascending key order to keys[] and the table, clearing the
 // bitmap.  Tiles without a set bit return at once.  Workgroup 0 also sets the count
 // and clears the other pass's tile counts (zero again before that pass marks).
-__global__ __launch_bounds__(kThreads) void BitmapScanEmitKernel(uint32_t* bitmap, const uint32_t* tile_cnt,
-                                                                 uint32_t* tile_cnt_other, uint32_t ntiles,
-                                                                 uint32_t* keys, uint32_t* table, LayoutCounts* lc,
-                                                                 uint32_t cap, int pass, Status* st) {
+__device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t wg, uint32_t) {
     __shared__ uint32_t sh[kThreads / 64];
     __shared__ uint32_t red[1];
-    const uint32_t w = blockIdx.x, tid = threadIdx.x;
+    uint32_t* bitmap = a.bitmap;
+    const uint32_t* tile_cnt = a.tile_cnt;
+    uint32_t* tile_cnt_other = a.tile_cnt_other;
+    const uint32_t ntiles = a.ntiles, cap = a.cap;
+    uint32_t* keys = a.keys;
+    uint32_t* table = a.table;
+    LayoutCounts* lc = a.lc;
+    const int pass = a.pass;
+    Status* st = a.st;
+    const uint32_t w = wg, tid = threadIdx.x;
     const uint32_t base = pass == 0 ? lc->A : lc->Nb;
     if (w == 0) {
         uint32_t v[1] = {0u};
@@ -493,6 +507,7 @@ __global__ __launch_bounds__(kThreads) void BitmapScanEmitKernel(uint32_t* bitma
         }
     }
 }
+__global__ __launch_bounds__(kThreads) void BitmapScanEmitKernel(EmitArgs a) { BitmapScanEmitBody(a, blockIdx.x, gridDim.x); }
 
 // Emit set bits in ascending key order at position base + word_off[w] + rank.
 // pass 0 / 1: append after A / after Nb and register in the table; pass 2: particle
@@ -605,17 +620,17 @@ __device__ __forceinline__ void ApplyBoundary(const ckmpm_boundary_desc& bc, con
 // kCustom: the scene's boundary description replaces the default sticky margin (a
 // separate instantiation, so that the default path keeps its registers and code size)
 template <bool kCustom>
-__global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
+__device__ __forceinline__ void GridUpdateBody(const GridUpdateArgs& a, uint32_t wg, uint32_t ng) {
     const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
     const uint32_t n = threadIdx.x & 127u;
-    const uint32_t stride = gridDim.x * 2;
+    const uint32_t stride = ng * 2;
     // clear the other parity slot for the next substep (nothing else touches it now)
-    if (blockIdx.x == 0) {
+    if (wg == 0) {
         if (threadIdx.x == 0) a.st->vmax2_bits[a.slot ^ 1] = 0u;
         if (threadIdx.x < static_cast<uint32_t>(kMaxModels)) a.st->dep_bits[a.slot ^ 1][threadIdx.x] = 0u;
     }
     float vmax = 0.0f;
-    for (uint32_t i0 = blockIdx.x * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
+    for (uint32_t i0 = wg * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
         uint32_t key[kGridIlp], io[kGridIlp];
 #pragma unroll
         for (int u = 0; u < kGridIlp; ++u) {
@@ -689,7 +704,7 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
     for (int m = 0; m < kMaxModels; ++m) {
         bmax[m] = 0.0f;
         if (m < a.nmodels)
-            for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < Ap; j += gridDim.x * kThreads)
+            for (uint32_t j = wg * kThreads + threadIdx.x; j < Ap; j += ng * kThreads)
                 bmax[m] = fmaxf(bmax[m], a.blkmax[m][j]);
     }
 #pragma unroll
@@ -712,6 +727,10 @@ __global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
         for (int w = 0; w < kThreads / 64; ++w) mx = fmaxf(mx, sMax[threadIdx.x][w]);
         MaxBits(threadIdx.x == 0 ? &a.st->vmax2_bits[a.slot] : &a.st->dep_bits[a.slot][threadIdx.x - 1], mx);
     }
+}
+template <bool kCustom>
+__global__ __launch_bounds__(kThreads) void GridUpdateKernel(GridUpdateArgs a) {
+    GridUpdateBody<kCustom>(a, blockIdx.x, gridDim.x);
 }
 
 // ============================================================== G2P2G
@@ -827,7 +846,7 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
 
 // 4 waves per SIMD: LDS allows 4 workgroups per CU; this caps the VGPRs at 128
 template <int KIND, int THREADS>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GKernel(G2P2GArgs a) {
+__device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint32_t ng) {
     __shared__ float sV[kVChannels * kMWin];
     __shared__ __align__(16) LdsAcc sM[kMChannels * kMWin];
     __shared__ uint32_t sHalo[8];
@@ -856,7 +875,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     // Round-robin over the block list: particle counts per block vary by orders of
     // magnitude along the list (a model occupies only part of it), so contiguous
     // per-XCD ranges leave most XCDs idle; interleaving balances them.
-    for (uint32_t j = blockIdx.x; j < A; j += gridDim.x) {
+    for (uint32_t j = wg; j < A; j += ng) {
         const uint32_t beg = a.off[j], end = a.off[j + 1];
         if (beg == end) {
             // no particle of this model here: only the (zero) move counts are written
@@ -1204,6 +1223,10 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         __syncthreads();
     }
 }
+template <int KIND, int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GKernel(G2P2GArgs a) {
+    G2P2GBody<KIND, THREADS>(a, blockIdx.x, gridDim.x);
+}
 
 // ============================================================== rebin
 __device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
@@ -1219,21 +1242,24 @@ __device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
 // consecutive exterior blocks and writes the tile's sums (occupied blocks, entering
 // particles per model) for the scan kernel.  The same launch clears the next layout's
 // table entries (its keys are still those of two substeps ago).
-__global__ __launch_bounds__(kCountThreads) void RebinCountKernel(DomainDev d, const uint32_t* keys,
-                                                                  const uint32_t* table, const LayoutCounts* lc,
-                                                                  RebinModels rm, uint32_t* occ, uint32_t* tsum,
-                                                                  const uint32_t* keys_next,
-                                                                  const LayoutCounts* lc_next, uint32_t* table_next,
-                                                                  uint32_t cap) {
+__device__ __forceinline__ void RebinCountBody(const RebinCountArgs& a, uint32_t wg, uint32_t ng) {
     __shared__ uint32_t sG[kCountTile][kTileSums];
-    const uint32_t exn = min(lc_next->Ex, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < exn; i += gridDim.x * blockDim.x)
-        table_next[keys_next[i]] = kEmpty;
+    const DomainDev& d = a.d;
+    const uint32_t* keys = a.keys;
+    const uint32_t* table = a.table;
+    const RebinModels& rm = a.rm;
+    uint32_t* occ = a.occ;
+    uint32_t* tsum = a.tsum;
+    const uint32_t cap = a.cap;
+    const LayoutCounts* lc = a.lc;
+    const uint32_t exn = min(a.lc_next->Ex, cap);
+    for (uint32_t i = wg * kCountThreads + threadIdx.x; i < exn; i += ng * kCountThreads)
+        a.table_next[a.keys_next[i]] = kEmpty;
     const uint32_t A = lc->A, Ex = min(lc->Ex, cap);
     const uint32_t lane = threadIdx.x & 31u, grp = threadIdx.x >> 5;
     const int nq = 1 + rm.nmodels;
     const uint32_t ntiles = (Ex + kCountTile - 1) / kCountTile;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (uint32_t tile = wg; tile < ntiles; tile += ng) {
         const uint32_t e = tile * kCountTile + grp;
         if (e < Ex) {
             int bx, by, bz;
@@ -1272,6 +1298,9 @@ __global__ __launch_bounds__(kCountThreads) void RebinCountKernel(DomainDev d, c
         __syncthreads();
     }
 }
+__global__ __launch_bounds__(kCountThreads) void RebinCountKernel(RebinCountArgs a) {
+    RebinCountBody(a, blockIdx.x, gridDim.x);
+}
 
 // Next layout's particle blocks and particle offsets in one pass: workgroup w owns
 // exterior blocks [w, w+1) * kRebinScanTile of the current layout, takes the sums of
@@ -1280,20 +1309,25 @@ __global__ __launch_bounds__(kCountThreads) void RebinCountKernel(DomainDev d, c
 // counts (offsets of the next layout).  Unoccupied blocks enter no particle of any
 // model, so the scan over all exterior blocks equals the scan over the compacted
 // list.  Workgroup 0 also sets A of the next layout and the end offsets (= n).
-__global__ __launch_bounds__(kThreads) void RebinScanKernel(const uint32_t* keys, const LayoutCounts* lc,
-                                                            RebinModels rm, const uint32_t* tsum, uint32_t* newidx,
-                                                            uint32_t* keys_next, uint32_t* table_next,
-                                                            LayoutCounts* lc_next, uint32_t cap, Status* st) {
+// The launch covers the capacity: ng * kRebinScanTile >= cap.
+__device__ __forceinline__ void RebinScanBody(const RebinScanArgs& a, uint32_t wg, uint32_t) {
     __shared__ uint32_t sh[kThreads / 64];
     __shared__ uint32_t red[kTileSums];
     __shared__ uint32_t sPre[kTileSums];
-    const uint32_t Ex = min(lc->Ex, cap), tid = threadIdx.x;
+    const uint32_t* keys = a.keys;
+    const RebinModels& rm = a.rm;
+    const uint32_t* tsum = a.tsum;
+    uint32_t* newidx = a.newidx;
+    uint32_t* keys_next = a.keys_next;
+    uint32_t* table_next = a.table_next;
+    const uint32_t cap = a.cap;
+    const uint32_t Ex = min(a.lc->Ex, cap), tid = threadIdx.x;
     const int nq = 1 + rm.nmodels;
-    const uint32_t base = blockIdx.x * kRebinScanTile;
-    if (base >= Ex && blockIdx.x != 0) return;
+    const uint32_t base = wg * kRebinScanTile;
+    if (base >= Ex && wg != 0) return;
     // sums of the count tiles before this tile (workgroup 0: of all tiles, the totals)
     const uint32_t ntc = (Ex + kCountTile - 1) / kCountTile;
-    const uint32_t upto = blockIdx.x == 0 ? ntc : min(ntc, base / kCountTile);
+    const uint32_t upto = wg == 0 ? ntc : min(ntc, base / kCountTile);
     // every global load of the workgroup is issued before the first barrier: the count
     // tiles before this tile, and the keys and entering counts of the thread's blocks
     // (thread i owns [base + i*8, base + i*8 + 8); a block is occupied iff a model's
@@ -1315,15 +1349,15 @@ __global__ __launch_bounds__(kThreads) void RebinScanKernel(const uint32_t* keys
 #pragma unroll
         for (int k = 0; k < kRebinScanItems; ++k) c[m][k] = m < rm.nmodels && e0 + k < Ex ? rm.cnt_e[m][e0 + k] : 0u;
     BlockSums<kTileSums>(v, nq, red);
-    if (tid < static_cast<uint32_t>(nq)) sPre[tid] = blockIdx.x == 0 ? 0u : red[tid];
-    if (blockIdx.x == 0 && tid == 0) {
-        uint32_t a = red[0];
-        if (a > cap) {
-            SetFlag(st, kFlagCapacity);
-            a = cap;
+    if (tid < static_cast<uint32_t>(nq)) sPre[tid] = wg == 0 ? 0u : red[tid];
+    if (wg == 0 && tid == 0) {
+        uint32_t an = red[0];
+        if (an > cap) {
+            SetFlag(a.st, kFlagCapacity);
+            an = cap;
         }
-        lc_next->A = a;
-        for (int m = 0; m < rm.nmodels; ++m) rm.off_next[m][a] = rm.n[m];
+        a.lc_next->A = an;
+        for (int m = 0; m < rm.nmodels; ++m) rm.off_next[m][an] = rm.n[m];
     }
     __syncthreads();
     uint32_t omask = 0;
@@ -1366,19 +1400,25 @@ __global__ __launch_bounds__(kThreads) void RebinScanKernel(const uint32_t* keys
         }
     }
 }
+__global__ __launch_bounds__(kThreads) void RebinScanKernel(RebinScanArgs a) { RebinScanBody(a, blockIdx.x, gridDim.x); }
 
 // First destination of every (source block, move) pair entering an occupied block:
 // sources are ordered by their list index, so the destination order is the stable
 // order of the previous processing position.  Half a wave (32 lanes) per exterior
 // block: lane d owns move d; the ordering sum is done with lane shuffles.
-__global__ __launch_bounds__(kThreads) void RebinBaseKernel(DomainDev d, const uint32_t* keys, const uint32_t* table,
-                                                            const LayoutCounts* lc, const uint32_t* occ,
-                                                            const uint32_t* newidx, RebinModels rm, uint32_t cap) {
-    const uint32_t A = lc->A, Ex = min(lc->Ex, cap);
+__device__ __forceinline__ void RebinBaseBody(const RebinBaseArgs& a, uint32_t wg, uint32_t ng) {
+    const DomainDev& d = a.d;
+    const uint32_t* keys = a.keys;
+    const uint32_t* table = a.table;
+    const uint32_t* occ = a.occ;
+    const uint32_t* newidx = a.newidx;
+    const RebinModels& rm = a.rm;
+    const uint32_t cap = a.cap;
+    const uint32_t A = a.lc->A, Ex = min(a.lc->Ex, cap);
     const uint32_t lane = LaneId(), half = lane >> 5, dd = lane & 31u;
-    const uint32_t waves = blockDim.x >> 6;
-    const uint32_t step = gridDim.x * waves * 2;
-    for (uint32_t e0 = (blockIdx.x * waves + (threadIdx.x >> 6)) * 2; e0 < Ex; e0 += step) {
+    const uint32_t waves = kThreads / 64;
+    const uint32_t step = ng * waves * 2;
+    for (uint32_t e0 = (wg * waves + (threadIdx.x >> 6)) * 2; e0 < Ex; e0 += step) {
         const uint32_t e = e0 + half;
         const bool valid = e < Ex && occ[e] != 0u;
         // the extension blocks at the end of the list are mostly unoccupied: whole waves skip
@@ -1408,16 +1448,18 @@ __global__ __launch_bounds__(kThreads) void RebinBaseKernel(DomainDev d, const u
         }
     }
 }
+__global__ __launch_bounds__(kThreads) void RebinBaseKernel(RebinBaseArgs a) { RebinBaseBody(a, blockIdx.x, gridDim.x); }
 
 // Particle permutation of the next layout for every model: a wavefront per (model,
 // block) pair, kScatterUnroll particles per lane in flight (a block holds a few hundred
 // particles per model, so most pairs take one pass of dependent loads: 512 per pass).
 constexpr int kScatterUnroll = 8;
-__global__ __launch_bounds__(kThreads) void RebinScatterKernel(const LayoutCounts* lc, RebinModels rm) {
-    const uint32_t A = lc->A;
+__device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint32_t wg, uint32_t ng) {
+    const RebinModels& rm = a.rm;
+    const uint32_t A = a.lc->A;
     const uint32_t pairs = A * static_cast<uint32_t>(rm.nmodels);
     const uint32_t lane = LaneId(), waves = kThreads / 64;
-    for (uint32_t p = blockIdx.x * waves + (threadIdx.x >> 6); p < pairs; p += gridDim.x * waves) {
+    for (uint32_t p = wg * waves + (threadIdx.x >> 6); p < pairs; p += ng * waves) {
         const uint32_t m = p / A, j = p - m * A;
         const uint32_t* off = rm.off_cur[m];
         const uint32_t beg = off[j], end = off[j + 1];
@@ -1441,6 +1483,9 @@ __global__ __launch_bounds__(kThreads) void RebinScatterKernel(const LayoutCount
             }
         }
     }
+}
+__global__ __launch_bounds__(kThreads) void RebinScatterKernel(RebinScatterArgs a) {
+    RebinScatterBody(a, blockIdx.x, gridDim.x);
 }
 
 // rank of each element inside its run of equal keys (from run heads and their positions)
@@ -1549,25 +1594,11 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s) {
     TableInsertAKernel<<<Blocks(cap, kThreads, 1024), kThreads, 0, s>>>(keys, lc, cap, table);
 }
-void LaunchExtensionPass(const DomainDev& d, uint32_t* keys, uint32_t* table, LayoutCounts* lc, uint32_t cap,
-                         uint32_t* bitmap, uint32_t emit_tiles, uint32_t* tile_cnt, int pass, Status* st,
-                         uint32_t a_hint, hipStream_t s) {
-    const uint32_t per = pass == 0 ? 8 : 27;
-    MarkExtensionKernel<<<Blocks(static_cast<uint64_t>(std::min(cap, a_hint)) * per, kThreads), kThreads, 0, s>>>(
-        d, keys, lc, cap, table, bitmap, tile_cnt + pass * emit_tiles, pass == 0 ? 0 : -1);
-    BitmapScanEmitKernel<<<emit_tiles, kThreads, 0, s>>>(bitmap, tile_cnt + pass * emit_tiles,
-                                                         tile_cnt + (1 - pass) * emit_tiles, emit_tiles, keys, table,
-                                                         lc, cap, pass, st);
-}
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s) {
     BitmapEmitKernel<<<Blocks(nwords, kThreads), kThreads, 0, s>>>(bitmap, nwords, word_off, keys, nullptr, lc, cap, 2,
                                                                     st);
     SetCountKernel<<<1, 64, 0, s>>>(lc, 0, total, cap, st);
-}
-void LaunchGridUpdate(const GridUpdateArgs& a, uint32_t grid, hipStream_t s) {
-    if (a.custom) GridUpdateKernel<true><<<grid, kThreads, 0, s>>>(a);
-    else GridUpdateKernel<false><<<grid, kThreads, 0, s>>>(a);
 }
 template <int T>
 void LaunchG2P2GT(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
@@ -1580,33 +1611,151 @@ void LaunchG2P2GT(const G2P2GArgs& a, uint32_t grid, hipStream_t s) {
         default: G2P2GKernel<kFluid, T><<<grid, T, 0, s>>>(a); break;
     }
 }
-void LaunchG2P2G(const G2P2GArgs& a, uint32_t grid, int threads, hipStream_t s) {
-    grid = ((grid + 7) / 8) * 8;
-    if (threads == 512) LaunchG2P2GT<512>(a, grid, s);
-    else LaunchG2P2GT<256>(a, grid, s);
+uint32_t G2P2GGrid(uint32_t grid) { return ((grid + 7) / 8) * 8; }
+// one tile per workgroup for up to ex_hint exterior blocks; the loops cover any count
+uint32_t RebinCountGrid(uint32_t cap, uint32_t ex_hint) { return Blocks(std::min(ex_hint, cap), kCountTile, 2048); }
+uint32_t RebinScanGrid(uint32_t cap) { return std::max<uint32_t>(1u, (cap + kRebinScanTile - 1) / kRebinScanTile); }
+uint32_t RebinBaseGrid(uint32_t cap, uint32_t ex_hint) { return Blocks(std::min(cap, ex_hint), 8); }
+uint32_t MarkGrid(uint32_t cap, uint32_t a_hint, int pass) {
+    return Blocks(static_cast<uint64_t>(std::min(cap, a_hint)) * (pass == 0 ? 8 : 27), kThreads);
 }
-void LaunchRebinCount(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                      const RebinModels& rm, uint32_t* occ, uint32_t* tsum, const uint32_t* keys_next,
-                      const LayoutCounts* lc_next, uint32_t* table_next, uint32_t cap, uint32_t ex_hint,
-                      hipStream_t s) {
-    // one tile per workgroup for up to ex_hint exterior blocks; the loops cover any count
-    const uint32_t grid = Blocks(std::min(ex_hint, cap), kCountTile, 2048);
-    RebinCountKernel<<<grid, kCountThreads, 0, s>>>(d, keys, table, lc, rm, occ, tsum, keys_next, lc_next, table_next,
-                                                    cap);
+
+void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStream_t s) {
+    switch (kernel) {
+        case kFkGrid: {
+            const auto& a = *static_cast<const GridUpdateArgs*>(args);
+            if (variant) GridUpdateKernel<true><<<grid, kThreads, 0, s>>>(a);
+            else GridUpdateKernel<false><<<grid, kThreads, 0, s>>>(a);
+            break;
+        }
+        case kFkG2P2G: {
+            const auto& a = *static_cast<const G2P2GArgs*>(args);
+            if (variant & 1) LaunchG2P2GT<512>(a, grid, s);
+            else LaunchG2P2GT<256>(a, grid, s);
+            break;
+        }
+        case kFkRebinCount:
+            RebinCountKernel<<<grid, kCountThreads, 0, s>>>(*static_cast<const RebinCountArgs*>(args));
+            break;
+        case kFkRebinScan: RebinScanKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinScanArgs*>(args)); break;
+        case kFkRebinBase: RebinBaseKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinBaseArgs*>(args)); break;
+        case kFkRebinScatter:
+            RebinScatterKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinScatterArgs*>(args));
+            break;
+        case kFkMark: MarkExtensionKernel<<<grid, kThreads, 0, s>>>(*static_cast<const MarkArgs*>(args)); break;
+        default: BitmapScanEmitKernel<<<grid, kThreads, 0, s>>>(*static_cast<const EmitArgs*>(args)); break;
+    }
 }
-void LaunchRebinScan(const uint32_t* keys, const LayoutCounts* lc, const RebinModels& rm, const uint32_t* tsum, uint32_t* newidx, uint32_t* keys_next, uint32_t* table_next,
-                     LayoutCounts* lc_next, uint32_t cap, Status* st, hipStream_t s) {
-    const uint32_t grid = (cap + kRebinScanTile - 1) / kRebinScanTile;
-    RebinScanKernel<<<grid ? grid : 1, kThreads, 0, s>>>(keys, lc, rm, tsum, newidx, keys_next, table_next, lc_next,
-                                                         cap, st);
+
+// ---------------------------------------------------------------- fused batches
+// Scene of this workgroup in a fused launch: the last i with begin[i] <= blockIdx.x
+// (scenes of zero workgroups share their begin with the next scene and are skipped).
+// Every wave counts the entries at or below blockIdx.x with a ballot over 64 entries at
+// a time, so the search costs one round of loads for up to 64 scenes.
+__device__ __forceinline__ uint32_t FusedScene(const uint32_t* __restrict__ begin, uint32_t n, uint32_t& wg,
+                                               uint32_t& ng) {
+    const uint32_t b = blockIdx.x, lane = LaneId();
+    uint32_t cnt = 0;
+    for (uint32_t c = 0; c < n; c += 64) {
+        const uint32_t t = c + lane;
+        cnt += static_cast<uint32_t>(__popcll(__ballot(t < n && begin[t] <= b)));
+    }
+    const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(cnt - 1u)));
+    const uint32_t b0 = begin[i];
+    wg = b - b0;
+    ng = begin[i + 1] - b0;
+    return i;
 }
-void LaunchRebinBase(const DomainDev& d, const uint32_t* keys, const uint32_t* table, const LayoutCounts* lc,
-                     const uint32_t* occ, const uint32_t* newidx, const RebinModels& rm, uint32_t cap,
-                     uint32_t ex_hint, hipStream_t s) {
-    RebinBaseKernel<<<Blocks(std::min(cap, ex_hint), 8), kThreads, 0, s>>>(d, keys, table, lc, occ, newidx, rm, cap);
+
+template <bool kCustom>
+__global__ __launch_bounds__(kThreads) void GridUpdateFused(const GridUpdateArgs* __restrict__ tab,
+                                                            const uint32_t* __restrict__ begin, uint32_t n) {
+    uint32_t wg, ng;
+    const uint32_t i = FusedScene(begin, n, wg, ng);
+    GridUpdateBody<kCustom>(tab[i], wg, ng);
 }
-void LaunchRebinScatter(const LayoutCounts* lc, const RebinModels& rm, uint32_t grid, hipStream_t s) {
-    RebinScatterKernel<<<grid, kThreads, 0, s>>>(lc, rm);
+template <int KIND, int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GFused(
+    const G2P2GArgs* __restrict__ tab, const uint32_t* __restrict__ begin, uint32_t n) {
+    uint32_t wg, ng;
+    const uint32_t i = FusedScene(begin, n, wg, ng);
+    G2P2GBody<KIND, THREADS>(tab[i], wg, ng);
+}
+#define CKMPM_FUSED(NAME, BODY, ARGS, THREADS)                                                                  \
+    __global__ __launch_bounds__(THREADS) void NAME(const ARGS* __restrict__ tab, const uint32_t* __restrict__ begin, \
+                                                    uint32_t n) {                                                 \
+        uint32_t wg, ng;                                                                                        \
+        const uint32_t i = FusedScene(begin, n, wg, ng);                                                        \
+        BODY(tab[i], wg, ng);                                                                                   \
+    }
+CKMPM_FUSED(RebinCountFused, RebinCountBody, RebinCountArgs, kCountThreads)
+CKMPM_FUSED(RebinScanFused, RebinScanBody, RebinScanArgs, kThreads)
+CKMPM_FUSED(RebinBaseFused, RebinBaseBody, RebinBaseArgs, kThreads)
+CKMPM_FUSED(RebinScatterFused, RebinScatterBody, RebinScatterArgs, kThreads)
+CKMPM_FUSED(MarkExtensionFused, MarkExtensionBody, MarkArgs, kThreads)
+CKMPM_FUSED(BitmapScanEmitFused, BitmapScanEmitBody, EmitArgs, kThreads)
+#undef CKMPM_FUSED
+
+size_t FusedArgsSize(int kernel) {
+    switch (kernel) {
+        case kFkGrid: return sizeof(GridUpdateArgs);
+        case kFkG2P2G: return sizeof(G2P2GArgs);
+        case kFkRebinCount: return sizeof(RebinCountArgs);
+        case kFkRebinScan: return sizeof(RebinScanArgs);
+        case kFkRebinBase: return sizeof(RebinBaseArgs);
+        case kFkRebinScatter: return sizeof(RebinScatterArgs);
+        case kFkMark: return sizeof(MarkArgs);
+        default: return sizeof(EmitArgs);
+    }
+}
+
+template <int T>
+void LaunchG2P2GFusedT(int kind, const G2P2GArgs* t, const uint32_t* begin, uint32_t n, uint32_t total, hipStream_t s) {
+    switch (kind) {
+        case kFixedCorotated: G2P2GFused<kFixedCorotated, T><<<total, T, 0, s>>>(t, begin, n); break;
+        case kNeoHookean: G2P2GFused<kNeoHookean, T><<<total, T, 0, s>>>(t, begin, n); break;
+        case kDruckerPrager: G2P2GFused<kDruckerPrager, T><<<total, T, 0, s>>>(t, begin, n); break;
+        case kVonMises: G2P2GFused<kVonMises, T><<<total, T, 0, s>>>(t, begin, n); break;
+        case kCamClay: G2P2GFused<kCamClay, T><<<total, T, 0, s>>>(t, begin, n); break;
+        default: G2P2GFused<kFluid, T><<<total, T, 0, s>>>(t, begin, n); break;
+    }
+}
+
+void LaunchFused(int kernel, int variant, const void* table, const uint32_t* begin, uint32_t n, uint32_t total,
+                 hipStream_t s) {
+    if (n == 0 || total == 0) return;
+    switch (kernel) {
+        case kFkGrid: {
+            const auto* t = static_cast<const GridUpdateArgs*>(table);
+            if (variant) GridUpdateFused<true><<<total, kThreads, 0, s>>>(t, begin, n);
+            else GridUpdateFused<false><<<total, kThreads, 0, s>>>(t, begin, n);
+            break;
+        }
+        case kFkG2P2G: {
+            const auto* t = static_cast<const G2P2GArgs*>(table);
+            if (variant & 1) LaunchG2P2GFusedT<512>(variant >> 1, t, begin, n, total, s);
+            else LaunchG2P2GFusedT<256>(variant >> 1, t, begin, n, total, s);
+            break;
+        }
+        case kFkRebinCount:
+            RebinCountFused<<<total, kCountThreads, 0, s>>>(static_cast<const RebinCountArgs*>(table), begin, n);
+            break;
+        case kFkRebinScan:
+            RebinScanFused<<<total, kThreads, 0, s>>>(static_cast<const RebinScanArgs*>(table), begin, n);
+            break;
+        case kFkRebinBase:
+            RebinBaseFused<<<total, kThreads, 0, s>>>(static_cast<const RebinBaseArgs*>(table), begin, n);
+            break;
+        case kFkRebinScatter:
+            RebinScatterFused<<<total, kThreads, 0, s>>>(static_cast<const RebinScatterArgs*>(table), begin, n);
+            break;
+        case kFkMark:
+            MarkExtensionFused<<<total, kThreads, 0, s>>>(static_cast<const MarkArgs*>(table), begin, n);
+            break;
+        default:
+            BitmapScanEmitFused<<<total, kThreads, 0, s>>>(static_cast<const EmitArgs*>(table), begin, n);
+            break;
+    }
 }
 
 }  // namespace ckmpm

@@ -332,9 +332,11 @@ class MPMBatch:
 
     mode 0 runs the scenes one after another with plain launches (the sequential
     baseline), mode 1 interleaves them on their own streams, mode 2 replays a hipGraph
-    of two substeps of every engine.  The engines stay owned by the caller."""
+    of two substeps of every engine, mode 3 one hipGraph per engine, and mode 4 a
+    hipGraph in which each kernel of a substep is one launch for all scenes.  The
+    engines stay owned by the caller."""
 
-    SEQUENTIAL, STREAMS, GRAPH, ENGINE_GRAPHS = 0, 1, 2, 3
+    SEQUENTIAL, STREAMS, GRAPH, ENGINE_GRAPHS, FUSED = 0, 1, 2, 3, 4
 
     def __init__(self, engines: Sequence[MPMEngine]):
         self._lib = native.load()

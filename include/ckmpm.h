@@ -280,8 +280,9 @@ int ckmpm_get_positions_input_order(ckmpm_engine* eng, int32_t model, float* xyz
 /* ---------------------------------------------------------------- batched scenes
  * No reference counterpart (the reference engine is a singleton, mpm_engine.cuh:80-84):
  * many independent engines advanced together.  The batch does not own the engines;
- * each must have run ckmpm_initial_setup.  Every engine uses its own sim.dt (an engine
- * that has not run a substep yet starts with dt = 0, as mpm_engine.cuh:479 does). */
+ * each must have run ckmpm_initial_setup, and appears in the batch once
+ * (CKMPM_ERR_INVALID otherwise).  Every engine uses its own sim.dt (an engine that has
+ * not run a substep yet starts with dt = 0, as mpm_engine.cuh:479 does). */
 typedef struct ckmpm_batch ckmpm_batch;
 int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch** out);
 /* mode 0: one engine after the other, plain launches, synchronised per engine (the
@@ -289,7 +290,11 @@ int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch
  * mode 1: substep k of every engine enqueued on the engines' own streams, no sync;
  * mode 2: as 1, but pairs of substeps replay one hipGraph captured over all engine
  *         streams (recaptured when an engine's block count drifts by more than 1/4);
- * mode 3: pairs of substeps replay one hipGraph per engine on the engine's own stream. */
+ * mode 3: pairs of substeps replay one hipGraph per engine on the engine's own stream;
+ * mode 4: as 2, but each kernel of a substep is one launch for all engines (one per
+ *         material kind and workgroup size for G2P2G, per boundary kind for the grid
+ *         update), its workgroups split by engine; about 12 launches per substep for the
+ *         whole batch.  The graph reads argument tables that it holds in device memory. */
 int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode);
 /* waits for every engine; returns the first engine error */
 int ckmpm_batch_synchronize(ckmpm_batch* b);

@@ -1,8 +1,9 @@
 """Batched scenes (ckmpm_batch_*).
 
-The graph replay must give each engine the same result as stepping it alone with
-ckmpm_substep: same block lists and orderings (integer, bit-exact), same particle
-state up to the order of the floating-point atomics (the only nondeterminism).
+The graph replays (one graph over all engine streams, one per engine, or fused launches
+whose workgroups are split by scene) must give each engine the same result as stepping
+it alone with ckmpm_substep: same block lists and orderings (integer, bit-exact), same
+particle state up to the order of the floating-point atomics (the only nondeterminism).
 """
 import numpy as np
 import pytest
@@ -38,7 +39,7 @@ def test_batch_scenes_are_seeded_and_within_spec():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("substeps", [7, 10])
-@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("mode", [2, 3, 4])
 def test_graph_batch_matches_separate_engines(substeps, mode):
     from ckmpm_amd.engine import MPMBatch, engine_from_scene
 
@@ -52,7 +53,7 @@ def test_graph_batch_matches_separate_engines(substeps, mode):
     batch = MPMBatch(together)
     batch.Run(substeps, mode)
     batch.Synchronize()
-    per_capture = 1 if mode == MPMBatch.GRAPH else len(scs)
+    per_capture = len(scs) if mode == MPMBatch.ENGINE_GRAPHS else 1
     assert batch.captures == per_capture
     for a, b, sc in zip(alone, together, scs):
         ka, ca = a.layout(0)
@@ -79,13 +80,13 @@ def test_sequential_and_stream_modes_match_graph():
 
     scs = _small_batch()
     res = {}
-    for mode in (MPMBatch.SEQUENTIAL, MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS):
+    for mode in (MPMBatch.SEQUENTIAL, MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS, MPMBatch.FUSED):
         engs = [engine_from_scene(sc) for sc in scs]
         b = MPMBatch(engs)
         b.Run(6, mode)
         b.Synchronize()
         res[mode] = [e.state(0) for e in engs]
         b.close()
-    for m in (MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS):
+    for m in (MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS, MPMBatch.FUSED):
         for x, y in zip(res[MPMBatch.SEQUENTIAL], res[m]):
             np.testing.assert_allclose(y, x, rtol=0, atol=1e-5)
