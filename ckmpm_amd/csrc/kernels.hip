@@ -1667,26 +1667,36 @@ __device__ __forceinline__ uint32_t FusedScene(const uint32_t* __restrict__ begi
     return i;
 }
 
+// The scene's argument struct, read through the constant address space: the table is
+// never written while a launch runs, so every field load can be a scalar load (through
+// a generic pointer, the loads after the kernel's first stores become vector loads; the
+// fused G2P2G spilled 4-10 VGPRs that way, none this way).
+template <class T>
+__device__ __forceinline__ const T& FusedArgs(const T* tab, uint32_t i) {
+    using CT = const __attribute__((address_space(4))) T;
+    return *(const T*)((CT*)tab + i);
+}
+
 template <bool kCustom>
 __global__ __launch_bounds__(kThreads) void GridUpdateFused(const GridUpdateArgs* __restrict__ tab,
                                                             const uint32_t* __restrict__ begin, uint32_t n) {
     uint32_t wg, ng;
     const uint32_t i = FusedScene(begin, n, wg, ng);
-    GridUpdateBody<kCustom>(tab[i], wg, ng);
+    GridUpdateBody<kCustom>(FusedArgs(tab, i), wg, ng);
 }
 template <int KIND, int THREADS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GFused(
     const G2P2GArgs* __restrict__ tab, const uint32_t* __restrict__ begin, uint32_t n) {
     uint32_t wg, ng;
     const uint32_t i = FusedScene(begin, n, wg, ng);
-    G2P2GBody<KIND, THREADS>(tab[i], wg, ng);
+    G2P2GBody<KIND, THREADS>(FusedArgs(tab, i), wg, ng);
 }
 #define CKMPM_FUSED(NAME, BODY, ARGS, THREADS)                                                                  \
     __global__ __launch_bounds__(THREADS) void NAME(const ARGS* __restrict__ tab, const uint32_t* __restrict__ begin, \
                                                     uint32_t n) {                                                 \
         uint32_t wg, ng;                                                                                        \
         const uint32_t i = FusedScene(begin, n, wg, ng);                                                        \
-        BODY(tab[i], wg, ng);                                                                                   \
+        BODY(FusedArgs(tab, i), wg, ng);                                                                        \
     }
 CKMPM_FUSED(RebinCountFused, RebinCountBody, RebinCountArgs, kCountThreads)
 CKMPM_FUSED(RebinScanFused, RebinScanBody, RebinScanArgs, kThreads)

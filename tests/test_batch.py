@@ -90,3 +90,60 @@ def test_sequential_and_stream_modes_match_graph():
     for m in (MPMBatch.STREAMS, MPMBatch.GRAPH, MPMBatch.ENGINE_GRAPHS, MPMBatch.FUSED):
         for x, y in zip(res[MPMBatch.SEQUENTIAL], res[m]):
             np.testing.assert_allclose(y, x, rtol=0, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_fused_batch_variants_match_separate_engines():
+    """The fused launches' variants: a custom boundary (second grid-update instantiation),
+    two models of one engine in one G2P2G launch (two fixed-corotated spheres, with the
+    corner box's model: three entries), two kinds of one engine (fluid + sand) and
+    512-thread G2P2G workgroups (the fluid)."""
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+    from tests.test_boundary import _corner_scene, _regions
+
+    def scenes_():
+        corner = _corner_scene(None)
+        corner.boundary = scenes.Boundary(scenes.WALLS_SLIP, _regions(corner.dx))
+        return [corner, scenes.colliding_spheres(), scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8)]
+
+    def engines():
+        engs = [engine_from_scene(sc) for sc in scenes_()]
+        engs[2].set_tuning(0, 512)
+        return engs
+
+    substeps = 9
+    print(f"fused variants: corner (custom boundary), spheres (2 FC models), dam+sand; {substeps} substeps", flush=True)
+    alone = engines()
+    for e, sc in zip(alone, scenes_()):
+        for k in range(substeps):
+            e.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        e.Synchronize()
+    together = engines()
+    batch = MPMBatch(together)
+    batch.Run(substeps, MPMBatch.FUSED)
+    batch.Synchronize()
+    assert batch.captures == 1
+    for a, b in zip(alone, together):
+        ka, ca = a.layout(0)
+        kb, cb = b.layout(0)
+        assert ca == cb
+        np.testing.assert_array_equal(ka, kb)
+        for m in range(len(a.models)):
+            pa, oa = a.ordering(m)
+            pb, ob = b.ordering(m)
+            np.testing.assert_array_equal(pa, pb)
+            np.testing.assert_array_equal(oa, ob)
+            np.testing.assert_allclose(b.state(m), a.state(m), rtol=0, atol=1e-5)
+        assert b.counters().flags == 0
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_batch_rejects_an_engine_twice():
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+    from ckmpm_amd.native import CkmpmError
+
+    e = engine_from_scene(scenes.colliding_spheres())
+    with pytest.raises(CkmpmError, match="ckmpm_batch_create"):
+        MPMBatch([e, e])
+    e.close()
