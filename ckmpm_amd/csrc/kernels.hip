@@ -1223,9 +1223,16 @@ __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint3
         __syncthreads();
     }
 }
+// The body reads the arguments through the kernel-argument segment pointer (the explicit
+// argument a sits at offset 0), hidden from the optimizer by an empty asm: the fields are
+// then reloaded with scalar loads where they are needed instead of being held in SGPRs.
+// Read from a directly, the plastic kinds spilled 13 SGPRs and 2-4 VGPRs; this way none.
 template <int KIND, int THREADS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void G2P2GKernel(G2P2GArgs a) {
-    G2P2GBody<KIND, THREADS>(a, blockIdx.x, gridDim.x);
+    using CT = const __attribute__((address_space(4))) G2P2GArgs;
+    CT* p = (CT*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    G2P2GBody<KIND, THREADS>(*(const G2P2GArgs*)p, blockIdx.x, gridDim.x);
 }
 
 // ============================================================== rebin
