@@ -352,7 +352,7 @@ class MPMBatch:
             msg = self._lib.ckmpm_batch_last_error(self._h)
             raise CkmpmError(rc, where, msg.decode() if msg else "")
 
-    def Run(self, substeps: int, mode: int = 2):
+    def Run(self, substeps: int, mode: int = 4):
         self._check(self._lib.ckmpm_batch_run(self._h, int(substeps), int(mode)), "ckmpm_batch_run")
 
     def Synchronize(self):

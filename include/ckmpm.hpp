@@ -305,7 +305,8 @@ class Engine {
  * the batch. */
 class Batch {
    public:
-    enum Mode { kSequential = 0, kStreams = 1, kSingleGraph = 2, kPerSceneGraphs = 3 };
+    /* kFused: each kernel of a substep is one launch for all engines (the fastest) */
+    enum Mode { kSequential = 0, kStreams = 1, kSingleGraph = 2, kPerSceneGraphs = 3, kFused = 4 };
     explicit Batch(const std::vector<Engine*>& engines) {
         std::vector<ckmpm_engine*> h;
         for (Engine* e : engines) h.push_back(e->handle());
@@ -317,7 +318,7 @@ class Batch {
     }
     Batch(const Batch&) = delete;
     Batch& operator=(const Batch&) = delete;
-    void Run(uint32_t substeps, Mode mode = kPerSceneGraphs) {
+    void Run(uint32_t substeps, Mode mode = kFused) {
         detail::Check(ckmpm_batch_run(b_, substeps, mode), "ckmpm_batch_run", ckmpm_batch_last_error(b_));
     }
     void Synchronize() {
