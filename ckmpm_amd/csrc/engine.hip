@@ -800,6 +800,9 @@ struct ckmpm_batch {
     // mode 4: one graph per parity of fused launches (each kernel of a substep once for
     // all scenes)
     BatchGraph fz[2];
+    // fused mode: streams and events that run a substep's G2P2G launches side by side
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> aux_done;
     // mode 3: one graph per engine and parity, replayed on the engine's own stream
     std::vector<BatchGraph> pe[2];
     uint64_t captures = 0;
@@ -909,7 +912,8 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
 // Two substeps of every engine as fused launches: each (kernel, variant) of a substep
 // runs once for all scenes, its workgroups split by scene (LaunchFused), from argument
 // tables built here on the host and copied once.  The stages run in order on the capture
-// stream, so every engine's launches keep their order; the scenes' buffers are disjoint.
+// stream, so every engine's launches keep their order (the G2P2G launches of one stage are
+// independent and run side by side); the scenes' buffers are disjoint.
 int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
     ReleaseGraph(g);
     const size_t ne = b->eng.size();
@@ -969,10 +973,51 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
     BATCH_HIP(b, hipMalloc(&g.tables, img.size()));
     BATCH_HIP(b, hipMemcpy(g.tables, img.data(), img.size(), hipMemcpyHostToDevice));
     uint8_t* dev = static_cast<uint8_t*>(g.tables);
+    // the G2P2G launches of a substep (one per kind and workgroup size) are independent:
+    // they run side by side on auxiliary streams, so their tails overlap
+    size_t most = 0;
+    for (size_t j = 0; j < launches.size();) {
+        size_t k = j;
+        while (k < launches.size() && launches[k].kernel == kFkG2P2G) ++k;
+        most = std::max(most, k - j);
+        j = k > j ? k : j + 1;
+    }
+    while (b->aux.size() + 1 < most) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        BATCH_HIP(b, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->aux.push_back(st);
+        BATCH_HIP(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->aux_done.push_back(ev);
+    }
     BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
-    for (const Launch& l : launches)
-        LaunchFused(l.kernel, l.variant, dev + l.args, reinterpret_cast<const uint32_t*>(dev + l.begin), l.n, l.total,
-                    b->cs);
+    for (size_t j = 0; j < launches.size();) {
+        size_t k = j;
+        while (k < launches.size() && launches[k].kernel == kFkG2P2G) ++k;
+        if (k - j < 2) {
+            const Launch& l = launches[j];
+            LaunchFused(l.kernel, l.variant, dev + l.args, reinterpret_cast<const uint32_t*>(dev + l.begin), l.n,
+                        l.total, b->cs);
+            ++j;
+            continue;
+        }
+        (void)hipEventRecord(b->fork, b->cs);
+        for (size_t q = j; q < k; ++q) {
+            const Launch& l = launches[q];
+            hipStream_t st = b->cs;
+            if (q > j) {
+                st = b->aux[q - j - 1];
+                (void)hipStreamWaitEvent(st, b->fork, 0);
+            }
+            LaunchFused(l.kernel, l.variant, dev + l.args, reinterpret_cast<const uint32_t*>(dev + l.begin), l.n,
+                        l.total, st);
+        }
+        for (size_t q = j + 1; q < k; ++q) {
+            (void)hipEventRecord(b->aux_done[q - j - 1], b->aux[q - j - 1]);
+            (void)hipStreamWaitEvent(b->cs, b->aux_done[q - j - 1], 0);
+        }
+        j = k;
+    }
     const hipError_t rc = hipStreamEndCapture(b->cs, &g.graph);
     if (rc != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(rc));
     BATCH_HIP(b, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
@@ -1123,6 +1168,8 @@ int ckmpm_batch_destroy(ckmpm_batch* b) {
         for (BatchGraph& g : b->pe[p]) ReleaseGraph(g);
     for (hipEvent_t ev : b->join)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : b->aux_done) (void)hipEventDestroy(ev);
+    for (hipStream_t st : b->aux) (void)hipStreamDestroy(st);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->cs) (void)hipStreamDestroy(b->cs);
