@@ -912,8 +912,9 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
 // Two substeps of every engine as fused launches: each (kernel, variant) of a substep
 // runs once for all scenes, its workgroups split by scene (LaunchFused), from argument
 // tables built here on the host and copied once.  The stages run in order on the capture
-// stream, so every engine's launches keep their order (the G2P2G launches of one stage are
-// independent and run side by side); the scenes' buffers are disjoint.
+// stream, so every engine's launches keep their order (independent launches run side by
+// side: the G2P2G launches of a substep, and base + scatter beside the extension passes);
+// the scenes' buffers are disjoint.
 int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
     ReleaseGraph(g);
     const size_t ne = b->eng.size();
@@ -935,7 +936,7 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
     for (ckmpm_engine* e : b->eng) e->substeps -= 2;  // recorded, not run
     // host image of the tables: per launch, the argument structs then the workgroup starts
     struct Launch {
-        int kernel, variant;
+        int stage, kernel, variant;
         size_t args, begin;
         uint32_t n, total;
     };
@@ -964,7 +965,8 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
             begin.back() = static_cast<uint32_t>(tot);
             if (p.args.size() != FusedArgsSize(p.kernel) * p.grids.size())
                 return BatchFail(b, CKMPM_ERR_INVALID, "fused launch: argument table size");
-            Launch l{p.kernel, p.variant, 0, 0, static_cast<uint32_t>(p.grids.size()), static_cast<uint32_t>(tot)};
+            Launch l{p.stage, p.kernel, p.variant, 0, 0, static_cast<uint32_t>(p.grids.size()),
+                     static_cast<uint32_t>(tot)};
             l.args = place(p.args.data(), p.args.size());
             l.begin = place(begin.data(), begin.size() * sizeof(uint32_t));
             launches.push_back(l);
@@ -982,7 +984,7 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
         most = std::max(most, k - j);
         j = k > j ? k : j + 1;
     }
-    while (b->aux.size() + 1 < most) {
+    while (b->aux.size() + 1 < std::max<size_t>(most, 2)) {
         hipStream_t st = nullptr;
         hipEvent_t ev = nullptr;
         BATCH_HIP(b, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -990,8 +992,25 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
         BATCH_HIP(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         b->aux_done.push_back(ev);
     }
+    auto launch = [&](const Launch& l, hipStream_t st) {
+        LaunchFused(l.kernel, l.variant, dev + l.args, reinterpret_cast<const uint32_t*>(dev + l.begin), l.n, l.total,
+                    st);
+    };
     BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
     for (size_t j = 0; j < launches.size();) {
+        if (launches[j].stage == kStBase) {
+            // after the scan, two independent chains: base + scatter (particle permutation)
+            // on the capture stream, the layout extension's passes on an auxiliary stream
+            size_t k = j;
+            while (k < launches.size() && launches[k].stage >= kStBase && launches[k].stage <= kStEmit1) ++k;
+            (void)hipEventRecord(b->fork, b->cs);
+            (void)hipStreamWaitEvent(b->aux[0], b->fork, 0);
+            for (size_t q = j; q < k; ++q) launch(launches[q], launches[q].stage >= kStMark0 ? b->aux[0] : b->cs);
+            (void)hipEventRecord(b->aux_done[0], b->aux[0]);
+            (void)hipStreamWaitEvent(b->cs, b->aux_done[0], 0);
+            j = k;
+            continue;
+        }
         size_t k = j;
         while (k < launches.size() && launches[k].kernel == kFkG2P2G) ++k;
         if (k - j < 2) {
