@@ -164,15 +164,19 @@ def bench_adjoint(args, torch):
     N = m.x.shape[0]
     logE = torch.tensor(np.log(m.mu_p.astype(np.float64) * 2 * 1.3), requires_grad=True)
     v0 = torch.zeros((N, 3), dtype=torch.float64, requires_grad=True)
-    # forward alone: plain substeps on an engine without a tape
-    eng = engine_from_scene(sc)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
-    eng.Synchronize()
-    t_fwd = time.perf_counter() - t0
-    eng.close()
+    # forward alone: plain substeps on an engine without a tape; the median over three
+    # fresh engines (one 300-substep run is ~23 ms, so a single host hiccup shows)
+    fwd_samples = []
+    for _ in range(3):
+        eng = engine_from_scene(sc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        eng.Synchronize()
+        fwd_samples.append(time.perf_counter() - t0)
+        eng.close()
+    t_fwd = float(np.median(fwd_samples))
     target = torch.tensor([0.35, 0.5, 0.5], dtype=torch.float64)
 
     def fwd_bwd(segment):
@@ -194,11 +198,15 @@ def bench_adjoint(args, torch):
     # reuses that engine rewound to its setup state (as every iteration of a gradient
     # loop after the first does).  Full tape (every substep recorded in HBM, no replay),
     # then checkpoints every 25.
+    # The timed figure of each mode is the median of three calls after the first.
     t_tape_first, _ = fwd_bwd(None)
-    t_tape, _ = fwd_bwd(None)
+    tape_samples = [fwd_bwd(None)[0] for _ in range(3)]
+    t_tape = float(np.median(tape_samples))
     g_tape = (logE.grad.clone(), v0.grad.clone())
     t_first, _ = fwd_bwd(25)
-    t_all, loss = fwd_bwd(25)
+    ck = [fwd_bwd(25) for _ in range(3)]
+    ck_samples = [t for t, _ in ck]
+    t_all, loss = float(np.median(ck_samples)), ck[-1][1]
     tape_vs_ck = max(float((g_tape[0] - logE.grad).norm() / logE.grad.norm()),
                      float((g_tape[1] - v0.grad).norm() / v0.grad.norm()))
     return {
@@ -216,12 +224,16 @@ def bench_adjoint(args, torch):
         "data": "synthetic (seeded splitmix64 scene generator)",
         "config": {"workload": "cantilever", "particles": N, "grid": "128x128x128 cells", "dt": sc.dt,
                    "substeps": steps, "checkpoint_every": 25, "material": "fixed-corotated"},
-        "fwd": {"value": N * steps / t_fwd, "ms_per_step": 1e3 * t_fwd / steps},
+        "fwd": {"value": N * steps / t_fwd, "ms_per_step": 1e3 * t_fwd / steps,
+                "samples_ms_per_step": [1e3 * t / steps for t in fwd_samples]},
         "fwd_bwd_over_fwd": t_all / t_fwd,
+        "samples_ms_per_step": [1e3 * t / steps for t in ck_samples],
+        "timing": "median of three calls after a first call (forward: of three fresh engines)",
         "first_call": {"ms_per_step": 1e3 * t_first / steps, "fwd_bwd_over_fwd": t_first / t_fwd,
                        "note": "includes building and setting up the engine"},
         "full_tape": {"value": N * steps / t_tape, "ms_per_step": 1e3 * t_tape / steps,
                       "fwd_bwd_over_fwd": t_tape / t_fwd, "grad_rel_diff_vs_checkpointed": tape_vs_ck,
+                      "samples_ms_per_step": [1e3 * t / steps for t in tape_samples],
                       "first_call_ms_per_step": 1e3 * t_tape_first / steps},
         "loss": float(loss.detach()),
         "grad_norms": {"logE": float(logE.grad.norm()), "v0": float(v0.grad.norm())},
