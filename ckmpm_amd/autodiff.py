@@ -111,7 +111,12 @@ class _Simulate(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gx):
+        if ctx.eng is None:
+            # the first backward released the checkpoints and handed the engine on
+            raise RuntimeError("simulate(): backward through the same call a second time is not supported "
+                               "(retain_graph / repeated autograd.grad); call simulate() again")
         eng, cks, sc, steps, seg = ctx.eng, ctx.cks, ctx.sc, ctx.steps, ctx.segment
+        ctx.eng, ctx.cks = None, None
         eng.adjoint_reset()
         eng.adjoint_seed(0, gx.detach().cpu().numpy().astype(np.float32))
         tail = 0 if seg is None else (steps - 1) // seg * seg

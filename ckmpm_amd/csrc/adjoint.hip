@@ -1037,6 +1037,10 @@ int ckmpm_checkpoint_save(ckmpm_engine* e, ckmpm_checkpoint** out) {
 
 int ckmpm_checkpoint_restore(ckmpm_engine* e, const ckmpm_checkpoint* c) {
     if (!e || !c || !e->adj) return CKMPM_ERR_INVALID;
+    // a checkpoint holds its own engine's sizes and memory: another engine's, or one whose
+    // engine is gone, would copy the wrong extents
+    if (c->eng != e || !c->mem || !e->adj->live.count(const_cast<ckmpm_checkpoint*>(c)))
+        return Fail(e, CKMPM_ERR_INVALID, "checkpoint belongs to another engine or was released");
     return Put(e, c->snap);
 }
 

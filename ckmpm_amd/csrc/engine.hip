@@ -266,13 +266,14 @@ extern "C" {
 int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckmpm_engine** out) {
     if (!domain || !sim || !out) return CKMPM_ERR_INVALID;
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CKMPM_ERR_NO_DEVICE;
+    // descriptors first (no device needed to reject them)
     for (int a = 0; a < 3; ++a)
         if (domain->dim[a] <= 0 || domain->dim[a] > 1024) return CKMPM_ERR_INVALID;
     if (!(sim->dx > 0.0f) || sim->block_capacity == 0) return CKMPM_ERR_INVALID;
     const uint64_t nblocks = static_cast<uint64_t>(domain->dim[0]) * domain->dim[1] * domain->dim[2];
     if (nblocks > (1ull << 26)) return CKMPM_ERR_INVALID;  // block key << 6 | cell fits 32 bits
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CKMPM_ERR_NO_DEVICE;
     ckmpm_engine* e = new ckmpm_engine();
     for (int a = 0; a < 3; ++a) {
         e->dom.dim[a] = domain->dim[a];
@@ -520,7 +521,9 @@ int ckmpm_set_boundary(ckmpm_engine* e, const ckmpm_boundary_desc* b) {
 // Simulate's frame loop (mpm_engine.cuh:514-876) with its dt rules.
 int ckmpm_run_frames(ckmpm_engine* e, uint32_t frames) {
     CK_TRY(CheckReady(e));
-    const double spf = e->sim.fps ? 1.0 / e->sim.fps : std::numeric_limits<double>::infinity();
+    // a frame lasts 1 / fps: without a frame rate the loop below would never end
+    if (e->sim.fps == 0) return Fail(e, CKMPM_ERR_INVALID, "ckmpm_run_frames needs sim.fps > 0");
+    const double spf = 1.0 / e->sim.fps;
     for (uint32_t f = 0; f < frames; ++f) {
         const double frame_end = e->time + spf;
         while (e->time < frame_end) {

@@ -89,7 +89,11 @@ __device__ __forceinline__ void StressNeoHookean(float lambda, float mu, const f
 
 // Drucker-Prager StVK-Hencky return map, mpm_material.cuh:360-464 (cohesion 0, beta 1,
 // volume correction).  Projects E = F - I in place and updates logJp.  In the elastic
-// branch the reference rebuilds F = U diag(s) V^T, which equals F; E is kept as is.
+// branch (deltaGamma <= 0) the reference rebuilds F = U diag(exp eps) V^T with
+// eps = log max(|s|, 1e-4) (:385,417-443): that is F itself only when every singular
+// value is at least 1e-4 (s_2 > 0); otherwise the rebuild clamps |s| at 1e-4 and
+// un-inverts an inverted F, so E is rebuilt then and kept as is only in the identity
+// case.  With mu == 0 and eps_tr < 0 no branch runs and F stays as it is (:412).
 __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, float sin_phi, float (&E)[9], float& logJp,
                                                     float (&tau)[9]) {
     const float alpha = 0.816496580927726f * 2.0f * sin_phi / (3.0f - sin_phi);  // sqrt(2/3) 2 sin / (3 - sin)
@@ -107,7 +111,7 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
     for (int a = 0; a < 3; ++a) eh[a] = eps[a] - eps_tr * (1.0f / 3.0f);
     const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
     float H[3] = {eps[0], eps[1], eps[2]};
-    bool rebuild = true;
+    bool rebuild = true, keep = false;  // keep: mu == 0 branch, F unchanged
     if (eps_tr >= 0.0f) {
         H[0] = H[1] = H[2] = 0.0f;
         logJp = eps_sum + logJp;
@@ -119,10 +123,12 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
 #pragma unroll
             for (int a = 0; a < 3; ++a) H[a] = eps[a] - r * eh[a];
         } else {
-            rebuild = false;
+            // H = eps: the rebuild is an identity unless a singular value is clamped
+            rebuild = !(sv.s[0] >= 1e-4f && sv.s[1] >= 1e-4f && sv.s[2] >= 1e-4f);
         }
     } else {
         rebuild = false;
+        keep = true;
     }
     const float htr = H[0] + H[1] + H[2];
     float k[3], ns[3];
@@ -130,6 +136,8 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
     for (int a = 0; a < 3; ++a) {
         ns[a] = expf(H[a]);
         k[a] = smu * H[a] + lambda * htr;  // (2 mu log ns + lambda tr log ns) / ns * ns
+        // F kept with singular values s: tau = U diag(phat s) U^T
+        if (keep) k[a] *= sv.s[a] / ns[a];
     }
     if (rebuild) {
         // E = U diag(ns) V^T - I

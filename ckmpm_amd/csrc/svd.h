@@ -183,8 +183,8 @@ __device__ __forceinline__ void SignedSvd3(const float (&A)[9], Svd3& out) {
 // Signed SVD of F = I + E that also returns d_i = s_i - 1 to full relative precision.
 // The Jacobi stage diagonalises M = F^T F - I = E + E^T + E^T E, whose entries are of
 // strain size, so its eigenvalues m_i (s_i^2 = 1 + m_i) keep the strain's relative
-// precision; d_i = m_i / (1 + sqrt(1 + m_i)) for a positive s_i and -(1 + sqrt(1 + m_i))
-// for the negative s_2 of an inverted F.  U comes from the Givens QR of F V as above.
+// precision; d_i = m_i / (1 + sqrt(1 + m_i)) for s_i >= 1/2, and s_i = R_ii (the Givens QR
+// of F V, which also gives U as above) for a smaller or negative s_i.
 __device__ __forceinline__ void SignedSvdStrain(const float (&E)[9], Svd3& out, float (&d)[3]) {
     using namespace svd_detail;
     float S[9];
@@ -220,9 +220,15 @@ __device__ __forceinline__ void SignedSvdStrain(const float (&E)[9], Svd3& out, 
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
+        // near 1, s_i = 1 + m_i / (1 + sqrt(1 + m_i)) keeps the relative precision of a
+        // small strain; below 1/2 (a compressed or inverted F) the diagonal of R is the
+        // accurate value (1 + m_i carries an absolute error ~1e-7, which is the whole of
+        // a singular value near 1e-5)
         const float root = sqrtf(fmaxf(1.0f + m[i], 0.0f));
-        d[i] = B[i * 4] >= 0.0f ? m[i] / (1.0f + root) : -(1.0f + root);
-        out.s[i] = 1.0f + d[i];
+        const float r = B[i * 4];
+        const bool direct = r < 0.5f;
+        d[i] = direct ? r - 1.0f : m[i] / (1.0f + root);
+        out.s[i] = direct ? r : 1.0f + d[i];
     }
 }
 

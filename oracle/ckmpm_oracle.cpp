@@ -308,6 +308,7 @@ void StressDruckerPrager(double lambda, double mu, double sin_phi, double* F, do
     for (int a = 0; a < 3; ++a) eh[a] = eps[a] - eps_tr / 3.0;
     const double ehn = std::sqrt(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
     double ns[3];
+    bool rebuild = true;
     if (eps_tr >= 0.0) {
         ns[0] = ns[1] = ns[2] = 1.0;
         logJp = eps_sum + logJp;
@@ -318,16 +319,22 @@ void StressDruckerPrager(double lambda, double mu, double sin_phi, double* F, do
             const double dg = ehn + (3.0 * lambda + smu) / smu * eps_tr * alpha;
             if (dg > 0.0)
                 for (int a = 0; a < 3; ++a) H[a] = eps[a] - (dg / ehn) * eh[a];
+        } else {
+            // :412: with mu == 0 neither branch runs and F is left as it is (the
+            // reference's newS is then uninitialised; exp(eps) stands in for it here)
+            rebuild = false;
         }
         for (int a = 0; a < 3; ++a) ns[a] = std::exp(H[a]);
     }
-    // F = U diag(ns) V^T
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double v = 0;
-            for (int k = 0; k < 3; ++k) v += U[i * 3 + k] * ns[k] * V[j * 3 + k];
-            F[i * 3 + j] = v;
-        }
+    // F = U diag(ns) V^T: in the elastic branch (:417-443) too, where it clamps |s| at
+    // 1e-4 and un-inverts an inverted F
+    if (rebuild)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double v = 0;
+                for (int k = 0; k < 3; ++k) v += U[i * 3 + k] * ns[k] * V[j * 3 + k];
+                F[i * 3 + j] = v;
+            }
     // tau = U diag(phat) V^T F^T with phat = (2 mu log ns + lambda tr log ns) / ns
     double lg[3] = {std::log(ns[0]), std::log(ns[1]), std::log(ns[2])};
     const double ltr = lg[0] + lg[1] + lg[2];

@@ -37,3 +37,19 @@ def test_create_without_device_fails_cleanly():
     rc = lib.ckmpm_create(ctypes.byref(d), ctypes.byref(s), ctypes.byref(h))
     assert rc in (native.ERR_NO_DEVICE, native.ERR_HIP)
     assert not h
+
+
+def test_invalid_descriptors_are_refused_before_any_device_call():
+    """Domain and simulation descriptors are checked first: an empty or oversized
+    domain, dx <= 0 or a zero block capacity give CKMPM_ERR_INVALID with or without a GPU."""
+    lib = native.load()
+    good_d, good_s = ((16, 16, 16), (0, 0, 0)), (1 / 64, 0, 0, 0.5, 1e-4, 48, 4096)
+    cases = [(((0, 16, 16), (0, 0, 0)), good_s), (((16, 2048, 16), (0, 0, 0)), good_s),
+             (((1024, 1024, 128), (0, 0, 0)), good_s),
+             (good_d, (0.0, 0, 0, 0.5, 1e-4, 48, 4096)), (good_d, (1 / 64, 0, 0, 0.5, 1e-4, 48, 0))]
+    for (dim, off), sim in cases:
+        d = native.DomainDesc((ctypes.c_int32 * 3)(*dim), (ctypes.c_int32 * 3)(*off))
+        s = native.SimDesc(*sim)
+        h = ctypes.c_void_p()
+        assert lib.ckmpm_create(ctypes.byref(d), ctypes.byref(s), ctypes.byref(h)) == native.ERR_INVALID, (dim, sim)
+        assert not h

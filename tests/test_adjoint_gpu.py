@@ -162,3 +162,34 @@ def test_checkpointed_gradients_match_the_full_tape(segment):
     np.testing.assert_allclose(ck[0], full[0], rtol=0, atol=1e-5 * sc.dx)
     assert np.abs(full[1]).max() > 0 and np.abs(full[2]).max() > 0
     assert _rel(ck[1], full[1]) < 1e-5 and _rel(ck[2], full[2]) < 1e-5
+
+
+@pytest.mark.gpu
+def test_second_backward_and_foreign_checkpoint_are_refused():
+    """A second backward through one simulate() call raises instead of replaying freed
+    checkpoints; restoring a checkpoint into another engine returns CKMPM_ERR_INVALID."""
+    from ckmpm_amd.autodiff import clear_engine_cache, simulate
+    from ckmpm_amd.engine import engine_from_scene
+    from ckmpm_amd.native import ERR_INVALID, CkmpmError
+
+    sc, logE0 = _case()
+    logE = torch.tensor(logE0, requires_grad=True)
+    v0 = torch.tensor(sc.models[0].v.astype(np.float64), requires_grad=True)
+    x = simulate(logE, v0, sc, 6, 3)
+    L = (x.mean(0) ** 2).sum()
+    L.backward(retain_graph=True)
+    g = logE.grad.clone()
+    with pytest.raises(RuntimeError, match="second time"):
+        L.backward()
+    assert torch.isfinite(g).all()
+    clear_engine_cache()
+    a = engine_from_scene(sc, adjoint_segment=4)
+    b = engine_from_scene(sc, adjoint_segment=4)
+    ck = a.checkpoint()
+    with pytest.raises(CkmpmError) as ei:
+        b.restore(ck)
+    assert ei.value.code == ERR_INVALID
+    a.restore(ck)
+    a.free_checkpoint(ck)
+    a.close()
+    b.close()

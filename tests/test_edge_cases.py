@@ -71,3 +71,26 @@ def test_capacity_overflow_is_reported():
     c = eng.counters()
     assert c.exterior_blocks <= Ex + 8 and c.neighbor_blocks <= Ex + 8
     eng.close()
+
+
+def test_frame_loop_without_frame_rate_is_refused():
+    """ckmpm_run_frames with sim.fps = 0 returns CKMPM_ERR_INVALID at once (a frame of
+    1 / fps would never end) and leaves the engine usable."""
+    from ckmpm_amd.engine import MPMConfig, MPMDomain, MPMEngine, MPMMaterial, MPMModel
+    from ckmpm_amd.native import ERR_INVALID, CkmpmError
+
+    sc = _one_particle(np.array([0.1, 0.0, 0.0]))
+    m = sc.models[0]
+    eng = MPMEngine()
+    eng.Initialize(MPMDomain(sc.dim, sc.offset), MPMConfig(dx=sc.dx, gravity=sc.gravity, dt=sc.dt, fps=0,
+                                                           max_active_block_count=sc.block_capacity()))
+    mat = MPMMaterial(m.material.kind, m.material.lam, m.material.mu)
+    eng.InitializeParticle([MPMModel(m.mass, m.volume, m.x, m.v, mat)])
+    eng.InitialSetup()
+    with pytest.raises(CkmpmError) as ei:
+        eng.Simulate(1)
+    assert ei.value.code == ERR_INVALID
+    eng.Substep(0.0, sc.dt)
+    eng.Synchronize()
+    assert eng.counters().substeps == 1
+    eng.close()

@@ -171,6 +171,59 @@ def test_material_box_lockstep(kind):
     _lockstep(sc, 6, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
 
 
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def test_drucker_prager_inverted_and_near_singular_lockstep():
+    """Sand whose state holds inverted particles (F = R1 diag(s0, s1, -s2) R2^T) and
+    near-singular ones (a singular value of 1e-5 or -3e-5), imported into the engine and
+    the oracle alike.  The first substep (dt = 0, so the stress sees F as imported) takes
+    them through the elastic branch of the return map, which rebuilds F from the clamped
+    |s| (mpm_material.cuh:385,417-443): un-inverted, smallest singular value 1e-4.  The
+    stored F, the grid and the orderings are compared with the oracle for that substep
+    and the next."""
+    sc = _box_scene(scenes.DRUCKER_PRAGER, seed=41, speed=0.0)
+    eng = engine_from_scene(sc)
+    orc = Oracle.from_scene(sc)
+    orc.setup()
+    st = eng.state(0)
+    n = st.shape[1]
+    rng = np.random.default_rng(5)
+    picks = rng.choice(n, 96, replace=False)
+    sv = [np.array([0.998, 0.995, -0.99]), np.array([1.0, 0.99, 1e-5]), np.array([1.0, 0.99, -3e-5])]
+    for i, p in enumerate(picks):
+        F = _rotation(rng) @ np.diag(sv[i % 3]) @ _rotation(rng).T
+        st[3:12, p] = (F - np.eye(3)).reshape(9).astype(np.float32)
+    eng.set_state(0, st)
+    nm = 1
+    for k in range(2):
+        dt = 0.0 if k == 0 else sc.dt
+        _import(eng, orc, nm)
+        orc.physics(dt, sc.dt)
+        eng.Substep(dt, sc.dt)
+        eng.Synchronize()
+        _grid_close(eng.grid_m(), orc.grid_mnext())
+        sg, so = eng.global_state(0), orc.state_next(0)
+        np.testing.assert_allclose(sg[:3], so[:3], rtol=0, atol=2e-7)
+        np.testing.assert_allclose(sg[3:], so[3:], rtol=0, atol=2e-5)
+        if k == 0:
+            # every imported particle left the substep un-inverted with |s| >= 1e-4
+            perm, _ = eng.ordering(0, 1)  # the ordering the stored state was written in
+            where = np.empty(n, np.int64)
+            where[perm & SLOT_MASK] = np.arange(n)
+            F = sg[3:12, where[picks]].T.reshape(-1, 3, 3)
+            s = np.linalg.svd(F, compute_uv=False)
+            assert (np.linalg.det(F) > 0).all()
+            np.testing.assert_allclose(s[:, 2], np.tile([0.99, 1e-4, 1e-4], 32), rtol=2e-3, atol=2e-6)
+        orc.rebin([eng.state(0)[:3].T.copy()])
+        _compare_layout(eng, orc, 0)
+        _compare_ordering(eng, orc, nm)
+    assert eng.counters().flags == 0
+
+
 def test_prestressed_fluid_global_deposits_then_fixed_point():
     """A compressed fluid (J0 = 0.95) at rest.  The first substep's deposits are pressure
     impulses while no deposit bound has been recorded and the grid is at rest, so the

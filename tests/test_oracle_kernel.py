@@ -214,6 +214,42 @@ def test_drucker_prager_branches():
     assert abs(lj - np.log(sig).sum()) < 1e-12
 
 
+def test_drucker_prager_elastic_branch_clamps_and_uninverts():
+    """Elastic branch (deltaGamma <= 0) of mpm_material.cuh:417-443: F is rebuilt as
+    U diag(exp eps) V^T with eps = log max(|s|, 1e-4) (:385).  Built here from the
+    rotations the test constructs F with: an inverted F = R1 diag(s0, s1, -s2) R2^T comes
+    back as R1 diag(s0, s1, s2) R2^T, and a singular value below 1e-4 comes back as 1e-4.
+    The stress is R1 diag(2 mu eps + lambda tr eps) R1^T on the rebuilt F.  With mu = 0
+    no branch runs (:412) and F is returned unchanged."""
+    lam, mu = 201923.0, 134615.0
+    params = [lam, mu, 0, 0, 0, 1, 30]
+    R1, R2 = _rot(3), _rot(4)
+    cases = [(np.array([0.998, 0.995, -0.99]), np.array([0.998, 0.995, 0.99])),
+             (np.array([1.0, 0.99, 1e-5]), np.array([1.0, 0.99, 1e-4])),
+             (np.array([1.0, 0.99, -3e-5]), np.array([1.0, 0.99, 1e-4]))]
+    for s, ns in cases:
+        F = R1 @ np.diag(s) @ R2.T
+        Fo, lj, tau = pyoracle.stress(DP, params, F, logJp=-0.5)
+        assert lj == 0.0   # compressive branch with mu != 0 resets logJp
+        np.testing.assert_allclose(Fo, R1 @ np.diag(ns) @ R2.T, atol=1e-12)
+        assert np.linalg.det(Fo) > 0
+        eps = np.log(ns)
+        np.testing.assert_allclose(tau, R1 @ np.diag(2 * mu * eps + lam * eps.sum()) @ R1.T, rtol=1e-9,
+                                   atol=1e-9 * lam * abs(eps).max())
+    # the same inverted F is elastic under the sign-blind Hencky measure: the deviator is
+    # small, so a projection (which would change the ratios of the singular values) must
+    # not happen
+    s, ns = cases[0]
+    Fo, _, _ = pyoracle.stress(DP, params, R1 @ np.diag(s) @ R2.T)
+    sv = np.linalg.svd(Fo, compute_uv=False)
+    np.testing.assert_allclose(sv, np.sort(ns)[::-1], atol=1e-12)
+    # mu = 0: neither branch of the compressive case runs, F stays inverted
+    F = R1 @ np.diag(cases[0][0]) @ R2.T
+    Fo, lj, _ = pyoracle.stress(DP, [lam, 0.0, 0, 0, 0, 1, 30], F, logJp=-0.5)
+    np.testing.assert_allclose(Fo, F, atol=0)
+    assert lj == -0.5
+
+
 def test_fluid_stress():
     K, gamma, nu = 1e5, 7.0, 0.1
     params = [0, 0, K, gamma, nu, 1, 30]
