@@ -250,6 +250,52 @@ def test_drucker_prager_elastic_branch_clamps_and_uninverts():
     assert lj == -0.5
 
 
+def test_clamped_sand_grid_sensitivity_to_one_ulp():
+    """The FP32 conditioning behind the GPU bar of the substep after a clamp
+    (test_gpu_parity.py, CLAMPED_NEXT_SUBSTEP_TOL).  In the FP64 oracle alone: sand with
+    inverted and near-singular particles takes one substep (F rebuilt, smallest singular
+    value 1e-4), F is stored in FP32 as E = F - I, and the next substep runs twice, once
+    with one ulp added to or taken from the stored E of the particles at the clamp.  The
+    grid momentum moves by more than 1e-5 of its largest value (so the usual bar cannot
+    hold for any FP32 engine) and by less than the 1e-4 bar."""
+    from tests.test_gpu_parity import CLAMPED_NEXT_SUBSTEP_TOL, _clamped_sand_scene
+
+    sc, picks, Fs = _clamped_sand_scene()
+    orc = Oracle.from_scene(sc)
+    orc.setup()
+    st = orc.state(0)
+    for p, F in zip(picks, Fs):
+        E = (F - np.eye(3)).astype(np.float32).astype(np.float64)
+        st[3:12, p] = (E + np.eye(3)).reshape(9)
+    orc.set_state(0, st)
+    orc.substep(0.0, sc.dt)
+    st1 = orc.state(0)
+    M1 = orc.grid_m().astype(np.float32)
+    E32 = (st1[3:12] - np.eye(3).reshape(9, 1)).astype(np.float32)
+    F32 = (E32.astype(np.float64) + np.eye(3).reshape(9, 1)).T.reshape(-1, 3, 3)
+    clamped = np.linalg.svd(F32, compute_uv=False)[:, 2] < 1e-3
+    assert clamped.sum() == 64
+
+    def next_grid(E):
+        o = Oracle.from_scene(sc)
+        o.setup()
+        o.substep(0.0, sc.dt)   # the same layout and ordering as orc
+        s = st1.copy()
+        s[3:12] = E.astype(np.float64) + np.eye(3).reshape(9, 1)
+        o.set_state(0, s)
+        o.set_grid_m(M1)
+        o.physics(sc.dt, sc.dt)
+        return o.grid_mnext()[:, 1:4]
+
+    base = next_grid(E32)
+    rng = np.random.default_rng(1)
+    Ep = E32.copy()
+    step = rng.integers(-1, 2, size=Ep[:, clamped].shape).astype(np.float32)
+    Ep[:, clamped] += step * np.spacing(np.abs(Ep[:, clamped]))
+    moved = np.abs(next_grid(Ep) - base).max() / np.abs(base).max()
+    assert 1e-5 < moved < CLAMPED_NEXT_SUBSTEP_TOL, moved
+
+
 def test_fluid_stress():
     K, gamma, nu = 1e5, 7.0, 0.1
     params = [0, 0, K, gamma, nu, 1, 30]
