@@ -12,6 +12,7 @@
 // (gpu_hash_table.cuh:463-513) does with its 16-thread tiles.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -37,10 +38,17 @@ __global__ void HashPackKernel(const uint32_t* keys, const uint32_t* vals, uint3
     }
 }
 
-__global__ void HashClearKernel(uint64_t* slots, uint32_t nslots, uint32_t* fill, uint32_t buckets) {
+// Empties the slots, zeroes the bucket fills, the error word and the stash count (the
+// stash pass sets the count of a non-empty build).
+__global__ void HashClearKernel(uint64_t* slots, uint32_t nslots, uint32_t* fill, uint32_t buckets, uint32_t* bad,
+                                uint32_t* nstash) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) {
         slots[i] = kHashEmpty;
         if (i < buckets) fill[i] = 0u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *bad = 0u;
+        *nstash = 0u;
     }
 }
 
@@ -224,50 +232,113 @@ size_t TmpWords(uint32_t m) {
     return 10ull * m + std::max(RadixTempWords(m), ScanTempWords(m)) + 64;
 }
 
+// Table geometry for at most max_keys pairs: load factor at most 0.7 (three rounds then
+// leave almost nothing for the stash).
+uint32_t BucketsFor(uint32_t max_keys) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(max_keys) * 10 + 7 * kHashBucket - 1) / (7 * kHashBucket));
+}
+uint32_t StashFor(uint32_t max_keys) { return 256 + max_keys / 64; }
+
+// Device memory of one build: the table (slots, stash, count) and its temporaries.
+struct BuildMem {
+    HashView v;        // the table as Find reads it: buckets, seeds, domain
+    uint64_t* slots;   // [buckets * 16]
+    uint64_t* stash;   // [stash_cap]
+    uint32_t* nstash;
+    uint32_t nblocks;
+    uint32_t max_keys;
+    uint32_t stash_cap;
+    uint32_t* bad;     // error bits: 1 key outside the domain, 2 duplicate, 4 stash overflow
+    uint32_t* fill;    // [buckets]
+    uint32_t* tmp;     // TmpWords(max_keys)
+};
+
+BuildMem MemOf(const ckmpm_hash* h) {
+    BuildMem b{};
+    b.v = View(h);
+    b.slots = h->slots;
+    b.stash = h->stash;
+    b.nstash = h->nstash;
+    b.nblocks = h->nblocks;
+    b.max_keys = h->max_keys;
+    b.stash_cap = h->stash_cap;
+    b.bad = h->bad;
+    b.fill = h->fill;
+    b.tmp = h->tmp;
+    return b;
+}
+
 // Enqueue the build from device arrays; no host sync.
-void EnqueueBuild(ckmpm_hash* h, const uint32_t* keys, const uint32_t* vals, uint32_t n) {
-    hipStream_t s = h->stream;
-    const uint32_t m = h->max_keys;
-    uint32_t* t = h->tmp;
+void EnqueueBuild(const BuildMem& b, const uint32_t* keys, const uint32_t* vals, uint32_t n, hipStream_t s) {
+    const uint32_t m = b.max_keys;
+    uint32_t* t = b.tmp;
     uint32_t *sk = t, *sv = t + m, *placed = t + 2 * m, *bucket = t + 3 * m, *idx = t + 4 * m;
     uint32_t *rank = t + 5 * m, *t0 = t + 6 * m, *t1 = t + 7 * m, *t2 = t + 8 * m, *alt = t + 9 * m;
     uint32_t* radix = t + 10 * static_cast<size_t>(m);
     uint32_t* total = radix + std::max(RadixTempWords(m), ScanTempWords(m));
-    const uint32_t nslots = h->buckets * kHashBucket;
-    (void)hipMemsetAsync(h->bad, 0, sizeof(uint32_t), s);
-    HashClearKernel<<<LaunchBlocks(nslots), kHashThreads, 0, s>>>(h->slots, nslots, h->fill, h->buckets);
-    h->n = n;
-    if (n == 0) {
-        (void)hipMemsetAsync(h->nstash, 0, sizeof(uint32_t), s);
-        return;
-    }
+    const uint32_t buckets = b.v.buckets, nslots = buckets * kHashBucket;
+    HashClearKernel<<<LaunchBlocks(nslots), kHashThreads, 0, s>>>(b.slots, nslots, b.fill, buckets, b.bad, b.nstash);
+    if (n == 0) return;
     const uint32_t g = LaunchBlocks(n);
-    HashPackKernel<<<g, kHashThreads, 0, s>>>(keys, vals, n, sk, sv, placed, h->nblocks, h->bad);
-    RadixSortPairs(sk, sv, t0, t1, n, KeyBits(h->nblocks), radix, s);
-    HashDupKernel<<<g, kHashThreads, 0, s>>>(sk, n, h->bad);
-    const HashView v = View(h);
-    const int bbits = KeyBits(h->buckets + 1);
+    HashPackKernel<<<g, kHashThreads, 0, s>>>(keys, vals, n, sk, sv, placed, b.nblocks, b.bad);
+    RadixSortPairs(sk, sv, t0, t1, n, KeyBits(b.nblocks), radix, s);
+    HashDupKernel<<<g, kHashThreads, 0, s>>>(sk, n, b.bad);
+    const int bbits = KeyBits(buckets + 1);
     for (int r = 0; r < kHashRounds; ++r) {
-        HashBucketKernel<<<g, kHashThreads, 0, s>>>(v, r, sk, placed, n, bucket, idx);
+        HashBucketKernel<<<g, kHashThreads, 0, s>>>(b.v, r, sk, placed, n, bucket, idx);
         RadixSortPairs(bucket, idx, t0, alt, n, bbits, radix, s);
         SegmentRanks(bucket, n, rank, t0, t1, t2, radix, s);
-        HashPlaceKernel<<<g, kHashThreads, 0, s>>>(bucket, idx, rank, n, h->buckets, sk, sv, h->fill, h->slots,
-                                                   placed);
-        HashFillKernel<<<g, kHashThreads, 0, s>>>(bucket, rank, n, h->buckets, h->fill);
+        HashPlaceKernel<<<g, kHashThreads, 0, s>>>(bucket, idx, rank, n, buckets, sk, sv, b.fill, b.slots, placed);
+        HashFillKernel<<<g, kHashThreads, 0, s>>>(bucket, rank, n, buckets, b.fill);
     }
     HashUnplacedKernel<<<g, kHashThreads, 0, s>>>(placed, n, t0);
     ExclusiveScan(t0, t1, nullptr, n, n, total, false, radix, s);
-    HashStashKernel<<<g, kHashThreads, 0, s>>>(t0, t1, n, sk, sv, h->stash, h->stash_cap, total, h->nstash, h->bad);
+    HashStashKernel<<<g, kHashThreads, 0, s>>>(t0, t1, n, sk, sv, b.stash, b.stash_cap, total, b.nstash, b.bad);
 }
 
-int CheckBuild(ckmpm_hash* h) {
+// Waits for the build and turns its error bits into a status code (and message).
+int BuildStatus(const uint32_t* bad_dev, hipStream_t s, std::string* err) {
     uint32_t bad = 0;
-    HASH_HIP(h, hipMemcpyAsync(&bad, h->bad, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
-    HASH_HIP(h, hipStreamSynchronize(h->stream));
-    if (bad & 1u) return HashFail(h, CKMPM_ERR_INVALID, "key outside the domain");
-    if (bad & 2u) return HashFail(h, CKMPM_ERR_INVALID, "duplicate key");
-    if (bad & 4u) return HashFail(h, CKMPM_ERR_CAPACITY, "hash stash overflow (too many keys for the table)");
-    return CKMPM_OK;
+    hipError_t e = hipMemcpyAsync(&bad, bad_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        if (err) *err = std::string("hash build: ") + hipGetErrorString(e);
+        return CKMPM_ERR_HIP;
+    }
+    const char* msg = nullptr;
+    int rc = CKMPM_OK;
+    if (bad & 1u) msg = "key outside the domain", rc = CKMPM_ERR_INVALID;
+    else if (bad & 2u) msg = "duplicate key", rc = CKMPM_ERR_INVALID;
+    else if (bad & 4u) msg = "hash stash overflow (too many keys for the table)", rc = CKMPM_ERR_CAPACITY;
+    if (msg && err) *err = msg;
+    return rc;
+}
+
+int CheckBuild(ckmpm_hash* h) { return BuildStatus(h->bad, h->stream, &h->err); }
+
+// Scratch of the build that allocates nothing (ckmpm_hash_build_noalloc): the
+// temporaries, bucket fills and error word of a table of up to
+// CKMPM_HASH_NOALLOC_MAX_KEYS pairs.  Each GPU holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress gives for
+// the device that is current, kept per device below.
+constexpr size_t kNoAllocScratchWords = 11u << 18;  // 11.5 MB
+__device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {};
+
+// Address of this device's scratch (looked up once per device), or null on error.
+uint32_t* DeviceScratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    return g_scratch[dev];
 }
 
 }  // namespace
@@ -286,9 +357,8 @@ int ckmpm_hash_create(const ckmpm_domain_desc* domain, uint32_t max_keys, ckmpm_
     for (int a = 0; a < 3; ++a) h->dim[a] = domain->dim[a];
     h->nblocks = static_cast<uint32_t>(nb);
     h->max_keys = max_keys;
-    // load factor at most 0.7: three rounds then leave almost nothing for the stash
-    h->buckets = static_cast<uint32_t>((static_cast<uint64_t>(max_keys) * 10 + 7 * kHashBucket - 1) / (7 * kHashBucket));
-    h->stash_cap = 256 + max_keys / 64;
+    h->buckets = BucketsFor(max_keys);
+    h->stash_cap = StashFor(max_keys);
     std::mt19937 rng(2);  // the reference's seeds (gpu_hash_table.cuh:303-309)
     for (int r = 0; r < kHashRounds; ++r) h->seed[r] = static_cast<uint32_t>(rng());
     bool ok = hipStreamCreate(&h->stream) == hipSuccess;
@@ -303,7 +373,7 @@ int ckmpm_hash_create(const ckmpm_domain_desc* domain, uint32_t max_keys, ckmpm_
         ckmpm_hash_destroy(h);
         return CKMPM_ERR_HIP;
     }
-    EnqueueBuild(h, nullptr, nullptr, 0);
+    EnqueueBuild(MemOf(h), nullptr, nullptr, 0, h->stream);
     if (hipStreamSynchronize(h->stream) != hipSuccess) {
         ckmpm_hash_destroy(h);
         return CKMPM_ERR_HIP;
@@ -315,8 +385,46 @@ int ckmpm_hash_create(const ckmpm_domain_desc* domain, uint32_t max_keys, ckmpm_
 int ckmpm_hash_build_device(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n) {
     if (!h || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
     if (n > h->max_keys) return HashFail(h, CKMPM_ERR_CAPACITY, "more keys than the table was created for");
-    EnqueueBuild(h, keys, values, n);
+    EnqueueBuild(MemOf(h), keys, values, n, h->stream);
     return CheckBuild(h);
+}
+
+int ckmpm_hash_build_noalloc(const ckmpm_domain_desc* domain, uint32_t max_keys, const uint32_t* keys,
+                             const uint32_t* values, uint32_t n, uint64_t* slots, uint64_t* stash, uint32_t* nstash,
+                             void* stream) {
+    if (!domain || max_keys == 0 || !slots || !stash || !nstash || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
+    const uint64_t nb = static_cast<uint64_t>(domain->dim[0]) * domain->dim[1] * domain->dim[2];
+    if (domain->dim[0] <= 0 || domain->dim[1] <= 0 || domain->dim[2] <= 0 || nb >= 0xFFFFFFFFull)
+        return CKMPM_ERR_INVALID;
+    if (n > max_keys) return CKMPM_ERR_CAPACITY;
+    const uint32_t buckets = BucketsFor(max_keys);
+    // the scratch holds the temporaries, the bucket fills and the error word
+    if (max_keys > CKMPM_HASH_NOALLOC_MAX_KEYS ||
+        TmpWords(max_keys) + buckets + 1 > kNoAllocScratchWords)
+        return CKMPM_ERR_CAPACITY;
+    uint32_t* base = DeviceScratch();
+    if (!base) return CKMPM_ERR_HIP;
+    BuildMem b{};
+    b.slots = slots;
+    b.stash = stash;
+    b.nstash = nstash;
+    b.v.slots = slots;
+    b.v.stash = stash;
+    b.v.nstash = nstash;
+    b.v.buckets = buckets;
+    std::mt19937 rng(2);  // the seeds of ckmpm_hash_create (gpu_hash_table.cuh:303-309)
+    for (int r = 0; r < kHashRounds; ++r) b.v.seed[r] = static_cast<uint32_t>(rng());
+    for (int a = 0; a < 3; ++a) b.v.dim[a] = domain->dim[a];
+    b.nblocks = static_cast<uint32_t>(nb);
+    b.max_keys = max_keys;
+    b.stash_cap = StashFor(max_keys);
+    b.tmp = base;
+    b.fill = base + TmpWords(max_keys);
+    b.bad = b.fill + buckets;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    EnqueueBuild(b, keys, values, n, s);
+    if (hipGetLastError() != hipSuccess) return CKMPM_ERR_HIP;
+    return BuildStatus(b.bad, s, nullptr);
 }
 
 int ckmpm_hash_build(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n) {
@@ -328,7 +436,7 @@ int ckmpm_hash_build(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values
         HASH_HIP(h, hipMemcpyAsync(dk, keys, sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
         HASH_HIP(h, hipMemcpyAsync(dv, values, sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
     }
-    EnqueueBuild(h, dk, dv, n);
+    EnqueueBuild(MemOf(h), dk, dv, n, h->stream);
     return CheckBuild(h);
 }
 

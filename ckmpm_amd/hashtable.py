@@ -89,3 +89,26 @@ class BlockHashTable:
             self.close()
         except Exception:
             pass
+
+
+NOALLOC_MAX_KEYS = 1 << 18   # CKMPM_HASH_NOALLOC_MAX_KEYS
+
+
+def table_sizes(max_keys: int):
+    """(slot words, stash words) of a table created for max_keys pairs."""
+    buckets = (int(max_keys) * 10 + 7 * 16 - 1) // (7 * 16)
+    return buckets * 16, 256 + int(max_keys) // 64
+
+
+def build_noalloc(dim: Sequence[int], max_keys: int, keys_ptr: int, values_ptr: int, n: int, slots_ptr: int,
+                  stash_ptr: int, nstash_ptr: int, stream: int = 0) -> int:
+    """ckmpm_hash_build_noalloc on device arrays (the table is the caller's memory, sized by
+    table_sizes); returns the status code."""
+    lib = native.load()
+    d = native.DomainDesc()
+    for a in range(3):
+        d.dim[a] = int(dim[a])
+        d.offset[a] = 0
+    vp = ctypes.c_void_p
+    return lib.ckmpm_hash_build_noalloc(ctypes.byref(d), int(max_keys), vp(keys_ptr), vp(values_ptr), int(n),
+                                        vp(slots_ptr), vp(stash_ptr), vp(nstash_ptr), vp(stream))

@@ -32,7 +32,7 @@ EXPORTS = [
     "ckmpm_destroy", "ckmpm_last_error", "ckmpm_autotune", "ckmpm_set_tuning", "ckmpm_get_tuning",
     "ckmpm_hash_create", "ckmpm_hash_build", "ckmpm_hash_build_device", "ckmpm_hash_find",
     "ckmpm_hash_find_device", "ckmpm_hash_synchronize", "ckmpm_hash_info", "ckmpm_hash_export",
-    "ckmpm_hash_destroy", "ckmpm_hash_last_error",
+    "ckmpm_hash_destroy", "ckmpm_hash_last_error", "ckmpm_hash_build_noalloc",
     "ckmpm_batch_create", "ckmpm_batch_run", "ckmpm_batch_synchronize", "ckmpm_batch_captures",
     "ckmpm_batch_destroy", "ckmpm_batch_last_error",
     "ckmpm_adjoint_enable", "ckmpm_checkpoint_save", "ckmpm_checkpoint_restore", "ckmpm_checkpoint_free",
@@ -143,6 +143,8 @@ def load():
     L.ckmpm_hash_info.argtypes = [_vp, ctypes.POINTER(HashInfo)]
     L.ckmpm_hash_export.argtypes = [_vp, _u64p, ctypes.c_uint64, _u64p, ctypes.c_uint32, _u32p]
     L.ckmpm_hash_destroy.argtypes = [_vp]
+    L.ckmpm_hash_build_noalloc.argtypes = [ctypes.POINTER(DomainDesc), ctypes.c_uint32, _vp, _vp, ctypes.c_uint32,
+                                           _vp, _vp, _vp, _vp]
     L.ckmpm_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(_vp)]
     L.ckmpm_batch_run.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32]
     L.ckmpm_batch_synchronize.argtypes = [_vp]

@@ -326,6 +326,22 @@ int ckmpm_hash_create(const ckmpm_domain_desc* domain, uint32_t max_keys, ckmpm_
 int ckmpm_hash_build(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n);
 /* Same from device arrays; the table's stream is synchronised before return */
 int ckmpm_hash_build_device(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values, uint32_t n);
+/* The build of ckmpm_hash_build_device, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The caller passes the
+ * device arrays: keys and values (n pairs), and the table to fill, sized for max_keys
+ * pairs as ckmpm_hash_create sizes it: slots of ceil(10 max_keys / 112) * 16 words, stash
+ * of 256 + max_keys / 64 words, and one word for the stash count.  The result is the
+ * table ckmpm_hash_create(domain, max_keys) + ckmpm_hash_build_device give, word for word.
+ * The temporaries come from a static device scratch of the library (11.5 MB on each GPU
+ * it runs on, held until the process ends), so max_keys is at most
+ * CKMPM_HASH_NOALLOC_MAX_KEYS (CKMPM_ERR_CAPACITY above it, or for n > max_keys).  Enqueued
+ * on stream (a hipStream_t; null: the null stream), which is synchronised before return.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+#define CKMPM_HASH_NOALLOC_MAX_KEYS (1u << 18)
+int ckmpm_hash_build_noalloc(const ckmpm_domain_desc* domain, uint32_t max_keys, const uint32_t* keys,
+                             const uint32_t* values, uint32_t n, uint64_t* slots, uint64_t* stash, uint32_t* nstash,
+                             void* stream);
 /* Find (CooperativeFind, gpu_hash_table.cuh:463-513) of host keys; missing -> 0xFFFFFFFF */
 int ckmpm_hash_find(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values);
 /* Same on device arrays, enqueued on the table's stream (no sync).  cooperative = 1: a
