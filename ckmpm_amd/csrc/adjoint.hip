@@ -965,6 +965,7 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
         if (m.kind != kFixedCorotated && m.kind != kNeoHookean)
             return Fail(e, CKMPM_ERR_INVALID, "the adjoint supports fixed-corotated and Neo-Hookean models");
         if (!m.pp_attr) return Fail(e, CKMPM_ERR_INVALID, "the adjoint needs per-particle lambda and mu");
+        if (m.rigid_attr) return Fail(e, CKMPM_ERR_INVALID, "the adjoint does not differentiate rigid particles");
     }
     AdjointState* A = new AdjointState();
     A->seg = segment;

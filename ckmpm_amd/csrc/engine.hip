@@ -107,6 +107,21 @@ void Mark(ckmpm_engine* e) {
 
 // Grid update with dt: velocities of the current layout from the previous layout's
 // mass / momentum; clears the next P2G target.
+// The grid's max speed (the CFL dt, the fixed-point range) leaves out nodes holding less
+// than 2^-20 of the lightest particle's mass (see GridUpdateArgs::vmax_mass_floor).
+float VmaxMassFloor(const ckmpm_engine* e) {
+    float mmin = 0.0f;
+    for (const ModelBuf& m : e->models) mmin = (mmin == 0.0f || m.mat.mass < mmin) ? m.mat.mass : mmin;
+    return std::ldexp(mmin, -20);
+}
+
+// A substep swaps the roles of the double-buffered layouts, particle states and grids.
+void FlipRoles(ckmpm_engine* e) {
+    e->cur = 1 - e->cur;
+    e->sc = 1 - e->sc;
+    e->mprev = 1 - e->mprev;
+}
+
 void PhaseGrid(ckmpm_engine* e, double dt, FusedRec* rec = nullptr) {
     hipStream_t s = e->stream;
     Mark(e);
@@ -122,6 +137,7 @@ void PhaseGrid(ckmpm_engine* e, double dt, FusedRec* rec = nullptr) {
     g.dt = static_cast<float>(dt);
     g.gravity = e->sim.gravity;
     g.mass_clamp = e->sim.mass_clamp;
+    g.vmax_mass_floor = VmaxMassFloor(e);
     g.cap = e->cap;
     g.st = e->status;
     g.nmodels = static_cast<int>(e->models.size());
@@ -192,6 +208,8 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
         a.code = m.code;
         a.cnt = m.cnt;
         a.pp_attr = m.pp_attr;
+        a.rigid_attr = m.rigid_attr;
+        for (int k = 0; k < 3; ++k) a.rigid_v[k] = m.rigid_v[k];
         a.dt = static_cast<float>(dt);
         a.next_dt = static_cast<float>(next_dt);
         a.st = e->status;
@@ -247,10 +265,15 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
         ExtendLayout(e, P, false, s, rec);
     }
     Mark(e);
-    e->cur = 1 - e->cur;
-    e->sc = sn;
-    e->mprev = 1 - e->mprev;
+    FlipRoles(e);
     e->substeps++;
+}
+
+// A copy ordered after the work queued on the engine's stream, waited for.
+int CopyOnStream(ckmpm_engine* e, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    CK_HIP(e, hipMemcpyAsync(dst, src, bytes, kind, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
+    return CKMPM_OK;
 }
 
 int CheckReady(ckmpm_engine* e) {
@@ -393,6 +416,39 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     return CKMPM_OK;
 }
 
+int ckmpm_set_rigid(ckmpm_engine* e, int32_t model, uint64_t count, const float velocity[3]) {
+    if (!e || !velocity) return CKMPM_ERR_INVALID;
+    if (model < 0 || model >= static_cast<int32_t>(e->models.size())) return Fail(e, CKMPM_ERR_INVALID, "model index");
+    ModelBuf& m = e->models[model];
+    if (m.kind == kFluid && count) return Fail(e, CKMPM_ERR_INVALID, "rigid particles need a solid model");
+    if (count > m.n) return Fail(e, CKMPM_ERR_INVALID, "more rigid particles than the model has");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(velocity[k])) return Fail(e, CKMPM_ERR_INVALID, "rigid velocity must be finite");
+    if (e->setup_done) {
+        if (count != m.rigid_count) return Fail(e, CKMPM_ERR_INVALID, "the rigid count is fixed at ckmpm_initial_setup");
+    } else if (count && !m.rigid_attr) {
+        if (e->adj) return Fail(e, CKMPM_ERR_INVALID, "the adjoint does not differentiate rigid particles");
+        // one more state attribute: replace the two state buffers
+        for (int i = 0; i < 2; ++i) {
+            auto it = std::find(e->allocs.begin(), e->allocs.end(), static_cast<void*>(m.st[i]));
+            if (it != e->allocs.end()) e->allocs.erase(it);
+            CK_HIP(e, hipFree(m.st[i]));
+            m.st[i] = nullptr;
+        }
+        m.rigid_attr = m.nattr;
+        m.nattr += 1;
+        for (int i = 0; i < 2; ++i) CK_TRY(Alloc(e, &m.st[i], static_cast<size_t>(m.n) * m.nattr));
+    }
+    m.rigid_count = count;
+    bool changed = false;
+    for (int k = 0; k < 3; ++k) {
+        changed |= m.rigid_v[k] != velocity[k];
+        m.rigid_v[k] = velocity[k];
+    }
+    if (changed) ++e->args_version;
+    return CKMPM_OK;
+}
+
 int ckmpm_initial_setup(ckmpm_engine* e) {
     if (!e) return CKMPM_ERR_INVALID;
     if (e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "setup already done");
@@ -429,7 +485,8 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
             CK_HIP(e, hipMemcpyAsync(lam, m.lam0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
             CK_HIP(e, hipMemcpyAsync(mu, m.mu0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));
         }
-        LaunchInitState(e->dom, m.kind, xyz, lam, mu, m.pp_attr, vals, m.n, InitAttr(m.mat), m.st[0], s);
+        LaunchInitState(e->dom, m.kind, xyz, lam, mu, m.pp_attr, vals, m.n, InitAttr(m.mat), m.st[0], s, m.rigid_attr,
+                        m.rigid_count);
         CK_HIP(e, hipStreamSynchronize(s));
     }
     ExclusiveScan(e->bitmap, e->word_off, nullptr, e->nwords, e->nwords, e->total, true, e->scan_tmp, s);
@@ -645,14 +702,17 @@ int ckmpm_get_positions(ckmpm_engine* e, int32_t model, float* xyz, uint64_t cap
     if (model < 0 || model >= static_cast<int32_t>(e->models.size()) || !xyz) return CKMPM_ERR_INVALID;
     ModelBuf& m = e->models[model];
     if (capacity < m.n) return Fail(e, CKMPM_ERR_INVALID, "capacity below particle count");
-    float* tmp = nullptr;
-    CK_HIP(e, hipMalloc(&tmp, 3 * sizeof(float) * static_cast<size_t>(m.n)));
+    // staging buffer for the largest model, allocated on the first call and kept (a frame
+    // loop calls this once per frame)
+    if (!e->pos_stage) {
+        uint32_t nmax = 0;
+        for (const ModelBuf& q : e->models) nmax = std::max(nmax, q.n);
+        CK_TRY(Alloc(e, &e->pos_stage, 3 * static_cast<size_t>(nmax)));
+    }
     const LayoutBuf& P = e->L[1 - e->cur];
-    LaunchExportPositions(e->dom, P.keys, P.lc, m.off[1 - e->sc], m.st[e->sc], m.n, tmp, GridFor(e->a_hint + 8), e->stream);
-    hipError_t r = hipMemcpyAsync(xyz, tmp, 3 * sizeof(float) * static_cast<size_t>(m.n), hipMemcpyDeviceToHost, e->stream);
-    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
-    (void)hipFree(tmp);
-    if (r != hipSuccess) return Fail(e, CKMPM_ERR_HIP, std::string("get_positions: ") + hipGetErrorString(r));
+    LaunchExportPositions(e->dom, P.keys, P.lc, m.off[1 - e->sc], m.st[e->sc], m.n, e->pos_stage,
+                          GridFor(e->a_hint + 8), e->stream);
+    CK_TRY(CopyOnStream(e, xyz, e->pos_stage, 3 * sizeof(float) * static_cast<size_t>(m.n), hipMemcpyDeviceToHost));
     if (n) *n = m.n;
     return CKMPM_OK;
 }
@@ -670,7 +730,7 @@ int ckmpm_export_layout(ckmpm_engine* e, int32_t which, uint32_t* keys, uint32_t
     }
     if (keys) {
         if (capacity < lc.Ex) return Fail(e, CKMPM_ERR_INVALID, "capacity below exterior count");
-        CK_HIP(e, hipMemcpy(keys, L.keys, lc.Ex * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        CK_TRY(CopyOnStream(e, keys, L.keys, lc.Ex * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
     return CKMPM_OK;
 }
@@ -685,10 +745,10 @@ int ckmpm_export_ordering(ckmpm_engine* e, int32_t model, int32_t which, uint32_
     LayoutCounts lc;
     CK_HIP(e, hipMemcpyAsync(&lc, e->L[li].lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
     CK_HIP(e, hipStreamSynchronize(e->stream));
-    if (perm) CK_HIP(e, hipMemcpy(perm, m.perm[oi], m.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (perm) CK_TRY(CopyOnStream(e, perm, m.perm[oi], m.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (offsets) {
         if (offsets_capacity < lc.A + 1) return Fail(e, CKMPM_ERR_INVALID, "offsets capacity below A + 1");
-        CK_HIP(e, hipMemcpy(offsets, m.off[oi], (lc.A + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        CK_TRY(CopyOnStream(e, offsets, m.off[oi], (lc.A + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
     return CKMPM_OK;
 }
@@ -697,8 +757,7 @@ int ckmpm_export_state(ckmpm_engine* e, int32_t model, float* soa) {
     CK_TRY(CheckReady(e));
     if (model < 0 || model >= static_cast<int32_t>(e->models.size()) || !soa) return CKMPM_ERR_INVALID;
     ModelBuf& m = e->models[model];
-    CK_HIP(e, hipStreamSynchronize(e->stream));
-    CK_HIP(e, hipMemcpy(soa, m.st[e->sc], static_cast<size_t>(m.n) * m.nattr * sizeof(float), hipMemcpyDeviceToHost));
+    CK_TRY(CopyOnStream(e, soa, m.st[e->sc], static_cast<size_t>(m.n) * m.nattr * sizeof(float), hipMemcpyDeviceToHost));
     return CKMPM_OK;
 }
 
@@ -706,8 +765,7 @@ int ckmpm_import_state(ckmpm_engine* e, int32_t model, const float* soa) {
     CK_TRY(CheckReady(e));
     if (model < 0 || model >= static_cast<int32_t>(e->models.size()) || !soa) return CKMPM_ERR_INVALID;
     ModelBuf& m = e->models[model];
-    CK_HIP(e, hipStreamSynchronize(e->stream));
-    CK_HIP(e, hipMemcpy(m.st[e->sc], soa, static_cast<size_t>(m.n) * m.nattr * sizeof(float), hipMemcpyHostToDevice));
+    CK_TRY(CopyOnStream(e, m.st[e->sc], soa, static_cast<size_t>(m.n) * m.nattr * sizeof(float), hipMemcpyHostToDevice));
     return CKMPM_OK;
 }
 
@@ -717,7 +775,7 @@ int ckmpm_export_grid_m(ckmpm_engine* e, float* grid, uint64_t capacity_blocks) 
     CK_HIP(e, hipMemcpyAsync(&lc, e->L[1 - e->cur].lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
     CK_HIP(e, hipStreamSynchronize(e->stream));
     if (capacity_blocks < lc.Nb || !grid) return Fail(e, CKMPM_ERR_INVALID, "capacity below neighbour count");
-    CK_HIP(e, hipMemcpy(grid, e->M[e->mprev], static_cast<size_t>(lc.Nb) * kMFloats * sizeof(float), hipMemcpyDeviceToHost));
+    CK_TRY(CopyOnStream(e, grid, e->M[e->mprev], static_cast<size_t>(lc.Nb) * kMFloats * sizeof(float), hipMemcpyDeviceToHost));
     return CKMPM_OK;
 }
 
@@ -727,7 +785,7 @@ int ckmpm_export_grid_v(ckmpm_engine* e, float* grid, uint64_t capacity_blocks) 
     CK_HIP(e, hipMemcpyAsync(&lc, e->L[1 - e->cur].lc, sizeof(lc), hipMemcpyDeviceToHost, e->stream));
     CK_HIP(e, hipStreamSynchronize(e->stream));
     if (capacity_blocks < lc.Nb || !grid) return Fail(e, CKMPM_ERR_INVALID, "capacity below neighbour count");
-    CK_HIP(e, hipMemcpy(grid, e->V, static_cast<size_t>(lc.Nb) * kVFloats * sizeof(float), hipMemcpyDeviceToHost));
+    CK_TRY(CopyOnStream(e, grid, e->V, static_cast<size_t>(lc.Nb) * kVFloats * sizeof(float), hipMemcpyDeviceToHost));
     return CKMPM_OK;
 }
 
@@ -789,6 +847,7 @@ struct BatchGraph {
     hipGraphExec_t exec = nullptr;
     std::vector<int> roles;         // engine buffer roles at capture
     std::vector<uint32_t> hints;    // block-count hints the launch sizes were chosen for
+    std::vector<uint64_t> versions; // engine argument versions at capture
     void* tables = nullptr;         // fused mode: argument tables and workgroup starts
 };
 
@@ -803,6 +862,7 @@ struct ckmpm_batch {
     // mode 4: one graph per parity of fused launches (each kernel of a substep once for
     // all scenes)
     BatchGraph fz[2];
+    BatchGraph fz1[2];  // fused mode, one substep: the odd remainder of a run
     // fused mode: streams and events that run a substep's G2P2G launches side by side
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> aux_done;
@@ -853,7 +913,7 @@ bool GraphCurrent(const ckmpm_batch* b, const BatchGraph& g) {
     if (!g.exec) return false;
     for (size_t i = 0; i < b->eng.size(); ++i) {
         const ckmpm_engine* e = b->eng[i];
-        if (EngineRoles(e) != g.roles[i]) return false;
+        if (EngineRoles(e) != g.roles[i] || e->args_version != g.versions[i]) return false;
         // launch sizes follow the block count; recapture once it drifts by more than 1/4
         const uint32_t h = g.hints[i], a = e->a_hint;
         if (a > h + h / 4 + 8 || a + a / 4 + 8 < h) return false;
@@ -865,6 +925,7 @@ int Capture(ckmpm_batch* b, BatchGraph& g) {
     ReleaseGraph(g);
     g.roles.assign(b->eng.size(), -1);
     g.hints.assign(b->eng.size(), 0);
+    g.versions.assign(b->eng.size(), 0);
     std::vector<bool> prof(b->eng.size());
     BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
     (void)hipEventRecord(b->fork, b->cs);
@@ -875,6 +936,7 @@ int Capture(ckmpm_batch* b, BatchGraph& g) {
         (void)hipStreamWaitEvent(e->stream, b->fork, 0);
         g.roles[i] = EngineRoles(e);
         g.hints[i] = e->a_hint;
+        g.versions[i] = e->args_version;
         for (int k = 0; k < 2; ++k) {
             PhaseGrid(e, e->dt_default);
             PhaseParticles(e, e->dt_default, e->dt_default);
@@ -896,6 +958,7 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
     ReleaseGraph(g);
     g.roles.assign(1, EngineRoles(e));
     g.hints.assign(1, e->a_hint);
+    g.versions.assign(1, e->args_version);
     const bool prof = e->profile;
     e->profile = false;
     BATCH_HIP(b, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
@@ -918,17 +981,20 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
 // stream, so every engine's launches keep their order (independent launches run side by
 // side: the G2P2G launches of a substep, and base + scatter beside the extension passes);
 // the scenes' buffers are disjoint.
-int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
+// Captures nsub (1 or 2) substeps of every engine as fused launches.
+int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
     ReleaseGraph(g);
     const size_t ne = b->eng.size();
     g.roles.assign(ne, -1);
     g.hints.assign(ne, 0);
+    g.versions.assign(ne, 0);
     FusedRec rec[2];
     for (size_t i = 0; i < ne; ++i) {
         g.roles[i] = EngineRoles(b->eng[i]);
         g.hints[i] = b->eng[i]->a_hint;
+        g.versions[i] = b->eng[i]->args_version;
     }
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < nsub; ++k)
         for (ckmpm_engine* e : b->eng) {
             const bool prof = e->profile;
             e->profile = false;
@@ -936,7 +1002,10 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
             PhaseParticles(e, e->dt_default, e->dt_default, &rec[k]);
             e->profile = prof;
         }
-    for (ckmpm_engine* e : b->eng) e->substeps -= 2;  // recorded, not run
+    for (ckmpm_engine* e : b->eng) e->substeps -= nsub;  // recorded, not run
+    // recording flipped the buffer roles nsub times; flip them back
+    if (nsub & 1)
+        for (ckmpm_engine* e : b->eng) FlipRoles(e);
     // host image of the tables: per launch, the argument structs then the workgroup starts
     struct Launch {
         int stage, kernel, variant;
@@ -951,7 +1020,7 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
         std::memcpy(img.data() + off, src, bytes);
         return off;
     };
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < nsub; ++k) {
         std::stable_sort(rec[k].ph.begin(), rec[k].ph.end(), [](const FusedPhase& x, const FusedPhase& y) {
             if (x.stage != y.stage) return x.stage < y.stage;
             if (x.kernel != y.kernel) return x.kernel < y.kernel;
@@ -1048,7 +1117,7 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g) {
 }
 
 bool EngineGraphCurrent(const ckmpm_engine* e, const BatchGraph& g) {
-    if (!g.exec || EngineRoles(e) != g.roles[0]) return false;
+    if (!g.exec || EngineRoles(e) != g.roles[0] || e->args_version != g.versions[0]) return false;
     const uint32_t h = g.hints[0], a = e->a_hint;
     return !(a > h + h / 4 + 8 || a + a / 4 + 8 < h);
 }
@@ -1137,26 +1206,38 @@ int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode) {
         --left;
     }
     const uint32_t pairs = left / 2;
-    if (pairs) {
-        BatchGraph& g = mode == 4 ? b->fz[Parity(b)] : b->g[Parity(b)];
-        if (!GraphCurrent(b, g)) {
-            BATCH_HIP(b, hipStreamSynchronize(b->cs));
-            for (ckmpm_engine* e : b->eng) BATCH_HIP(b, hipStreamSynchronize(e->stream));
-            const int rc = mode == 4 ? CaptureFused(b, g) : Capture(b, g);
-            if (rc != CKMPM_OK) return rc;
-        }
+    // fused mode runs an odd remainder as a one-substep fused graph; graph mode eagerly
+    const bool single = mode == 4 && (left & 1u);
+    if (pairs || single) {
         // the replay stream runs after everything already queued on the engine streams
         for (size_t i = 0; i < b->eng.size(); ++i) {
             BATCH_HIP(b, hipEventRecord(b->join[i], b->eng[i]->stream));
             BATCH_HIP(b, hipStreamWaitEvent(b->cs, b->join[i], 0));
         }
-        for (uint32_t p = 0; p < pairs; ++p) BATCH_HIP(b, hipGraphLaunch(g.exec, b->cs));
-        for (ckmpm_engine* e : b->eng) e->substeps += 2ull * pairs;
+    }
+    for (int nsub = 2; nsub >= 1; --nsub) {
+        const uint32_t reps = nsub == 2 ? pairs : (single ? 1u : 0u);
+        if (!reps) continue;
+        const int par = Parity(b);
+        BatchGraph& g = mode == 4 ? (nsub == 2 ? b->fz[par] : b->fz1[par]) : b->g[par];
+        if (!GraphCurrent(b, g)) {
+            BATCH_HIP(b, hipStreamSynchronize(b->cs));
+            for (ckmpm_engine* e : b->eng) BATCH_HIP(b, hipStreamSynchronize(e->stream));
+            const int rc = mode == 4 ? CaptureFused(b, g, nsub) : Capture(b, g);
+            if (rc != CKMPM_OK) return rc;
+        }
+        for (uint32_t p = 0; p < reps; ++p) BATCH_HIP(b, hipGraphLaunch(g.exec, b->cs));
+        for (ckmpm_engine* e : b->eng) {
+            e->substeps += static_cast<uint64_t>(nsub) * reps;
+            if (nsub == 1) FlipRoles(e);  // the engines' buffers now play the other roles
+        }
+    }
+    if (pairs || single) {
         // later work on an engine stream waits for the replays
         BATCH_HIP(b, hipEventRecord(b->done, b->cs));
         for (ckmpm_engine* e : b->eng) BATCH_HIP(b, hipStreamWaitEvent(e->stream, b->done, 0));
     }
-    if (left & 1u)
+    if ((left & 1u) && !single)
         for (ckmpm_engine* e : b->eng) EagerSubstep(e);
     const hipError_t r = hipGetLastError();
     if (r != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
@@ -1186,6 +1267,8 @@ int ckmpm_batch_destroy(ckmpm_batch* b) {
     ReleaseGraph(b->g[1]);
     ReleaseGraph(b->fz[0]);
     ReleaseGraph(b->fz[1]);
+    ReleaseGraph(b->fz1[0]);
+    ReleaseGraph(b->fz1[1]);
     for (int p = 0; p < 2; ++p)
         for (BatchGraph& g : b->pe[p]) ReleaseGraph(g);
     for (hipEvent_t ev : b->join)

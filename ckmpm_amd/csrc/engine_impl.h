@@ -24,6 +24,9 @@ struct ModelBuf {
     uint32_t* cnt_e = nullptr;
     float* blkmax = nullptr;  // [cap] largest P2G deposit bound per particle block
     int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
+    int rigid_attr = 0;   // state attribute of the rigid flag (1: rigid); 0: none
+    uint64_t rigid_count = 0;
+    float rigid_v[3] = {0.f, 0.f, 0.f};
     int threads = 256;    // G2P2G workgroup size (256 or 512; ckmpm_autotune chooses)
     hipEvent_t tune_ev[2] = {nullptr, nullptr};  // around the G2P2G launch while tuning
     // setup inputs (host), released after the initial setup
@@ -79,11 +82,15 @@ struct ckmpm_engine {
     uint32_t* newidx = nullptr;
     uint32_t* scan_tmp = nullptr;
     uint32_t* total = nullptr;
+    float* pos_stage = nullptr;  // ckmpm_get_positions staging (3 floats per particle of the largest model)
     std::vector<ModelBuf> models;
     int sc = 0;           // state / ordering buffer index of the current step
     bool setup_done = false;
     uint32_t a_hint = 0;  // particle-block count at the last host read
     uint64_t substeps = 0;
+    // bumped when a value that kernel arguments carry changes after setup (the rigid
+    // velocity): captured batch graphs hold arguments by value and are captured again
+    uint64_t args_version = 0;
     double dt_default = 0;
     double time = 0, cur_dt = 0;
     std::string err;

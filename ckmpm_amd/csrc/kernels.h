@@ -45,8 +45,11 @@ void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t*
 // order: in = setup sort permutation (processing position -> input index),
 // out = identity ordering with move 13.
 // pp_attr: attribute index of per-particle (lambda, mu), 0 when the material's are used.
+// rigid_attr: attribute index of the rigid flag (1 for the input particles below
+// rigid_count), 0: none.
 void LaunchInitState(const DomainDev& d, int kind, const float* xyz, const float* lam_p, const float* mu_p, int pp_attr,
-                     uint32_t* order, uint32_t n, float init, float* st, hipStream_t s);
+                     uint32_t* order, uint32_t n, float init, float* st, hipStream_t s, int rigid_attr = 0,
+                     uint64_t rigid_count = 0);
 void LaunchExportPositions(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, const uint32_t* off,
                            const float* st, uint32_t n, float* xyz, uint32_t grid, hipStream_t s);
 void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uint32_t n, uint32_t nvel, float mass,
@@ -78,6 +81,10 @@ struct GridUpdateArgs {
     float* V;
     float* M_zero;               // cleared for the current layout (next P2G target)
     float dt, gravity, mass_clamp;
+    // nodes with less mass are left out of the max speed (not of the velocity field): the
+    // fixed-point P2G gives such a node's momentum as a few units, so its p / m can be far
+    // off, while particles see the node only with a weight of that size
+    float vmax_mass_floor;
     uint32_t cap;
     Status* st;
     // per-block P2G deposit bounds of the previous layout's G2P2G, reduced into st->dep_bits
@@ -104,6 +111,8 @@ struct G2P2GArgs {
     uint32_t* code;
     uint32_t* cnt;
     int pp_attr;  // attribute index of per-particle (lambda, mu) in the state, 0: none
+    int rigid_attr;     // attribute index of the rigid flag, 0: none
+    float rigid_v[3];   // velocity of the rigid particles
     float dt, next_dt;
     Status* st;
     int model;      // index of st->dep_bits

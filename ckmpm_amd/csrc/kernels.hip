@@ -59,6 +59,12 @@ __device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t& tot
     return wbase + inc - v;
 }
 
+// Number of set bits of a wave mask below this lane (mbcnt).
+__device__ __forceinline__ uint32_t LanesBelow(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 // Lanes of the wave holding the same small integer value (bits significant bits).
 template <int bits>
 __device__ __forceinline__ uint64_t MatchPeers(uint32_t v) {
@@ -284,7 +290,8 @@ __global__ void SetupOffsetsKernel(const uint32_t* sorted, uint32_t n, const uin
 // ordering becomes the identity with move 13 (no move), like InitializeAdvectionBucket
 // (mpm_algorithm.cuh:949-964).
 __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const float* lam_p, const float* mu_p,
-                                int pp_attr, uint32_t* order, uint32_t n, float init, float* st) {
+                                int pp_attr, uint32_t* order, uint32_t n, float init, float* st, int rigid_attr,
+                                uint64_t rigid_count) {
     const size_t N = n;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const uint32_t src = order[p];
@@ -299,6 +306,7 @@ __global__ void InitStateKernel(DomainDev d, int kind, const float* xyz, const f
             st[pp_attr * N + p] = lam_p[src];
             st[(pp_attr + 1) * N + p] = mu_p[src];
         }
+        if (rigid_attr) st[rigid_attr * N + p] = src < rigid_count ? 1.0f : 0.0f;
         if (kind == kFluid) {
             st[3 * N + p] = init - 1.0f;  // the fluid state is q = J - 1 (init = J0)
         } else {
@@ -694,7 +702,7 @@ __device__ __forceinline__ void GridUpdateBody(const GridUpdateArgs& a, uint32_t
                 v2 = INFINITY;
                 SetFlag(a.st, kFlagNonFinite);
             }
-            vmax = fmaxf(vmax, v2);
+            if (m[u] >= a.vmax_mass_floor || v2 == INFINITY) vmax = fmaxf(vmax, v2);
         }
     }
     // largest deposit bound per model over the previous layout's particle blocks
@@ -748,20 +756,17 @@ constexpr int kWin = 512;  // 8^3 nodes of one sub-grid window
 // nodes [0,7]^3: 343 + 512 nodes per channel instead of 2 x 512.
 constexpr int kM0Win = 343;              // 7^3
 constexpr int kMWin = kM0Win + 512;      // both sub-grids, one channel
-// window entry r in [0, kMWin) -> sub-grid and node coordinates
-__device__ __forceinline__ void WindowNode(int r, int& g, int& lx, int& ly, int& lz) {
-    if (r >= kM0Win) {
-        const int l = r - kM0Win;
-        g = 1;
-        lx = l >> 6;
-        ly = (l >> 3) & 7;
-        lz = l & 7;
-    } else {
-        g = 0;
-        lx = r / 49 + 1;
-        ly = (r / 7) % 7 + 1;
-        lz = r % 7 + 1;
-    }
+// Halo entry (block o of the 2x2x2 neighbourhood b + {0,1}^3, bit 2 = x, bit 1 = y,
+// bit 0 = z; node nd = g*64 + x*16 + y*4 + z of its 128) -> window entry, or -1 for a
+// grid-0 node outside [1,7]^3.  The halo load and the flush walk the neighbourhood block
+// by block, so a wavefront reads or updates 64 consecutive nodes of one grid block.
+__device__ __forceinline__ int WindowOfHalo(int o, int nd) {
+    const int lx = ((o >> 2) & 1) * 4 + ((nd >> 4) & 3);
+    const int ly = ((o >> 1) & 1) * 4 + ((nd >> 2) & 3);
+    const int lz = (o & 1) * 4 + (nd & 3);
+    if (nd >= 64) return kM0Win + lx * 64 + ly * 8 + lz;
+    if (lx == 0 || ly == 0 || lz == 0) return -1;
+    return (lx - 1) * 49 + (ly - 1) * 7 + (lz - 1);
 }
 
 // P2G accumulator in LDS: 64-bit fixed point.  On gfx950 a wavefront's ds_add_f32
@@ -783,27 +788,40 @@ __device__ __forceinline__ void WindowNode(int r, int& g, int& lx, int& ly, int&
 //             reaches 2^(e-1) deposits its momentum with FP32 global atomics instead, so a
 //             sudden growth of the deposits is handled exactly, only more slowly.  The
 //             momentum resolution 2^(e - kq) is about 1e-13 of the largest deposit.
-// The conversion round(v 2^s) is one FP64 fma: for |v 2^s| < 2^51, d = v 2^s + 1.5 2^52
-// has a fixed exponent and holds the rounded integer in its mantissa, so the integer is
-// d's bit pattern minus that of 1.5 2^52 (whose low word is zero: one 32-bit subtract).
-// That is 3 VALU instructions per deposit instead of 11 for a float -> int64 cast.
+// The mass resolution must stay at least as fine as the momentum's relative to m vmax:
+// at a node that only a stencil tail reaches (w ~ 1e-12) a coarser mass makes p / m, and
+// so the grid's max speed that sets the CFL dt, wrong by a large factor.
+// Conversion: for an integer-valued |x| < 2^51 the FP64 sum x + 1.5 2^52 has a fixed
+// exponent and holds x in its mantissa, so x is that sum's bit pattern minus the constant
+// B = 0x4338 << 48 (whose low word is zero: one 32-bit subtract).  The momentum terms are
+// scaled by 2^s in FP32 once per particle (a power of two: exact) and converted to FP64
+// once per particle and sub-grid; each node's momenta are formed by FP64 adds, the weight
+// is converted once per node, and a deposit is one FP64 fma (w p exact in FP64, rounded
+// once) and one subtract: 2 VALU instructions per deposit.
 // e >= -64 keeps every scale 2^(kq - e) and its inverse a normal FP32 number.
 using LdsAcc = long long;
 constexpr int kMinExp = -64;
-__device__ __forceinline__ void LdsAdd(LdsAcc* p, float v, double scale) {
-    const double d = fma(static_cast<double>(v), scale, 6755399441055744.0);  // 1.5 * 2^52
+constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr unsigned long long kMagicBits = 0x4338000000000000ull;
+__device__ __forceinline__ void LdsAddFixed(LdsAcc* p, double d) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p),
-              static_cast<unsigned long long>(__double_as_longlong(d)) - 0x4338000000000000ull);
+              static_cast<unsigned long long>(__double_as_longlong(d)) - kMagicBits);
 }
 
 // Deposit of one particle into both sub-grid windows: node momentum pm - sum_c kc_c t_c
-// + sum_c o_c kc_c at stencil offset o (pm = m v, kc_c = dx K e_c); weights at scale
-// wscale, momentum at qscale, or (SLOW) momentum added straight to the grid M of the
-// halo blocks.
+// + sum_c o_c kc_c at stencil offset o (pm = m v, kc_c = dx K e_c), with pm and kc
+// already multiplied by the momentum scale 2^(kq - e); weights at the mass scale wscale.
+// SLOW: the momentum goes straight to the grid M of the halo blocks, scaled back by
+// inv_qs.
 template <bool SLOW>
 __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2], const float (&pm)[3],
-                                           const float (&kc)[3][3], double wscale, double qscale,
+                                           const float (&kc)[3][3], double wscale, float inv_qs,
                                            const uint32_t* sHalo, float* M) {
+    double kd[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) kd[c][r] = static_cast<double>(kc[c][r]);
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
         const Stencil32& sg = sn[g];
@@ -811,8 +829,12 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
         const int s1 = g ? 8 : 7, s2 = g ? 64 : 49, lo = g ? 0 : 1;
         LdsAcc* base = sM + (g ? kM0Win : 0) + (sg.c[0] - lo) * s2 + (sg.c[1] - lo) * s1 + (sg.c[2] - lo);
         float pa[3];
+        double pad[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) pa[r] = pm[r] - (kc[0][r] * sg.t[0] + kc[1][r] * sg.t[1] + kc[2][r] * sg.t[2]);
+        for (int r = 0; r < 3; ++r) {
+            pa[r] = pm[r] - (kc[0][r] * sg.t[0] + kc[1][r] * sg.t[1] + kc[2][r] * sg.t[2]);
+            pad[r] = static_cast<double>(pa[r]);
+        }
         float wxy[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -821,8 +843,9 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
         for (int o = 0; o < 8; ++o) {
             const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
             const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
+            const double wd = static_cast<double>(w);
             const int off = ox * s2 + oy * s1 + oz;
-            LdsAdd(base + off, w, wscale);
+            LdsAddFixed(base + off, fma(wd, wscale, kMagic));
             uint32_t bi = kEmpty;
             int node = 0;
             if (SLOW) {
@@ -833,31 +856,52 @@ __device__ __forceinline__ void DepositP2G(LdsAcc* sM, const Stencil32 (&sn)[2],
             }
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                float pr = pa[r];
-                if (ox) pr += kc[0][r];
-                if (oy) pr += kc[1][r];
-                if (oz) pr += kc[2][r];
-                if (!SLOW) LdsAdd(base + (r + 1) * kMWin + off, w * pr, qscale);
-                else if (bi != kEmpty) unsafeAtomicAdd(M + static_cast<size_t>(bi) * kMFloats + (r + 1) * 128 + node, w * pr);
+                if (!SLOW) {
+                    double pr = pad[r];
+                    if (ox) pr += kd[0][r];
+                    if (oy) pr += kd[1][r];
+                    if (oz) pr += kd[2][r];
+                    LdsAddFixed(base + (r + 1) * kMWin + off, fma(wd, pr, kMagic));
+                } else if (bi != kEmpty) {
+                    float pr = pa[r];
+                    if (ox) pr += kc[0][r];
+                    if (oy) pr += kc[1][r];
+                    if (oz) pr += kc[2][r];
+                    unsafeAtomicAdd(M + static_cast<size_t>(bi) * kMFloats + (r + 1) * 128 + node, w * pr * inv_qs);
+                }
             }
         }
     }
 }
 
-// 4 waves per SIMD: LDS allows 4 workgroups per CU; this caps the VGPRs at 128
+// Particles are taken in 64-particle chunks, one wavefront per chunk (chunk c of a block
+// goes to wave c mod waves): with a few hundred particles per block, 256-particle
+// chunks would leave about a quarter of the lanes idle.  The rank of a particle among
+// the block's particles with the same move (its counting-sort position) is fixed in
+// rounds of kRoundChunks chunks: each chunk stores its lanes' move and rank within the
+// chunk and its per-move counts in LDS; after the round, one thread per move turns the
+// counts into exclusive prefixes over the chunks, and the codes are written.
+constexpr int kRoundChunks = 12;
+constexpr uint32_t kRoundParticles = kRoundChunks * 64;
+constexpr int kHistStride = 28;  // per chunk: 27 move counts (then prefixes), padded
+
+// 4 waves per SIMD: LDS (40.1 KiB per workgroup) allows 4 workgroups per CU; this caps
+// the VGPRs at 128
 template <int KIND, int THREADS>
 __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint32_t ng) {
+    constexpr uint32_t kWaves = THREADS / 64;
     __shared__ float sV[kVChannels * kMWin];
     __shared__ __align__(16) LdsAcc sM[kMChannels * kMWin];
+    __shared__ uint16_t sCode[kRoundParticles];               // move | rank in the chunk << 5
+    __shared__ uint16_t sHist[kRoundChunks * kHistStride];    // per chunk and move
     __shared__ uint32_t sHalo[8];
-    __shared__ uint32_t sWave[THREADS / 64][32];
-    __shared__ uint32_t sRun[32];
-    __shared__ uint32_t sBlkMax;  // largest deposit bound of the block (FP32 bits)
-    __shared__ uint32_t sGlobal;  // particles of the block deposited with global atomics
+    __shared__ uint32_t sRun[32];   // particles of the block per move before this round
+    __shared__ uint32_t sTot[32];   // particles per move in this round
+    __shared__ uint32_t sBlkMax;    // largest deposit bound of the block (FP32 bits)
+    __shared__ uint32_t sGlobal;    // particles of the block deposited with global atomics
 
     const uint32_t A = a.lc->A, Nb = a.lc->Nb;
     const uint32_t tid = threadIdx.x, lane = LaneId(), wave = tid >> 6;
-    const uint64_t lt = (1ull << lane) - 1ull;
     const float dx = a.dom.dx, inv_dx = a.dom.inv_dx;
     const size_t N = a.n;
     // momentum fixed-point exponent of this launch (LdsAcc): deposits with a bound below
@@ -871,6 +915,12 @@ __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint3
         if (ref > 0.0f) mexp = max(kMinExp, static_cast<int>((__float_as_uint(ref) >> 23) & 0xFFu) - 127 + 2);
     }
     const float fast_limit = __int_as_float((mexp - 1 + 127) << 23);  // 2^(mexp-1)
+
+    // The accumulator and the move counts are cleared once here; the flush and the
+    // prefix pass leave them zero for the next block.  16-byte clears (the size is even).
+    for (int i = 2 * tid; i < kMChannels * kMWin; i += 2 * THREADS)
+        *reinterpret_cast<uint4*>(sM + i) = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = tid; i < kRoundChunks * kHistStride; i += THREADS) sHist[i] = 0;
 
     // Round-robin over the block list: particle counts per block vary by orders of
     // magnitude along the list (a model occupies only part of it), so contiguous
@@ -897,25 +947,23 @@ __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint3
             sBlkMax = 0u;
             sGlobal = 0u;
         }
-        // fixed-point scale of this block's weights (LdsAcc)
+        // fixed-point scales of this block (LdsAcc): weights 2^kq, momentum 2^(kq - mexp)
         const uint32_t nblk = end - beg;
         const int kq = min(50, 62 - (nblk > 1u ? 32 - __clz(nblk - 1u) : 0));
         // powers of two built from their bit patterns: integer (scalar) arithmetic, so the
         // block-uniform scales stay in scalar registers across the particle loop
-        const double wscale = __longlong_as_double(static_cast<long long>(kq + 1023) << 52);         // 2^kq
-        const double qscale = __longlong_as_double(static_cast<long long>(kq - mexp + 1023) << 52);  // 2^(kq - mexp)
+        const double wscale = __longlong_as_double(static_cast<long long>(kq + 1023) << 52);  // 2^kq
+        const float qs = __int_as_float((kq - mexp + 127) << 23);                             // 2^(kq - mexp)
+        const float inv_qs = __int_as_float((mexp - kq + 127) << 23);
         float tmax = 0.0f;  // largest deposit bound of this thread's particles
-        // 16-byte clears (kMChannels * kMWin is even)
-        for (int i = 2 * tid; i < kMChannels * kMWin; i += 2 * THREADS)
-            *reinterpret_cast<uint4*>(sM + i) = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
-        // velocity halo in the same windows as the P2G accumulator (3 channels; the
-        // window-node mapping is computed once per node)
-        for (int r = tid; r < kMWin; r += THREADS) {
-            int g, lx, ly, lz;
-            WindowNode(r, g, lx, ly, lz);
-            const uint32_t bi = sHalo[((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2)];
-            const float* src = a.V + static_cast<size_t>(bi) * kVFloats + g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
+        // velocity halo in the same windows as the P2G accumulator (3 channels)
+        for (int e = tid; e < 8 * kNodesPerBlock; e += THREADS) {
+            const int o = e >> 7, nd = e & 127;
+            const int r = WindowOfHalo(o, nd);
+            if (r < 0) continue;
+            const uint32_t bi = sHalo[o];
+            const float* src = a.V + static_cast<size_t>(bi) * kVFloats + nd;
 #pragma unroll
             for (int ch = 0; ch < kVChannels; ++ch) sV[ch * kMWin + r] = bi != kEmpty ? src[ch * 128] : 0.0f;
         }
@@ -926,301 +974,344 @@ __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint3
         }
         __syncthreads();
 
-        for (uint32_t c0 = beg; c0 < end; c0 += THREADS) {
-            const uint32_t t = c0 + tid;
-            const bool active = t < end;
-            uint32_t dir = 31u;
-            if (active) {
-                const uint32_t pe = a.perm[t];
-                uint32_t s = pe & kSlotMask;
-                int mv = static_cast<int>(pe >> kMoveShift);
-                if (s >= a.n || mv >= kDirections) {
-                    SetFlag(a.st, kFlagMissingBlock);
-                    s = 0;
-                    mv = 13;
-                }
-                if (a.ids_out) a.ids_out[t] = a.ids_in[s];
-                // position relative to this block's origin: the stored value is relative
-                // to the block that wrote it, which sits at this block - move
-                const int mov[3] = {mv / 9 - 1, (mv / 3) % 3 - 1, mv % 3 - 1};
-                float x[3], F[9], q = 0.0f, logJp = 0.0f;  // F holds E = F - I; q = J - 1 (fluid)
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    x[k] = __fsub_rn(a.st_in[k * N + s], __fmul_rn(static_cast<float>(4 * mov[k]), dx));
-                if (KIND == kFluid) {
-                    q = a.st_in[3 * N + s];
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) F[k] = a.st_in[(3 + k) * N + s];
-                    if (KIND == kDruckerPrager || KIND == kCamClay) logJp = a.st_in[12 * N + s];
-                }
-                float lam = a.mat.lambda, mu = a.mat.mu;
-                if (a.pp_attr) {
-                    lam = a.st_in[a.pp_attr * N + s];
-                    mu = a.st_in[(a.pp_attr + 1) * N + s];
-                }
-                // ---------------- G2P at x (both sub-grids, 16 nodes)
-                float xs[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
-                float v[3] = {0.f, 0.f, 0.f}, B[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                bool bad = false;
-                Stencil32 sp[2];
-                MakeStencilPair(xs, sp[0], sp[1]);
-#pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    const Stencil32& sg = sp[g];
-                    int l[3];
-                    float m1[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int lc = sg.c[k];
-                        // the binned cell (2..5 in the frame of the block that binned the
-                        // particle) can move by one in this block's frame through FP32
-                        // rounding of + 0.25, so the bases lie in [1,6] (grid 0), [0,6] (grid 1)
-                        const int lo = g ? 0 : 1;
-                        bad |= lc < lo || lc > 6;
-                        l[k] = min(max(lc, lo), 6) - lo;
-                        // D closed form per axis: second moment on the diagonal, products of
-                        // first moments off the diagonal (units of dx^2)
-                        const float w0 = sg.w0[k], w1 = 1.0f - w0, tt = sg.t[k];
-                        m1[k] = w1 - tt;
-                        D[k * 4] += w0 * tt * tt + w1 * (1.0f - tt) * (1.0f - tt);
+        for (uint32_t r0 = beg; r0 < end; r0 += kRoundParticles) {
+            const uint32_t rn = min(end - r0, kRoundParticles);
+            const uint32_t nch = (rn + 63u) >> 6;
+            const bool last = r0 + rn == end;
+            for (uint32_t c = wave; c < nch; c += kWaves) {
+                const uint32_t t = r0 + c * 64u + lane;
+                const bool active = t < r0 + rn;
+                uint32_t dir = 31u;
+                if (active) {
+                    const uint32_t pe = a.perm[t];
+                    uint32_t s = pe & kSlotMask;
+                    int mv = static_cast<int>(pe >> kMoveShift);
+                    if (s >= a.n || mv >= kDirections) {
+                        SetFlag(a.st, kFlagMissingBlock);
+                        s = 0;
+                        mv = 13;
                     }
-                    D[1] += m1[0] * m1[1];
-                    D[2] += m1[0] * m1[2];
-                    D[5] += m1[1] * m1[2];
-                    // B = sum_i v_i (x_i - x_p)^T w_i with x_i - x_p = (o - t) dx per axis:
-                    // B_rc = dx (S_rc - vg_r t_c), S_rc = sum over the nodes with o_c = 1 of w v_r
-                    const int s1 = g ? 8 : 7, s2 = g ? 64 : 49;
-                    const float* base = sV + (g ? kM0Win : 0) + l[0] * s2 + l[1] * s1 + l[2];
-                    float vg[3] = {0.f, 0.f, 0.f}, S[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                    float wxy[4];
+                    if (a.ids_out) a.ids_out[t] = a.ids_in[s];
+                    // position relative to this block's origin: the stored value is relative
+                    // to the block that wrote it, which sits at this block - move
+                    const int mov[3] = {mv / 9 - 1, (mv / 3) % 3 - 1, mv % 3 - 1};
+                    float x[3], F[9], q = 0.0f, logJp = 0.0f;  // F holds E = F - I; q = J - 1 (fluid)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        wxy[q] = ((q >> 1) ? 1.0f - sg.w0[0] : sg.w0[0]) * ((q & 1) ? 1.0f - sg.w0[1] : sg.w0[1]);
+                    for (int k = 0; k < 3; ++k)
+                        x[k] = __fsub_rn(a.st_in[k * N + s], __fmul_rn(static_cast<float>(4 * mov[k]), dx));
+                    if (KIND == kFluid) {
+                        q = a.st_in[3 * N + s];
+                    } else {
 #pragma unroll
-                    for (int o = 0; o < 8; ++o) {
-                        const int ox = (o >> 2) & 1, oy = (o >> 1) & 1, oz = o & 1;
-                        const float w = wxy[o >> 1] * (oz ? 1.0f - sg.w0[2] : sg.w0[2]);
-                        const int off = ox * s2 + oy * s1 + oz;
-                        const float wv[3] = {w * base[off], w * base[kMWin + off], w * base[2 * kMWin + off]};
+                        for (int k = 0; k < 9; ++k) F[k] = a.st_in[(3 + k) * N + s];
+                        if (KIND == kDruckerPrager || KIND == kCamClay) logJp = a.st_in[12 * N + s];
+                    }
+                    float lam = a.mat.lambda, mu = a.mat.mu;
+                    if (a.pp_attr) {
+                        lam = a.st_in[a.pp_attr * N + s];
+                        mu = a.st_in[(a.pp_attr + 1) * N + s];
+                    }
+                    const float rigid = a.rigid_attr ? a.st_in[a.rigid_attr * N + s] : 0.0f;
+                    // ---------------- G2P at x (both sub-grids, 16 nodes)
+                    float xs[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
+                    float v[3] = {0.f, 0.f, 0.f}, B[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    float D[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    bool bad = false;
+                    Stencil32 sp[2];
+                    MakeStencilPair(xs, sp[0], sp[1]);
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        const Stencil32& sg = sp[g];
+                        int l[3];
+                        float m1[3];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const int lc = sg.c[k];
+                            // the binned cell (2..5 in the frame of the block that binned the
+                            // particle) can move by one in this block's frame through FP32
+                            // rounding of + 0.25, so the bases lie in [1,6] (grid 0), [0,6] (grid 1)
+                            const int lo = g ? 0 : 1;
+                            bad |= lc < lo || lc > 6;
+                            l[k] = min(max(lc, lo), 6) - lo;
+                            // D closed form per axis: second moment on the diagonal, products of
+                            // first moments off the diagonal (units of dx^2)
+                            const float w0 = sg.w0[k], w1 = 1.0f - w0, tt = sg.t[k];
+                            m1[k] = w1 - tt;
+                            D[k * 4] += w0 * tt * tt + w1 * (1.0f - tt) * (1.0f - tt);
+                        }
+                        D[1] += m1[0] * m1[1];
+                        D[2] += m1[0] * m1[2];
+                        D[5] += m1[1] * m1[2];
+                        // B = sum_i v_i (x_i - x_p)^T w_i with x_i - x_p = (o - t) dx per axis:
+                        // B_rc = dx (S_rc - vg_r t_c), S_rc = sum over the nodes with o_c = 1 of w v_r.
+                        // The weights are a tensor product, so vg and S are contracted axis by
+                        // axis (z, then y, then x): 22 instructions per channel.
+                        const int s1 = g ? 8 : 7, s2 = g ? 64 : 49;
+                        const float* base = sV + (g ? kM0Win : 0) + l[0] * s2 + l[1] * s1 + l[2];
+                        const float wx0 = sg.w0[0], wy0 = sg.w0[1], wz0 = sg.w0[2];
+                        const float wx1 = 1.0f - wx0, wy1 = 1.0f - wy0, wz1 = 1.0f - wz0;
+                        float vg[3], S[9];
 #pragma unroll
                         for (int r = 0; r < 3; ++r) {
-                            vg[r] += wv[r];
-                            if (ox) S[r * 3 + 0] += wv[r];
-                            if (oy) S[r * 3 + 1] += wv[r];
-                            if (oz) S[r * 3 + 2] += wv[r];
+                            const float* vr = base + r * kMWin;
+                            float T[4], U[4];  // per (x, y) column: sum over z, z-moment
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const int off = (q >> 1) * s2 + (q & 1) * s1;
+                                U[q] = wz1 * vr[off + 1];
+                                T[q] = fmaf(wz0, vr[off], U[q]);
+                            }
+                            float P[2], Q[2], Z[2];  // per x row: sum over y and z, y-moment, z-moment
+#pragma unroll
+                            for (int ox = 0; ox < 2; ++ox) {
+                                Q[ox] = wy1 * T[ox * 2 + 1];
+                                P[ox] = fmaf(wy0, T[ox * 2], Q[ox]);
+                                Z[ox] = fmaf(wy0, U[ox * 2], wy1 * U[ox * 2 + 1]);
+                            }
+                            S[r * 3 + 0] = wx1 * P[1];
+                            vg[r] = fmaf(wx0, P[0], S[r * 3 + 0]);
+                            S[r * 3 + 1] = fmaf(wx0, Q[0], wx1 * Q[1]);
+                            S[r * 3 + 2] = fmaf(wx0, Z[0], wx1 * Z[1]);
+                        }
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            v[r] += vg[r];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) B[r * 3 + c] += dx * (S[r * 3 + c] - vg[r] * sg.t[c]);
                         }
                     }
+                    if (bad) SetFlag(a.st, kFlagOutOfWindow);
+                    const float dx2 = dx * dx;
+                    D[0] *= dx2;
+                    D[4] *= dx2;
+                    D[8] *= dx2;
+                    D[1] *= dx2;
+                    D[2] *= dx2;
+                    D[5] *= dx2;
+                    D[3] = D[1];
+                    D[6] = D[2];
+                    D[7] = D[5];
+                    float Di[9], C[9];
+                    InverseSym3(D, Di);
+                    MatMul3(B, Di, C);
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        v[r] += vg[r];
+                    for (int k = 0; k < 3; ++k) v[k] *= 0.5f;
+                    if (rigid != 0.0f) {
+                        // a rigid particle takes nothing from the grid and moves at the
+                        // prescribed velocity (mpm_algorithm.cuh:1609-1617)
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) B[r * 3 + c] += dx * (S[r * 3 + c] - vg[r] * sg.t[c]);
+                        for (int k = 0; k < 9; ++k) C[k] = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            x[k] += a.dt * a.rigid_v[k];
+                            v[k] = 0.0f;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) x[k] += a.dt * v[k];
                     }
-                }
-                if (bad) SetFlag(a.st, kFlagOutOfWindow);
-                const float dx2 = dx * dx;
-                D[0] *= dx2;
-                D[4] *= dx2;
-                D[8] *= dx2;
-                D[1] *= dx2;
-                D[2] *= dx2;
-                D[5] *= dx2;
-                D[3] = D[1];
-                D[6] = D[2];
-                D[7] = D[5];
-                float Di[9], C[9];
-                InverseSym3(D, Di);
-                MatMul3(B, Di, C);
+                    // ---------------- deformation update and stress
+                    float tau[9];
+                    if (KIND == kFluid) {
+                        StressFluid(a.mat, C, a.dt, q, tau);
+                    } else {
+                        // F' = (I + dt C) F  <=>  E' = E + dt C (I + E), E = F - I
+                        float CE[9];
+                        MatMul3(C, F, CE);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    v[k] *= 0.5f;
-                    x[k] += a.dt * v[k];
-                }
-                // ---------------- deformation update and stress
-                float tau[9];
-                if (KIND == kFluid) {
-                    StressFluid(a.mat, C, a.dt, q, tau);
-                } else {
-                    // F' = (I + dt C) F  <=>  E' = E + dt C (I + E), E = F - I
-                    float CE[9];
-                    MatMul3(C, F, CE);
+                        for (int k = 0; k < 9; ++k) F[k] += a.dt * (C[k] + CE[k]);
+                        if (KIND == kFixedCorotated) StressFixedCorotated(lam, mu, F, tau);
+                        else if (KIND == kNeoHookean) StressNeoHookean(lam, mu, F, tau);
+                        else if (KIND == kDruckerPrager) StressDruckerPrager(lam, mu, a.mat.sin_phi, F, logJp, tau);
+                        else if (KIND == kVonMises) StressVonMises(lam, mu, a.mat.yield, F, tau);
+                        else StressCamClay(a.mat, lam, mu, F, logJp, tau);
+                    }
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) F[k] += a.dt * (C[k] + CE[k]);
-                    if (KIND == kFixedCorotated) StressFixedCorotated(lam, mu, F, tau);
-                    else if (KIND == kNeoHookean) StressNeoHookean(lam, mu, F, tau);
-                    else if (KIND == kDruckerPrager) StressDruckerPrager(lam, mu, a.mat.sin_phi, F, logJp, tau);
-                    else if (KIND == kVonMises) StressVonMises(lam, mu, a.mat.yield, F, tau);
-                    else StressCamClay(a.mat, lam, mu, F, logJp, tau);
-                }
+                    for (int k = 0; k < 3; ++k) a.st_out[k * N + t] = x[k];
+                    if (KIND == kFluid) {
+                        a.st_out[3 * N + t] = q;
+                    } else {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) a.st_out[k * N + t] = x[k];
-                if (KIND == kFluid) {
-                    a.st_out[3 * N + t] = q;
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) a.st_out[(3 + k) * N + t] = F[k];
-                    if (KIND == kDruckerPrager || KIND == kCamClay) a.st_out[12 * N + t] = logJp;
-                }
-                if (a.pp_attr) {
-                    a.st_out[a.pp_attr * N + t] = lam;
-                    a.st_out[(a.pp_attr + 1) * N + t] = mu;
-                }
-                // ---------------- block move for the rebin (FP32 grid-0 binning)
-                int dlt[3];
-                bool clamped = false;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int dd = FloorDiv4(Cell0(x[k], inv_dx) - 2);
-                    clamped |= dd < -1 || dd > 1;
-                    dlt[k] = min(max(dd, -1), 1);
-                }
-                if (clamped) SetFlag(a.st, kFlagOutOfWindow);
-                if (!InDomain(a.dom, bb[0] + dlt[0], bb[1] + dlt[1], bb[2] + dlt[2])) {
-                    SetFlag(a.st, kFlagOutOfDomain);
-                    dlt[0] = dlt[1] = dlt[2] = 0;
-                }
-                dir = static_cast<uint32_t>((dlt[0] + 1) * 9 + (dlt[1] + 1) * 3 + (dlt[2] + 1));
-                // ---------------- P2G at the new x
-#pragma unroll
-                for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
-                Stencil32 sn[2];
-                MakeStencilPair(xs, sn[0], sn[1]);
-                bool inwin = true;
-                float Dn[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    float m1[3];
+                        for (int k = 0; k < 9; ++k) a.st_out[(3 + k) * N + t] = F[k];
+                        if (KIND == kDruckerPrager || KIND == kCamClay) a.st_out[12 * N + t] = logJp;
+                    }
+                    if (a.pp_attr) {
+                        a.st_out[a.pp_attr * N + t] = lam;
+                        a.st_out[(a.pp_attr + 1) * N + t] = mu;
+                    }
+                    if (a.rigid_attr) a.st_out[a.rigid_attr * N + t] = rigid;
+                    // ---------------- block move for the rebin (FP32 grid-0 binning)
+                    int dlt[3];
+                    bool clamped = false;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const int lc = sn[g].c[k];
-                        inwin &= g == 0 ? (lc >= 1 && lc <= 6) : (lc >= 0 && lc <= 6);
-                        const float w0 = sn[g].w0[k], w1 = 1.0f - w0, tt = sn[g].t[k];
-                        m1[k] = w1 - tt;
-                        Dn[k * 4] += w0 * tt * tt + w1 * (1.0f - tt) * (1.0f - tt);
+                        const int dd = FloorDiv4(Cell0(x[k], inv_dx) - 2);
+                        clamped |= dd < -1 || dd > 1;
+                        dlt[k] = min(max(dd, -1), 1);
                     }
-                    Dn[1] += m1[0] * m1[1];
-                    Dn[2] += m1[0] * m1[2];
-                    Dn[5] += m1[1] * m1[2];
-                }
-                if (!inwin) {
-                    SetFlag(a.st, kFlagOutOfWindow);
-                } else {
-                    // D^ = (0.5 D')^-1 = 2 D'^-1; K = m C - nextDt V D^ tau
-                    const float h = 0.5f * dx2;
-                    Dn[0] *= h;
-                    Dn[4] *= h;
-                    Dn[8] *= h;
-                    Dn[1] *= h;
-                    Dn[2] *= h;
-                    Dn[5] *= h;
-                    Dn[3] = Dn[1];
-                    Dn[6] = Dn[2];
-                    Dn[7] = Dn[5];
-                    float Dh[9], DT[9], K[9];
-                    InverseSym3(Dn, Dh);
-                    MatMul3(Dh, tau, DT);
-                    const float sv = a.next_dt * a.mat.volume;
-#pragma unroll
-                    for (int k = 0; k < 9; ++k) K[k] = C[k] * a.mat.mass - sv * DT[k];
-                    float pm[3], kc[3][3];  // m v and dx K e_c
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) pm[r] = a.mat.mass * v[r];
-                    // node momentum m v + K (x_i - x_p), x_i - x_p = (o - t) dx:
-                    // = a + sum_c o_c kc_c with a = m v - dx K t and kc_c = dx K e_c
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) kc[c][r] = dx * K[r * 3 + c];
-                    float bound = 0.0f, bsum = fabsf(x[0]) + fabsf(x[1]) + fabsf(x[2]);
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const float b = fabsf(pm[r]) + fabsf(kc[0][r]) + fabsf(kc[1][r]) + fabsf(kc[2][r]);
-                        bound = fmaxf(bound, b);
-                        bsum += b;
+                    if (clamped) SetFlag(a.st, kFlagOutOfWindow);
+                    if (!InDomain(a.dom, bb[0] + dlt[0], bb[1] + dlt[1], bb[2] + dlt[2])) {
+                        SetFlag(a.st, kFlagOutOfDomain);
+                        dlt[0] = dlt[1] = dlt[2] = 0;
                     }
-                    // a non-finite deposit has no fixed-point value: report it (the sum is
-                    // NaN or inf whenever an input is; fmaxf would drop a NaN)
-                    if (!(bsum <= 3.0e38f)) {
-                        SetFlag(a.st, kFlagNonFinite);
+                    dir = static_cast<uint32_t>((dlt[0] + 1) * 9 + (dlt[1] + 1) * 3 + (dlt[2] + 1));
+                    // ---------------- P2G at the new x
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) xs[k] = __fmul_rn(x[k], inv_dx);
+                    Stencil32 sn[2];
+                    MakeStencilPair(xs, sn[0], sn[1]);
+                    bool inwin = true;
+                    float Dn[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        float m1[3];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const int lc = sn[g].c[k];
+                            inwin &= g == 0 ? (lc >= 1 && lc <= 6) : (lc >= 0 && lc <= 6);
+                            const float w0 = sn[g].w0[k], w1 = 1.0f - w0, tt = sn[g].t[k];
+                            m1[k] = w1 - tt;
+                            Dn[k * 4] += w0 * tt * tt + w1 * (1.0f - tt) * (1.0f - tt);
+                        }
+                        Dn[1] += m1[0] * m1[1];
+                        Dn[2] += m1[0] * m1[2];
+                        Dn[5] += m1[1] * m1[2];
+                    }
+                    if (!inwin) {
+                        SetFlag(a.st, kFlagOutOfWindow);
                     } else {
-                        tmax = fmaxf(tmax, bound);
-                        if (bound < fast_limit) {
-                            DepositP2G<false>(sM, sn, pm, kc, wscale, qscale, sHalo, a.M);
+                        // D^ = (0.5 D')^-1 = 2 D'^-1; K = m C - nextDt V D^ tau
+                        const float h = 0.5f * dx2;
+                        Dn[0] *= h;
+                        Dn[4] *= h;
+                        Dn[8] *= h;
+                        Dn[1] *= h;
+                        Dn[2] *= h;
+                        Dn[5] *= h;
+                        Dn[3] = Dn[1];
+                        Dn[6] = Dn[2];
+                        Dn[7] = Dn[5];
+                        float Dh[9], DT[9], K[9];
+                        InverseSym3(Dn, Dh);
+                        MatMul3(Dh, tau, DT);
+                        const float sv = a.next_dt * a.mat.volume;
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) K[k] = C[k] * a.mat.mass - sv * DT[k];
+                        float pm[3], kc[3][3];  // m v and dx K e_c
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) pm[r] = a.mat.mass * v[r];
+                        // node momentum m v + K (x_i - x_p), x_i - x_p = (o - t) dx:
+                        // = a + sum_c o_c kc_c with a = m v - dx K t and kc_c = dx K e_c
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) kc[c][r] = dx * K[r * 3 + c];
+                        float bound = 0.0f, bsum = fabsf(x[0]) + fabsf(x[1]) + fabsf(x[2]);
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            const float b = fabsf(pm[r]) + fabsf(kc[0][r]) + fabsf(kc[1][r]) + fabsf(kc[2][r]);
+                            bound = fmaxf(bound, b);
+                            bsum += b;
+                        }
+                        // a non-finite deposit has no fixed-point value: report it (the sum is
+                        // NaN or inf whenever an input is; fmaxf would drop a NaN)
+                        if (!(bsum <= 3.0e38f)) {
+                            SetFlag(a.st, kFlagNonFinite);
                         } else {
-                            DepositP2G<true>(sM, sn, pm, kc, wscale, qscale, sHalo, a.M);
-                            const uint64_t on = __ballot(1);
-                            if (lane == static_cast<uint32_t>(__builtin_ctzll(on)))
-                                atomicAdd(&sGlobal, static_cast<uint32_t>(__popcll(on)));
+                            tmax = fmaxf(tmax, bound);
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) {
+                                pm[r] *= qs;
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) kc[c][r] *= qs;
+                            }
+                            if (bound < fast_limit) {
+                                DepositP2G<false>(sM, sn, pm, kc, wscale, inv_qs, sHalo, a.M);
+                            } else {
+                                DepositP2G<true>(sM, sn, pm, kc, wscale, inv_qs, sHalo, a.M);
+                                const uint64_t on = __ballot(1);
+                                if (lane == static_cast<uint32_t>(__builtin_ctzll(on)))
+                                    atomicAdd(&sGlobal, static_cast<uint32_t>(__popcll(on)));
+                            }
                         }
                     }
                 }
-            }
-            // ---------------- deterministic rank among this block's particles with the same move
-            if (tid < THREADS / 64 * 32) (&sWave[0][0])[tid] = 0;
-            __syncthreads();
-            const uint64_t peers = MatchPeers<5>(dir);
-            const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
-            if (rank == 0 && dir < 27u) sWave[wave][dir] = static_cast<uint32_t>(__popcll(peers));
-            __syncthreads();
-            if (active) {
-                uint32_t r = sRun[dir] + rank;
-                for (uint32_t w = 0; w < wave; ++w) r += sWave[w][dir];
-                a.code[t] = dir | (r << 5);
-            }
-            __syncthreads();
-            if (tid < 27) {
-                uint32_t add = 0;
-                for (uint32_t w = 0; w < THREADS / 64; ++w) add += sWave[w][tid];
-                sRun[tid] += add;
-            }
-            __syncthreads();
-        }
-        // ---------------- largest deposit bound of the block (for the next substep's scale)
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
-        if (lane == 0 && tmax > 0.0f) atomicMax(&sBlkMax, __float_as_uint(tmax));
-        // ---------------- flush: one global atomic per non-zero node and channel
-        // (the window-node mapping once per node; fixed point -> FP32 through an exact
-        // double, 2^-kq m for the weights, 2^(mexp-kq) for the momentum)
-        {
-            const double unit[2] = {static_cast<double>(__int_as_float((127 - kq) << 23)) * a.mat.mass,
-                                    static_cast<double>(__int_as_float((mexp - kq + 127) << 23))};
-            for (int r = tid; r < kMWin; r += THREADS) {
-                LdsAcc qv[kMChannels];
-                bool any = false;
-#pragma unroll
-                for (int ch = 0; ch < kMChannels; ++ch) {
-                    qv[ch] = sM[ch * kMWin + r];
-                    any |= qv[ch] != 0;
-                }
-                if (!any) continue;
-                int g, lx, ly, lz;
-                WindowNode(r, g, lx, ly, lz);
-                const uint32_t bi = sHalo[((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2)];
-                if (bi == kEmpty) continue;
-                float* dst = a.M + static_cast<size_t>(bi) * kMFloats + g * 64 + (lx & 3) * 16 + (ly & 3) * 4 + (lz & 3);
-#pragma unroll
-                for (int ch = 0; ch < kMChannels; ++ch) {
-                    if (qv[ch] == 0) continue;
-                    // |q| < 2^63: hi * 2^32 + lo is exact to double rounding (2^-53)
-                    const double d = fma(static_cast<double>(static_cast<int>(qv[ch] >> 32)), 4294967296.0,
-                                         static_cast<double>(static_cast<uint32_t>(qv[ch])));
-                    const float val = static_cast<float>(d * unit[ch == 0 ? 0 : 1]);
-                    if (val != 0.0f) unsafeAtomicAdd(dst + ch * 128, val);
+                // ---------------- rank among the chunk's particles with the same move
+                const uint64_t peers = MatchPeers<5>(dir);
+                const uint32_t rank = LanesBelow(peers);
+                if (active) {
+                    sCode[t - r0] = static_cast<uint16_t>(dir | (rank << 5));
+                    if (rank == 0) sHist[c * kHistStride + dir] = static_cast<uint16_t>(__popcll(peers));
                 }
             }
+            // largest deposit bound of the block (for the next substep's scale): one LDS
+            // max per wave instruction (non-negative floats order as their bits)
+            if (last && tmax > 0.0f) atomicMax(&sBlkMax, __float_as_uint(tmax));
+            __syncthreads();
+            // exclusive prefix of each move's counts over the round's chunks
+            if (tid < kDirections) {
+                uint32_t run = 0;
+                for (uint32_t c = 0; c < nch; ++c) {
+                    const uint32_t h = sHist[c * kHistStride + tid];
+                    sHist[c * kHistStride + tid] = static_cast<uint16_t>(run);
+                    run += h;
+                }
+                sTot[tid] = run;
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < rn; i += THREADS) {
+                const uint32_t v = sCode[i], dir = v & 31u;
+                const uint32_t r = sRun[dir] + sHist[(i >> 6) * kHistStride + dir] + (v >> 5);
+                a.code[r0 + i] = dir | (r << 5);
+            }
+            if (last) {
+                // ---------------- flush: one global atomic per non-zero node and channel,
+                // block by block of the neighbourhood (fixed point -> FP32 through an exact
+                // double, 2^-kq m for the weights, 2^(mexp-kq) for the momentum); every entry
+                // read non-zero is cleared for the next block
+                const double unit[2] = {static_cast<double>(__int_as_float((127 - kq) << 23)) * a.mat.mass,
+                                        static_cast<double>(__int_as_float((mexp - kq + 127) << 23))};
+                for (int e = tid; e < 8 * kNodesPerBlock; e += THREADS) {
+                    const int o = e >> 7, nd = e & 127;
+                    const int r = WindowOfHalo(o, nd);
+                    if (r < 0) continue;
+                    LdsAcc qv[kMChannels];
+                    bool any = false;
+#pragma unroll
+                    for (int ch = 0; ch < kMChannels; ++ch) {
+                        qv[ch] = sM[ch * kMWin + r];
+                        any |= qv[ch] != 0;
+                    }
+                    if (!any) continue;
+#pragma unroll
+                    for (int ch = 0; ch < kMChannels; ++ch) sM[ch * kMWin + r] = 0;
+                    const uint32_t bi = sHalo[o];
+                    if (bi == kEmpty) continue;
+                    float* dst = a.M + static_cast<size_t>(bi) * kMFloats + nd;
+#pragma unroll
+                    for (int ch = 0; ch < kMChannels; ++ch) {
+                        if (qv[ch] == 0) continue;
+                        // |q| < 2^63: hi * 2^32 + lo is exact to double rounding (2^-53)
+                        const double d = fma(static_cast<double>(static_cast<int>(qv[ch] >> 32)), 4294967296.0,
+                                             static_cast<double>(static_cast<uint32_t>(qv[ch])));
+                        const float val = static_cast<float>(d * unit[ch == 0 ? 0 : 1]);
+                        if (val != 0.0f) unsafeAtomicAdd(dst + ch * 128, val);
+                    }
+                }
+            }
+            __syncthreads();
+            // next round (or block): move totals carried, counts cleared
+            for (uint32_t i = tid; i < nch * kHistStride; i += THREADS) sHist[i] = 0;
+            if (tid < kDirections) sRun[tid] += sTot[tid];
+            if (!last) __syncthreads();
         }
         if (tid < 27) a.cnt[static_cast<size_t>(j) * kDirections + tid] = sRun[tid];
-        __syncthreads();
         if (tid == 0) {
             a.blkmax[j] = __uint_as_float(sBlkMax);
             if (sGlobal) atomicAdd(&a.st->p2g_global, sGlobal);
         }
-        __syncthreads();
     }
 }
 // The body reads the arguments through the kernel-argument segment pointer (the explicit
@@ -1587,8 +1678,10 @@ void LaunchSetupOffsets(const uint32_t* sorted_keys, uint32_t n, const uint32_t*
     SetupOffsetsKernel<<<Blocks(a_cap + 1, kThreads), kThreads, 0, s>>>(sorted_keys, n, akeys, lc, off);
 }
 void LaunchInitState(const DomainDev& d, int kind, const float* xyz, const float* lam_p, const float* mu_p, int pp_attr,
-                     uint32_t* order, uint32_t n, float init, float* st, hipStream_t s) {
-    InitStateKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, kind, xyz, lam_p, mu_p, pp_attr, order, n, init, st);
+                     uint32_t* order, uint32_t n, float init, float* st, hipStream_t s, int rigid_attr,
+                     uint64_t rigid_count) {
+    InitStateKernel<<<Blocks(n, kThreads), kThreads, 0, s>>>(d, kind, xyz, lam_p, mu_p, pp_attr, order, n, init, st,
+                                                             rigid_attr, rigid_count);
 }
 void LaunchExportPositions(const DomainDev& d, const uint32_t* keys, const LayoutCounts* lc, const uint32_t* off,
                            const float* st, uint32_t n, float* xyz, uint32_t grid, hipStream_t s) {

@@ -109,6 +109,13 @@ class MPMEngine:
             self._check(rc, "ckmpm_add_model")
             self.models.append(m)
 
+    def set_rigid(self, model: int, count: int, velocity) -> None:
+        """The first `count` particles of `model` (input order) are rigid and move at
+        `velocity`; the count is fixed before InitialSetup, the velocity may change
+        between substeps (GetRigidParticleCount / GetRigidParticleVelocity)."""
+        v = np.ascontiguousarray(velocity, np.float32).reshape(3)
+        self._check(self._lib.ckmpm_set_rigid(self._h, int(model), int(count), _ptr(v, _f32p)), "ckmpm_set_rigid")
+
     def InitialSetup(self):
         self._check(self._lib.ckmpm_initial_setup(self._h), "ckmpm_initial_setup")
 
@@ -380,13 +387,19 @@ class MPMBatch:
             pass
 
 
-def engine_from_scene(scene: Scene, capacity: Optional[int] = None, adjoint_segment: int = 0) -> MPMEngine:
+def engine_from_scene(scene: Scene, capacity: Optional[int] = None, adjoint_segment: int = 0,
+                      fps: Optional[int] = None, cfl: Optional[float] = None) -> MPMEngine:
     """Build and set up an engine for a synthetic scene (adjoint_segment > 0: with an
-    adjoint tape of that many substeps)."""
+    adjoint tape of that many substeps; fps and cfl: the frame loop's rate and CFL number
+    instead of MPMConfig's defaults)."""
     eng = MPMEngine()
-    eng.Initialize(MPMDomain(scene.dim, scene.offset),
-                   MPMConfig(dx=scene.dx, gravity=scene.gravity, mass_clamp=scene.mass_clamp, dt=scene.dt,
-                             max_active_block_count=capacity or scene.block_capacity()))
+    cfg = MPMConfig(dx=scene.dx, gravity=scene.gravity, mass_clamp=scene.mass_clamp, dt=scene.dt,
+                    max_active_block_count=capacity or scene.block_capacity())
+    if fps is not None:
+        cfg.fps = int(fps)
+    if cfl is not None:
+        cfg.cfl = float(cfl)
+    eng.Initialize(MPMDomain(scene.dim, scene.offset), cfg)
     models = []
     for m in scene.models:
         q = m.material
@@ -394,6 +407,9 @@ def engine_from_scene(scene: Scene, capacity: Optional[int] = None, adjoint_segm
                           q.alpha0, q.beta, q.hardening_factor, q.m, q.hardening_on)
         models.append(MPMModel(m.mass, m.volume, m.x, m.v, mat, m.lam_p, m.mu_p))
     eng.InitializeParticle(models)
+    for i, m in enumerate(scene.models):
+        if m.rigid_count:
+            eng.set_rigid(i, m.rigid_count, m.rigid_velocity)
     if scene.boundary is not None:
         eng.set_boundary(scene.boundary)
     if adjoint_segment:

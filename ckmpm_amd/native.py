@@ -28,7 +28,7 @@ EXPORTS = [
     "ckmpm_synchronize", "ckmpm_get_counters", "ckmpm_clear_flags", "ckmpm_get_positions", "ckmpm_model_info",
     "ckmpm_export_layout", "ckmpm_export_ordering", "ckmpm_export_state", "ckmpm_import_state",
     "ckmpm_export_grid_m", "ckmpm_export_grid_v", "ckmpm_set_profiling", "ckmpm_get_phase_times",
-    "ckmpm_set_conservation", "ckmpm_get_conservation", "ckmpm_set_boundary",
+    "ckmpm_set_conservation", "ckmpm_get_conservation", "ckmpm_set_boundary", "ckmpm_set_rigid",
     "ckmpm_destroy", "ckmpm_last_error", "ckmpm_autotune", "ckmpm_set_tuning", "ckmpm_get_tuning",
     "ckmpm_hash_create", "ckmpm_hash_build", "ckmpm_hash_build_device", "ckmpm_hash_find",
     "ckmpm_hash_find_device", "ckmpm_hash_synchronize", "ckmpm_hash_info", "ckmpm_hash_export",
@@ -128,6 +128,7 @@ def load():
                                          ctypes.POINTER(ctypes.c_uint64)]
     L.ckmpm_get_phase_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
     L.ckmpm_destroy.argtypes = [_vp]
+    L.ckmpm_set_rigid.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint64, _f32p]
     L.ckmpm_autotune.argtypes = [_vp, ctypes.c_uint32]
     L.ckmpm_set_tuning.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32]
     L.ckmpm_get_tuning.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]

@@ -119,6 +119,8 @@ class ModelSpec:
     v: np.ndarray            # (1, 3) or (n, 3) float32
     lam_p: Optional[np.ndarray] = None   # per-particle Lame parameters (float32) or None
     mu_p: Optional[np.ndarray] = None
+    rigid_count: int = 0                 # the first rigid_count particles are rigid
+    rigid_velocity: tuple = (0.0, 0.0, 0.0)
 
 
 @dataclasses.dataclass

@@ -90,7 +90,8 @@ typedef struct {
     uint32_t neighbor_blocks; /* grid blocks (activePartitionNeighborBlockCount_) */
     uint32_t exterior_blocks; /* activePartitionExteriorBlockCount_ */
     uint32_t flags;           /* CKMPM_FLAG_* accumulated since the last clear */
-    float max_speed;          /* sqrt(max |v|^2) of the last grid update */
+    float max_speed;          /* sqrt(max |v|^2) of the last grid update, over the nodes
+                                 holding at least 2^-20 of the lightest particle's mass */
     uint32_t p2g_global_deposits; /* particles whose P2G momentum went to the grid with FP32
                                      global atomics (deposit above the fixed-point range of
                                      the launch) since the last clear; 0 in steady runs */
@@ -163,6 +164,16 @@ typedef struct {
  * be called while the engine's stream is being captured.  An engine with the adjoint
  * enabled keeps the default (its grid backward differentiates that rule only). */
 int ckmpm_set_boundary(ckmpm_engine* eng, const ckmpm_boundary_desc* boundary);
+
+/* Rigid particles: replaces the scene hooks GetExistRigidParticle, GetRigidParticleCount
+ * and GetRigidParticleVelocity(frame) (mpm_config.h:92-106) and their use in G2P2G
+ * (mpm_algorithm.cuh:1065-1068 flag, 1316-1328 and 1609-1624 update).  The first `count`
+ * particles of a solid model (input order) are rigid: their G2P takes nothing from the
+ * grid (C = 0, v = 0, so F is unchanged) and they move by dt * velocity; they still
+ * deposit their mass and stress in P2G.  `count` is fixed before ckmpm_initial_setup
+ * (the flag travels with each particle as one more state attribute); `velocity` (m/s)
+ * may be changed between substeps, as the reference's velocity of the frame. */
+int ckmpm_set_rigid(ckmpm_engine* eng, int32_t model, uint64_t count, const float velocity[3]);
 
 /* Waits for the stream; returns the first error encoded in the device flags. */
 int ckmpm_synchronize(ckmpm_engine* eng);

@@ -510,6 +510,8 @@ struct Model {
     std::vector<uint32_t> setup_order; // processing position at setup -> input index
     std::vector<uint32_t> off;         // A+1 block offsets into processing order
     int pp_attr = 0;                   // state attribute of per-particle (lambda, mu); 0: none
+    int rigid_attr = 0;                // state attribute of the rigid flag (1: rigid); 0: none
+    double rigid_v[3] = {0, 0, 0};     // velocity of the rigid particles
 };
 
 inline int AttrCount(int kind) {
@@ -699,10 +701,19 @@ void ApplyBoundary(const Engine& e, const int* cell, double* v) {
     for (int k = 0; k < 3; ++k) v[k] -= vn * n[k] / nn;
 }
 
+// The max speed leaves out nodes holding less than 2^-20 of the lightest particle's
+// mass, as the engine does (its fixed-point P2G gives such a node's momentum as a few
+// units, so p / m there can be far off; particles see the node only with that weight).
+// The velocity field itself is the reference's at every node.
 void GridUpdate(Engine& e, double dt) {
     const Layout& L = e.cur;
     e.V.assign(static_cast<size_t>(L.Nb) * 384, 0.0);
     double vmax = 0.0;
+    double mmin = 0.0;
+    for (const Model& m : e.models) mmin = (mmin == 0.0 || m.mass < mmin) ? m.mass : mmin;
+    const double floor_mass = std::ldexp(static_cast<double>(static_cast<float>(mmin)), -20);
+    uint32_t flags = 0;
+#pragma omp parallel for schedule(static) reduction(max : vmax) reduction(| : flags)
     for (uint32_t i = 0; i < L.Nb; ++i) {
         const uint32_t key = L.keys[i];
         int b[3];
@@ -730,11 +741,12 @@ void GridUpdate(Engine& e, double dt) {
             for (int a = 0; a < 3; ++a) dst[128 * a + n] = v[a];
             if (std::isnan(v2)) {
                 v2 = INFINITY;
-                e.flags |= kFlagNonFinite;
+                flags |= kFlagNonFinite;
             }
-            vmax = std::max(vmax, v2);
+            if (mass >= floor_mass || std::isinf(v2)) vmax = std::max(vmax, v2);
         }
     }
+    e.flags |= flags;
     e.max_v2 = vmax;
 }
 
@@ -807,12 +819,29 @@ void Conservation(Engine& e, int mi, double* out) {
 //   window check on the new cell (:1669-1683) -> flag and skip P2G;
 //   D' at the new x, D^ = (0.5 D')^-1, K = m C - nextDt V D^ tau (:1687-1736);
 //   P2G m w, w (m v + K (x_i - x_p)) on both grids (:1739-1792).
+// Parallel form (OpenMP): the blocks are split into 8 colours by the parity of their
+// coordinates and the colours run one after the other.  Two blocks of one colour are at
+// least two blocks apart, so their P2G windows (the block and its +1 neighbours) are
+// disjoint and the blocks of a colour run concurrently without sharing a node; each
+// node receives its contributions in a fixed order (by colour, then list order), so the
+// result does not depend on the thread count.
 void Physics(Engine& e, double dt, double next_dt) {
     const Layout& L = e.cur;
     e.Mnext.assign(static_cast<size_t>(L.Nb) * 512, 0.0);
+    std::vector<uint32_t> colour[8];
+    for (uint32_t j = 0; j < L.A; ++j) {
+        int bb[3];
+        CoordsOf(e.dom, L.keys[j], bb);
+        colour[(bb[0] & 1) * 4 + (bb[1] & 1) * 2 + (bb[2] & 1)].push_back(j);
+    }
+    uint32_t flags = 0;
     for (Model& m : e.models) {
         m.st_next.assign(m.st.size(), 0.0);
-        for (uint32_t j = 0; j < L.A; ++j) {
+        for (int col = 0; col < 8; ++col) {
+            const std::vector<uint32_t>& list = colour[col];
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : flags)
+            for (size_t q = 0; q < list.size(); ++q) {
+            const uint32_t j = list[q];
             int bb[3];
             CoordsOf(e.dom, L.keys[j], bb);
             for (uint32_t t = m.off[j]; t < m.off[j + 1]; ++t) {
@@ -830,7 +859,7 @@ void Physics(Engine& e, double dt, double next_dt) {
                         const uint32_t bi = NodeBlock(e, L, st.c[0] + d3[0], st.c[1] + d3[1], st.c[2] + d3[2], local);
                         double vi[3] = {0, 0, 0};
                         if (bi == kEmpty)
-                            e.flags |= kFlagMissingBlock;
+                            flags |= kFlagMissingBlock;
                         else
                             for (int a = 0; a < 3; ++a) vi[a] = e.V[static_cast<size_t>(bi) * 384 + 128 * a + g * 64 + local];
                         double xx[3];
@@ -847,11 +876,19 @@ void Physics(Engine& e, double dt, double next_dt) {
                 double Dinv[9], C[9];
                 Inverse(D, Dinv);
                 MatMul(B, Dinv, C);
+                const size_t Nn = m.n;
+                // rigid particles (GetExistRigidParticle, mpm_algorithm.cuh:1609-1624): no
+                // velocity or velocity gradient from the grid, motion at the prescribed
+                // velocity; the stress of their (unchanged) F is still deposited
+                const bool rigid = m.rigid_attr && m.st[m.rigid_attr * Nn + s] != 0.0;
+                if (m.rigid_attr) m.st_next[m.rigid_attr * Nn + t] = m.st[m.rigid_attr * Nn + s];
                 for (int a = 0; a < 3; ++a) {
                     v[a] *= 0.5;
-                    x[a] += dt * v[a];
+                    if (rigid) v[a] = 0.0;
+                    x[a] += dt * (rigid ? m.rigid_v[a] : v[a]);
                 }
-                const size_t Nn = m.n;
+                if (rigid)
+                    for (double& c : C) c = 0.0;
                 const double lam = m.pp_attr ? m.st[m.pp_attr * Nn + s] : m.mat.lambda;
                 const double mu = m.pp_attr ? m.st[(m.pp_attr + 1) * Nn + s] : m.mat.mu;
                 if (m.pp_attr) {
@@ -887,7 +924,7 @@ void Physics(Engine& e, double dt, double next_dt) {
                     out |= (loc <= 0) || (loc >= 7);
                 }
                 if (out) {
-                    e.flags |= kFlagOutOfWindow;
+                    flags |= kFlagOutOfWindow;
                     continue;
                 }
                 double Dn[9] = {0};
@@ -916,7 +953,7 @@ void Physics(Engine& e, double dt, double next_dt) {
                         int local;
                         const uint32_t bi = NodeBlock(e, L, st.c[0] + d3[0], st.c[1] + d3[1], st.c[2] + d3[2], local);
                         if (bi == kEmpty) {
-                            e.flags |= kFlagMissingBlock;
+                            flags |= kFlagMissingBlock;
                             continue;
                         }
                         double xx[3];
@@ -932,8 +969,10 @@ void Physics(Engine& e, double dt, double next_dt) {
                     }
                 }
             }
+            }
         }
     }
+    e.flags |= flags;
     e.physics_pending = true;
 }
 
@@ -1080,6 +1119,21 @@ int orc_add_model(orc_engine* h, int kind, const double* params, double mass, do
 // InitialSetup (mpm_engine.cuh:215-383): particle blocks from FP32 positions,
 // ascending key; particles stably sorted by block; neighbour / exterior layout;
 // rasterize.  Returns the flags.
+// Rigid particles: the first `count` particles of model mi (input order) carry the
+// rigid flag; before orc_initial_setup (the flag becomes one more state attribute), and
+// the velocity may change at any time.
+void orc_set_rigid(orc_engine* h, int mi, uint32_t count, const double* v) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    Model& m = e->models[mi];
+    if (count && !m.rigid_attr) {
+        m.rigid_attr = m.nattr;
+        m.nattr += 1;
+        m.st.resize(static_cast<size_t>(m.nattr) * m.n, 0.0);
+        for (uint32_t p = 0; p < std::min(count, m.n); ++p) m.st[m.rigid_attr * static_cast<size_t>(m.n) + p] = 1.0;
+    }
+    for (int a = 0; a < 3; ++a) m.rigid_v[a] = v[a];
+}
+
 uint32_t orc_initial_setup(orc_engine* h) {
     Engine* e = reinterpret_cast<Engine*>(h);
     std::vector<uint32_t> keys;
@@ -1144,6 +1198,18 @@ uint32_t orc_initial_setup(orc_engine* h) {
 uint32_t orc_physics(orc_engine* h, double dt, double next_dt) {
     Engine* e = reinterpret_cast<Engine*>(h);
     GridUpdate(*e, dt);
+    Physics(*e, dt, next_dt);
+    return e->flags;
+}
+// The two halves of orc_physics, for the frame loop (mpm_engine.cuh:517-590), which
+// chooses the next dt from the grid update's max |v| before the particles move.
+uint32_t orc_grid_update(orc_engine* h, double dt) {
+    Engine* e = reinterpret_cast<Engine*>(h);
+    GridUpdate(*e, dt);
+    return e->flags;
+}
+uint32_t orc_particles(orc_engine* h, double dt, double next_dt) {
+    Engine* e = reinterpret_cast<Engine*>(h);
     Physics(*e, dt, next_dt);
     return e->flags;
 }
@@ -1224,6 +1290,9 @@ void orc_set_grid_v(orc_engine* h, const double* V) {
 
 uint32_t orc_flags(orc_engine* h) { return reinterpret_cast<Engine*>(h)->flags; }
 void orc_clear_flags(orc_engine* h) { reinterpret_cast<Engine*>(h)->flags = 0; }
+// OpenMP threads of the physics and grid update (the results do not depend on it)
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads() { return omp_get_max_threads(); }
 double orc_max_v2(orc_engine* h) { return reinterpret_cast<Engine*>(h)->max_v2; }
 
 // which: 0 = current layout, 1 = previous layout (the one M is indexed by)

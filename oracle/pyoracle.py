@@ -34,6 +34,7 @@ KATs or fixtures for this path, so the pins are mathematical):
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from typing import List, Optional
 
@@ -69,6 +70,16 @@ def lib():
             getattr(L, name).restype = ctypes.c_uint32
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.orc_clear_flags.argtypes = [ctypes.c_void_p]
+        L.orc_set_rigid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, _f64p]
+        L.orc_set_rigid.restype = None
+        L.orc_grid_update.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_grid_update.restype = ctypes.c_uint32
+        L.orc_particles.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_particles.restype = ctypes.c_uint32
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_max_threads.argtypes = []
+        L.orc_max_threads.restype = ctypes.c_int
         for name in ("orc_physics", "orc_substep"):
             getattr(L, name).restype = ctypes.c_uint32
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
@@ -171,8 +182,10 @@ class Oracle:
     @classmethod
     def from_scene(cls, scene) -> "Oracle":
         o = cls(scene.dim, scene.dx, scene.gravity, scene.mass_clamp, scene.block_capacity(), scene.offset)
-        for m in scene.models:
+        for i, m in enumerate(scene.models):
             o.add_model(m.material.kind, m.material.params(), m.mass, m.volume, m.x, m.v, m.lam_p, m.mu_p)
+            if getattr(m, "rigid_count", 0):
+                o.set_rigid(i, m.rigid_count, m.rigid_velocity)
         if getattr(scene, "boundary", None) is not None:
             o.set_boundary(scene.boundary)
         return o
@@ -202,6 +215,10 @@ class Oracle:
         self.nmodels += 1
         return idx
 
+    def set_rigid(self, m: int, count: int, velocity) -> None:
+        v = np.ascontiguousarray(velocity, np.float64).reshape(3)
+        lib().orc_set_rigid(self.h, int(m), int(count), _p(v, _f64p))
+
     def setup(self) -> int:
         return lib().orc_initial_setup(self.h)
 
@@ -217,6 +234,31 @@ class Oracle:
 
     def substep(self, dt, next_dt) -> int:
         return lib().orc_substep(self.h, float(dt), float(next_dt))
+
+    def run_frames(self, frames: int, fps: float, cfl: float, dt_default: float, dx: float, state=None):
+        """The reference's frame loop (mpm_engine.cuh:514-876): per frame of 1/fps, substeps
+        until the frame's end; each substep's grid update runs with the current dt, then
+        next_dt = min(dt_default, time left in the frame, dx cfl / max|v|) and the particles
+        move with (dt, next_dt); the first substep of a run has dt = 0 (:115,178,552-557).
+        `state` = (time, dt) carried between calls; returns (state, dt of each substep)."""
+        t, cur = state if state is not None else (0.0, 0.0)
+        dts = []
+        for _ in range(frames):
+            frame_end = t + 1.0 / fps
+            while t < frame_end:
+                lib().orc_grid_update(self.h, cur)
+                vmax = math.sqrt(self.max_v2)
+                if math.isinf(vmax):
+                    raise FloatingPointError("velocity reached infinity")
+                nxt = min(dt_default, frame_end - t)
+                if vmax > 0.0:
+                    nxt = min(nxt, dx * cfl / vmax)
+                lib().orc_particles(self.h, cur, nxt)
+                self.rebin()
+                dts.append(cur)
+                cur = nxt
+                t += cur
+        return (t, cur), dts
 
     @property
     def flags(self) -> int:

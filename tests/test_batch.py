@@ -37,6 +37,25 @@ def test_batch_scenes_are_seeded_and_within_spec():
         assert cell.min() >= 8 and cell.max() < g - 8   # clear of the sticky band
 
 
+def _graphs_captured(runs, fused):
+    """Graphs a batch captures over runs of the given substep counts: the first substep of
+    a batch runs eagerly (dt = 0), then one graph of two substeps per buffer parity, and in
+    fused mode one graph of a single substep per parity for odd remainders (the other
+    modes run those eagerly)."""
+    done, graphs = 0, set()
+    for n in runs:
+        if done == 0 and n:
+            done, n = 1, n - 1
+        if n // 2:
+            graphs.add(("pair", done & 1))
+            done += 2 * (n // 2)
+        if n & 1:
+            if fused:
+                graphs.add(("single", done & 1))
+            done += 1
+    return len(graphs)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("substeps", [7, 10])
 @pytest.mark.parametrize("mode", [2, 3, 4])
@@ -54,7 +73,8 @@ def test_graph_batch_matches_separate_engines(substeps, mode):
     batch.Run(substeps, mode)
     batch.Synchronize()
     per_capture = len(scs) if mode == MPMBatch.ENGINE_GRAPHS else 1
-    assert batch.captures == per_capture
+    fused = mode == MPMBatch.FUSED
+    assert batch.captures == per_capture * _graphs_captured([substeps], fused)
     for a, b, sc in zip(alone, together, scs):
         ka, ca = a.layout(0)
         kb, cb = b.layout(0)
@@ -65,12 +85,12 @@ def test_graph_batch_matches_separate_engines(substeps, mode):
         np.testing.assert_array_equal(pa, pb)
         np.testing.assert_array_equal(oa, ob)
         np.testing.assert_allclose(b.state(0), a.state(0), rtol=0, atol=1e-5)
-    # later runs reuse the graph of their substep parity (one per parity at most)
+    # later runs reuse the graphs of their substep parity (one per parity and kind at most)
     batch.Run(4, mode)
     batch.Run(5, mode)
     batch.Run(4, mode)
     batch.Synchronize()
-    assert batch.captures == 2 * per_capture
+    assert batch.captures == per_capture * _graphs_captured([substeps, 4, 5, 4], fused)
     batch.close()
 
 
@@ -122,7 +142,7 @@ def test_fused_batch_variants_match_separate_engines():
     batch = MPMBatch(together)
     batch.Run(substeps, MPMBatch.FUSED)
     batch.Synchronize()
-    assert batch.captures == 1
+    assert batch.captures == _graphs_captured([substeps], True)
     for a, b in zip(alone, together):
         ka, ca = a.layout(0)
         kb, cb = b.layout(0)
@@ -147,3 +167,76 @@ def test_batch_rejects_an_engine_twice():
     with pytest.raises(CkmpmError, match="ckmpm_batch_create"):
         MPMBatch([e, e])
     e.close()
+
+
+@pytest.mark.gpu
+def test_fused_batch_of_64_scenes_matches_separate_engines_and_the_oracle():
+    """The 64-scene workload (20k-500k particles per scene, every material, grids
+    64^3-256^3) in fused mode: 6 substeps (the first eager, two fused pairs, one fused
+    single substep) equal separate engines (block lists and orderings bit-exact, state
+    within 1e-5); then the smallest scene of each material is imported into the oracle
+    before each of two more fused single substeps (one per buffer parity) and compared
+    with it in lockstep: grid within 1e-5 per substep, state, and the rebin of the GPU's
+    FP32 positions bit-exact."""
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+    from oracle.pyoracle import Oracle
+    from tests.test_gpu_parity import _compare_layout, _compare_ordering, _grid_close, _import
+
+    scs = scenes.batch_scenes()
+    assert len(scs) == 64
+    together = [engine_from_scene(sc) for sc in scs]
+    batch = MPMBatch(together)
+    substeps = 6
+    print(f"64 scenes, {batch.n_particles} particles: {substeps} fused substeps", flush=True)
+    batch.Run(substeps, MPMBatch.FUSED)
+    batch.Synchronize()
+    assert batch.captures == _graphs_captured([substeps], True) == 2   # a pair graph, a single-substep graph
+    for i, (b, sc) in enumerate(zip(together, scs)):
+        a = engine_from_scene(sc)
+        for k in range(substeps):
+            a.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+        a.Synchronize()
+        ka, ca = a.layout(0)
+        kb, cb = b.layout(0)
+        assert ca == cb, i
+        np.testing.assert_array_equal(ka, kb, err_msg=f"scene {i}")
+        pa, oa = a.ordering(0)
+        pb, ob = b.ordering(0)
+        np.testing.assert_array_equal(pa, pb, err_msg=f"scene {i}")
+        np.testing.assert_array_equal(oa, ob, err_msg=f"scene {i}")
+        np.testing.assert_allclose(b.state(0), a.state(0), rtol=0, atol=1e-5, err_msg=f"scene {i}")
+        assert b.counters().flags == 0, i
+        a.close()
+    print("separate engines agree", flush=True)
+    picks = {}
+    for i, sc in enumerate(scs):
+        k = sc.models[0].material.kind
+        if k not in picks or sc.n_particles < scs[picks[k]].n_particles:
+            picks[k] = i
+    assert sorted(picks) == [scenes.FIXED_COROTATED, scenes.NEO_HOOKEAN, scenes.DRUCKER_PRAGER, scenes.FLUID]
+    orcs = {}
+    for i in picks.values():
+        orcs[i] = Oracle.from_scene(scs[i])
+        orcs[i].setup()
+    for step in range(2):
+        for i, o in orcs.items():
+            _import(together[i], o, 1)
+            o.physics(scs[i].dt, scs[i].dt)
+        batch.Run(1, MPMBatch.FUSED)
+        batch.Synchronize()
+        for i, o in orcs.items():
+            e = together[i]
+            print(f"oracle lockstep, fused single substep {step}: scene {i} "
+                  f"(kind {scs[i].models[0].material.kind}, {scs[i].n_particles} particles)", flush=True)
+            _grid_close(e.grid_m(), o.grid_mnext())
+            sg, so = e.global_state(0), o.state_next(0)
+            np.testing.assert_allclose(sg[:3], so[:3], rtol=0, atol=2e-7)
+            if sg.shape[0] > 3:
+                np.testing.assert_allclose(sg[3:], so[3:], rtol=0, atol=2e-5)
+            o.rebin([e.state(0)[:3].T.copy()])
+            _compare_layout(e, o, 0)
+            _compare_layout(e, o, 1)
+            _compare_ordering(e, o, 1)
+            assert o.flags == 0
+    assert batch.captures == _graphs_captured([substeps, 1, 1], True) == 3
+    batch.close()

@@ -106,9 +106,9 @@ def _ids_after(eng_or_orc, ids, nm, is_gpu):
     return out
 
 
-def test_colliding_spheres_trajectory_100_substeps():
-    """Independent FP32 GPU and FP64 oracle runs; positions within 1e-4 dx after 100 substeps."""
-    sc = scenes.colliding_spheres()
+def _trajectory(sc, substeps):
+    """Independent FP32 GPU and FP64 oracle runs of a scene; returns per model the final
+    positions of both, in input order."""
     nm = len(sc.models)
     eng = engine_from_scene(sc)
     orc = Oracle.from_scene(sc)
@@ -117,7 +117,7 @@ def test_colliding_spheres_trajectory_100_substeps():
     for m in range(nm):  # the GPU's setup order is the oracle's: same state in the same slots
         np.testing.assert_allclose(eng.global_state(m)[:3], orc.state(m)[:3], rtol=0, atol=1e-7)
     ids_g = [i.copy() for i in ids_o]
-    for k in range(100):
+    for k in range(substeps):
         dt = 0.0 if k == 0 else sc.dt
         ids_g = _ids_after(eng, ids_g, nm, True)
         ids_o = _ids_after(orc, ids_o, nm, False)
@@ -125,6 +125,7 @@ def test_colliding_spheres_trajectory_100_substeps():
         orc.substep(dt, sc.dt)
     eng.Synchronize()
     assert orc.flags == 0 and eng.counters().flags == 0
+    out = []
     for m in range(nm):
         xg = eng.global_state(m)[:3].T
         xo = orc.state(m)[:3].T
@@ -132,6 +133,30 @@ def test_colliding_spheres_trajectory_100_substeps():
         po = np.empty_like(xo)
         pg[ids_g[m]] = xg
         po[ids_o[m]] = xo
+        out.append((pg, po))
+    return out
+
+
+def test_mixed_fluid_sand_trajectory_100_substeps():
+    """The mixed workload shrunk 4x per axis (78k fluid + 47k Drucker-Prager sand
+    particles, 128^3 grid, so that most of the sand cone sits above the 8-cell sticky floor
+    band): independent FP32 GPU and FP64 oracle runs, positions within 1e-4 dx after 100
+    substeps, both materials having moved."""
+    sc = scenes.mixed_dam_sand(n_fluid=5_000_000 // 64, n_sand=3_000_000 // 64, scale=4)
+    for m, (pg, po) in enumerate(_trajectory(sc, 100)):
+        err = np.abs(pg - po).max()
+        print(f"  model {m}: max position error {err:.3e} (bar {1e-4 * sc.dx:.3e})")
+        assert err <= 1e-4 * sc.dx, f"model {m}: max position error {err:.3e} > {1e-4 * sc.dx:.3e}"
+        # both materials move: the parts above the sticky floor band fall (99 substeps of
+        # free fall would be 0.5 g t^2 = 1.9e-5)
+        moved = np.abs(po - sc.models[m].x).max()
+        assert moved > 0.5 * 0.5 * 9.8 * (99 * sc.dt) ** 2, moved
+
+
+def test_colliding_spheres_trajectory_100_substeps():
+    """Independent FP32 GPU and FP64 oracle runs; positions within 1e-4 dx after 100 substeps."""
+    sc = scenes.colliding_spheres()
+    for m, (pg, po) in enumerate(_trajectory(sc, 100)):
         err = np.abs(pg - po).max()
         assert err <= 1e-4 * sc.dx, f"model {m}: max position error {err:.3e} > {1e-4 * sc.dx:.3e}"
         # the spheres moved: 99 substeps at 1 m/s
@@ -260,7 +285,7 @@ def test_prestressed_fluid_global_deposits_then_fixed_point():
 
 def test_jelly_1m_full_size_lockstep_with_hash_occupancy():
     """The 1M-particle Neo-Hookean jelly (a 50^3-cell cube at 8 ppc on a 256^3 grid,
-    64^3 blocks) at full size for 4 substeps: block lists and particle order bit-exact,
+    64^3 blocks) at full size for 20 substeps: block lists and particle order bit-exact,
     grid and state within the bars above, and the hash table built over each substep's
     exterior block list slot for slot equal to the oracle's."""
     from ckmpm_amd.hashtable import BlockHashTable
@@ -280,7 +305,7 @@ def test_jelly_1m_full_size_lockstep_with_hash_occupancy():
         np.testing.assert_array_equal(slots, ho.slots, err_msg=f"substep {k}")
         np.testing.assert_array_equal(stash, ho.stash[:ho.nstash], err_msg=f"substep {k}")
 
-    _lockstep(sc, 4, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5), after=hash_check)
+    _lockstep(sc, 20, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5), after=hash_check)
     table.close()
 
 
@@ -291,6 +316,43 @@ def test_mixed_8m_full_size_lockstep():
     sc = scenes.mixed_dam_sand()
     assert sc.n_particles == 8_000_000
     _lockstep(sc, 2, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+
+
+def test_mixed_8m_lockstep_after_the_benchmark_run():
+    """The benchmark scene after the 200 substeps the benchmark times (run on the GPU
+    alone), then 10 substeps in lockstep with the oracle, while the fluid column falls and
+    its particles cross block boundaries (hundreds per substep): block lists and particle
+    order bit-exact, grid and state within the bars above."""
+    sc = scenes.mixed_dam_sand()
+    eng = engine_from_scene(sc)
+    for k in range(200):
+        eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+    eng.Synchronize()
+    orc = Oracle.from_scene(sc)
+    orc.setup()
+    nm = len(sc.models)
+    crossed = [0] * nm   # particles that changed block, over the 10 substeps
+    for k in range(10):
+        print(f"  lockstep substep {200 + k}", flush=True)
+        _import(eng, orc, nm)
+        orc.physics(sc.dt, sc.dt)
+        eng.Substep(sc.dt, sc.dt)
+        eng.Synchronize()
+        _grid_close(eng.grid_m(), orc.grid_mnext())
+        for m in range(nm):
+            sg, so = eng.global_state(m), orc.state_next(m)
+            np.testing.assert_allclose(sg[:3], so[:3], rtol=0, atol=2e-7)
+            np.testing.assert_allclose(sg[3:], so[3:], rtol=0, atol=2e-5)
+        orc.rebin([eng.state(m)[:3].T.copy() for m in range(nm)])
+        _compare_layout(eng, orc, 0)
+        _compare_layout(eng, orc, 1)
+        _compare_ordering(eng, orc, nm)
+        for m in range(nm):
+            crossed[m] += int(((eng.ordering(m)[0] >> 27) != 13).sum())
+        assert orc.flags == 0
+    print(f"  block crossings over the 10 substeps: fluid {crossed[0]}, sand {crossed[1]}")
+    assert crossed[0] > 1000
+    assert eng.counters().flags == 0
 
 
 def test_mixed_models_lockstep():
@@ -345,3 +407,54 @@ def test_workgroup_sizes_agree_and_autotune():
     assert all(a.tuning(m) in (256, 512) for m in range(2))
     a.close()
     b.close()
+
+
+def test_frame_loop_matches_the_oracle_frame_loop():
+    """ckmpm_run_frames against the oracle's frame loop (mpm_engine.cuh:514-876), two
+    frames at 240 fps of spheres flying at 10 m/s with a default dt of 1e-3, so that the
+    CFL rule dx cfl / max|v| sets the dt: the same number of substeps, and positions
+    within 1e-4 dx as point sets (the frame loop runs its substeps inside one call, so
+    particle identities are not followed: the Hausdorff distance between the two sets is
+    checked, every particle of each set having one of the other within the bar)."""
+    import dataclasses
+
+    from scipy.spatial import cKDTree
+
+    sc = dataclasses.replace(scenes.colliding_spheres(radius=0.06, speed=10.0), dt=1e-3)
+    fps, cfl = 240, 0.5
+    eng = engine_from_scene(sc, fps=fps, cfl=cfl)
+    orc = Oracle.from_scene(sc)
+    orc.setup()
+    eng.Simulate(2)
+    eng.Synchronize()
+    # the default dt: the configured one, capped by the CFL bound of the initial speeds
+    # (mpm_engine.cuh:109-117)
+    v0 = max(float(np.abs(m.v).max()) for m in sc.models)
+    _, dts = orc.run_frames(2, fps, cfl, min(sc.dt, sc.dx * cfl / v0), sc.dx)
+    assert max(dts) < 0.9 * sc.dt        # the CFL rule is active
+    assert eng.counters().substeps == len(dts), (eng.counters().substeps, len(dts))
+    assert eng.counters().flags == 0 and orc.flags == 0
+    for m in range(len(sc.models)):
+        xg = eng.global_state(m)[:3].T
+        xo = orc.state(m)[:3].T
+        assert xg.shape == xo.shape
+        d = max(cKDTree(xg).query(xo)[0].max(), cKDTree(xo).query(xg)[0].max())
+        print(f"  model {m}: Hausdorff distance {d:.3e} (bar {1e-4 * sc.dx:.3e})")
+        assert d <= 1e-4 * sc.dx
+        assert np.abs(xo[:, 0].mean() - sc.models[m].x[:, 0].mean()) > 0.02   # moved 2 frames at 10 m/s
+
+
+def test_rigid_particles_lockstep():
+    """A von Mises box whose first 600 particles are rigid and move at 0.4 m/s through the
+    rest (mpm_algorithm.cuh:1609-1624): lockstep with the oracle for 8 substeps at the
+    usual bars; the rigid particles' flag travels with them through the rebins."""
+    from tests.test_oracle_engine import _rigid_box
+
+    sc = _rigid_box(600)
+    eng, orc = _lockstep(sc, 8, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+    st = eng.state(0)
+    assert st.shape[0] == 13 and int(st[12].sum()) == 600
+    # a new velocity takes effect from the next substep; the count is fixed after setup
+    eng.set_rigid(0, 600, (0.0, 0.0, 0.0))
+    with pytest.raises(Exception):
+        eng.set_rigid(0, 10, (0.0, 0.0, 0.0))

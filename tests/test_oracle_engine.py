@@ -12,6 +12,8 @@
   in the stress term pulls it inward);
 * rebin and setup: independent numpy restatements of the canonical orderings.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -236,3 +238,80 @@ def test_setup_order_matches_numpy_restatement():
         # consecutive particles of a block sit in distinct cells while the block has
         # unused cells left for that rank
         assert (np.diff(ck[got]) != 0).mean() > 0.95
+
+
+def test_oracle_frame_loop_dt_rules():
+    """The oracle's frame loop (mpm_engine.cuh:514-876): the first substep moves nothing
+    (dt = 0); the time advances by each substep's next_dt, which is at most the default dt
+    and ends the frame exactly; at 0.1 m/s the CFL bound dx cfl / max|v| is far and the
+    steps are the default dt, at 10 m/s it sets them."""
+    from ckmpm_amd import scenes
+
+    fps, cfl = 240, 0.5
+    for speed, cfl_active in ((0.1, False), (10.0, True)):
+        sc = dataclasses.replace(scenes.colliding_spheres(radius=0.06, speed=speed), dt=1e-3)
+        o = Oracle.from_scene(sc)
+        o.setup()
+        state = None
+        for frame in range(2):
+            state, dts = o.run_frames(1, fps, cfl, sc.dt, sc.dx, state)
+            if frame == 0:
+                assert dts[0] == 0.0
+            steps = dts[1:] + [state[1]]   # the time increments of this frame
+            assert abs(sum(steps) - 1.0 / fps) < 1e-12
+            assert abs(state[0] - (frame + 1) / fps) < 1e-12
+            assert all(0.0 < d <= sc.dt for d in steps)
+            if cfl_active:
+                vmax = speed * 1.05
+                assert all(d < 0.9 * sc.dt for d in steps) and min(steps[:-1]) > 0.5 * sc.dx * cfl / vmax
+            else:
+                assert steps[:-1] == [sc.dt] * (len(steps) - 1)
+
+
+def _rigid_box(rigid_count, velocity=(0.4, 0.0, 0.0)):
+    """A von Mises box (12^3 blocks, 2^3 particles per cell) at rest under gravity whose
+    first rigid_count particles are rigid."""
+    from ckmpm_amd import scenes
+
+    dim = 12
+    dx = 1.0 / (4 * dim)
+    rng = scenes.Splitmix(44)
+    g = np.stack(np.meshgrid(*[np.arange(16, 30)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = scenes._stratified_cells(rng, g.astype(np.float64), dx, 2).astype(np.float32)
+    lam, mu = scenes.lame(2e5, 0.3)
+    vol = dx ** 3 / 8
+    m = scenes.ModelSpec(scenes.Material(scenes.VON_MISES, lam, mu, yield_stress=30.0), 1000 * vol, vol, x,
+                         np.zeros((1, 3), np.float32), rigid_count=rigid_count, rigid_velocity=velocity)
+    return scenes.Scene("rigid_box", (dim, dim, dim), (0, 0, 0), dx, -9.8, 2e-5, 10, [m])
+
+
+def test_oracle_rigid_particles_move_at_their_velocity():
+    """Rigid particles (mpm_algorithm.cuh:1609-1624): they take no velocity from the grid
+    and move by dt * velocity exactly, their F stays I; the other particles fall under
+    gravity; without rigid particles every particle falls."""
+    from ckmpm_amd.native import SLOT_MASK
+
+    sc = _rigid_box(600)
+    o = Oracle.from_scene(sc)
+    o.setup()
+    ids = o.setup_order(0).astype(np.int64)
+    steps = 6
+    for k in range(steps + 1):
+        ids = ids[o.perm(0) & SLOT_MASK]
+        o.substep(0.0 if k == 0 else sc.dt, sc.dt)
+    st = o.state(0)
+    x = np.empty((3, st.shape[1]))
+    x[:, ids] = st[:3]
+    F = np.empty((9, st.shape[1]))
+    F[:, ids] = st[3:12]
+    flag = np.empty(st.shape[1])
+    flag[ids] = st[12]          # von Mises state: x, F, then the rigid flag
+    x0 = sc.models[0].x.T.astype(np.float64)
+    rigid = np.arange(st.shape[1]) < 600
+    assert (flag[rigid] == 1).all() and (flag[~rigid] == 0).all()
+    np.testing.assert_allclose(x[:, rigid] - x0[:, rigid],
+                               np.outer([0.4, 0.0, 0.0], np.ones(rigid.sum())) * steps * sc.dt, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(F[:, rigid], np.tile(np.eye(3).reshape(9, 1), (1, rigid.sum())), rtol=0, atol=0)
+    assert (x[1, ~rigid] - x0[1, ~rigid]).mean() < -0.4 * 9.8 * (steps * sc.dt) ** 2
+    # the flag is the state's last attribute
+    assert st.shape[0] == 13
