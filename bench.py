@@ -11,8 +11,11 @@ bytes per substep = sum_m N_m (2 S_m + 8) + 3584 G (state read + write, old-slot
 move-code write per particle; velocity halo read 1536 B and mass/momentum write 2048 B
 per grid block), divided by its device time measured with hipEvents on the engine
 stream.  substep_hbm: the whole-substep byte model sum_m N_m (2 S_m + 12) + 7168 G.
-cpu_baseline: the FP64 oracle, single thread, on the same scene shrunk 4x per axis
-(1/64 of the particles).
+fwd_bwd_over_fwd and batched_over_sequential: the other two ratios of the headline metric,
+timed in the same run (cantilever with checkpoints every 25 substeps; the 64-scene fused
+batch against sequential launches).  cpu_baseline: on the same scene shrunk 4x per axis
+(1/64 of the particles), the FP64 oracle with OpenMP on all available cores (value), on one
+core, and a NumPy restatement of the reference's Python demo substep.
 """
 import argparse
 import json
@@ -27,9 +30,32 @@ HBM_PEAK_GBS = 8000.0
 STATE_BYTES = {0: 48, 1: 48, 2: 52, 3: 16}   # x + F | x + F + logJp | x + J
 
 
-def cpu_baseline(workload: str, seconds: float = 12.0):
+def host_cpu() -> dict:
+    """CPU model and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(workload: str, seconds: float = 8.0):
+    """Three CPU baselines on the same sample of the workload (about `seconds` each):
+    B1 the FP64 oracle on one core, B2 the same oracle with OpenMP on all the cores it
+    may use (the headline CPU figure), B3 the NumPy restatement of the reference's Python
+    CK-MPM demo substep (its own fixed-corotated material on its 256^3 grid, on the same
+    particle positions)."""
+    import numpy as np
+
     from ckmpm_amd import scenes
-    from oracle.pyoracle import Oracle
+    from oracle.pic_numpy import PicDemo
+    from oracle.pyoracle import Oracle, lib
 
     if workload == "mixed_dam_sand":
         sc = scenes.mixed_dam_sand(n_fluid=5_000_000 // 64, n_sand=3_000_000 // 64, scale=4)
@@ -37,19 +63,46 @@ def cpu_baseline(workload: str, seconds: float = 12.0):
     else:
         sc = scenes.SCENES[workload]()
         sample = f"{workload}, {sc.n_particles} particles"
-    o = Oracle.from_scene(sc)
-    o.setup()
-    o.substep(0.0, sc.dt)
-    n = 0
-    t0 = time.perf_counter()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+
+    def oracle_rate(nthreads):
+        lib().orc_set_threads(nthreads)
+        o = Oracle.from_scene(sc)
+        o.setup()
+        o.substep(0.0, sc.dt)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            o.substep(sc.dt, sc.dt)
+            n += 1
+            el = time.perf_counter() - t0
+            if el > seconds:
+                break
+        return sc.n_particles * n / el, n, el
+
+    r1, n1, e1 = oracle_rate(1)
+    r2, n2, e2 = oracle_rate(threads)
+    pic = PicDemo(np.concatenate([m.x for m in sc.models]))
+    n3, t0 = 0, time.perf_counter()
     while True:
-        o.substep(sc.dt, sc.dt)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > seconds:
+        pic.substep(1e-4)
+        n3 += 1
+        e3 = time.perf_counter() - t0
+        if e3 > seconds:
             break
-    return {"value": sc.n_particles * n / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{sample}; {n} substeps of the FP64 oracle in {el:.1f} s on one host core"}
+    r3 = pic.x.shape[0] * n3 / e3
+    unit = "particle-substeps/s"
+    return {"value": r2, "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{sample}; {n2} substeps of the FP64 oracle (OpenMP, {threads} threads) in {e2:.1f} s",
+            "host_cpu": host_cpu(),
+            "variants": {
+                "B1_oracle_1_thread": {"value": r1, "unit": unit, "cores": 1, "kind": "port",
+                                       "sample": f"{sample}; {n1} substeps in {e1:.1f} s"},
+                "B2_oracle_openmp": {"value": r2, "unit": unit, "cores": threads, "kind": "port",
+                                     "sample": f"{sample}; {n2} substeps in {e2:.1f} s"},
+                "B3_python_demo_numpy": {"value": r3, "unit": unit, "cores": 1, "kind": "port",
+                                         "sample": f"NumPy restatement of python-src/python_demo.py:101-215 (CK-MPM, "
+                                                   f"fixed-corotated E 5e3, 256^3 grid, dt 1e-4) on the same "
+                                                   f"{pic.x.shape[0]} positions; {n3} substeps in {e3:.1f} s"}}}
 
 
 def copy_bandwidth(torch, mib: int = 1024, reps: int = 5) -> float:
@@ -72,19 +125,39 @@ def copy_bandwidth(torch, mib: int = 1024, reps: int = 5) -> float:
     return 2.0 * n * 4 / best / 1e9
 
 
+KERNEL_SOURCES = ["kernels.hip", "kernels.h", "material.h", "svd.h", "stencil.h", "common.h"]
+
+
+def kernel_source_sha16() -> str:
+    """Hash of the G2P2G kernel sources: ties a traffic measurement to the code it read."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "ckmpm_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _measured_traffic(path: str, workload: str):
     """HBM bytes per G2P2G launch pair for this workload, as measured with rocprofv3
-    FETCH_SIZE + WRITE_SIZE and scaled by a plain copy's reading (None if absent)."""
+    FETCH_SIZE + WRITE_SIZE and scaled by a plain copy's reading (None if absent), and
+    whether that measurement was taken on the current kernel sources."""
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get(workload)
+    e = t.get(workload)
+    if e is not None:
+        e = dict(e)
+        e["current"] = e.get("kernel_source_sha16") == kernel_source_sha16()
+    return e
 
 
-def bench_batch(args, torch):
-    """64 independent scenes: hipGraph batch (mode 2) vs one scene after the other."""
+def bench_batch(args, torch, only=None):
+    """64 independent scenes: fused hipGraph batch vs one scene after the other (and the
+    other batch modes unless `only` names the ones to run)."""
     from ckmpm_amd import scenes
     from ckmpm_amd.engine import MPMBatch, engine_from_scene
 
@@ -102,6 +175,8 @@ def bench_batch(args, torch):
     modes = (("batched", MPMBatch.FUSED), ("engine_graphs", MPMBatch.ENGINE_GRAPHS), ("single_graph", MPMBatch.GRAPH),
              ("streams", MPMBatch.STREAMS), ("sequential", MPMBatch.SEQUENTIAL))
     for name, mode in modes:
+        if only is not None and name not in only:
+            continue
         b = MPMBatch(engines)
         b.Run(args.warmup, mode)
         b.Synchronize()
@@ -315,6 +390,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="mixed_dam_sand")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the fwd+bwd and batched ratios that the default run also times")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB three times (HBM counter calibration)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -414,12 +491,32 @@ def main():
     }
     traffic = _measured_traffic(args.traffic, sc.name)
     if traffic is not None:
-        out["roofline"]["traffic"] = traffic["bytes_per_launch_pair"]
+        # a measurement of other kernel sources is reported beside, not as the traffic
+        out["roofline"]["traffic"] = traffic["bytes_per_launch_pair"] if traffic["current"] else None
         out["roofline"]["traffic_source"] = traffic["source"]
+        out["roofline"]["traffic_kernel_source_sha16"] = traffic.get("kernel_source_sha16")
+        if not traffic["current"]:
+            out["roofline"]["traffic_measured_on_other_sources"] = traffic["bytes_per_launch_pair"]
+    out["roofline"]["kernel_source_sha16"] = kernel_source_sha16()
     eng.close()
     copy_gbs = copy_bandwidth(torch)
     out["roofline"]["copy_gbs"] = copy_gbs
     out["roofline"]["frac_of_copy"] = achieved / copy_gbs
+    if args.workload == "mixed_dam_sand" and not args.no_extras:
+        # the other two ratios of the headline metric, timed in the same run: forward +
+        # backward against forward on the differentiation workload (checkpoints every
+        # 25 substeps) and the fused 64-scene batch against sequential single-scene launches
+        ad = bench_adjoint(args, torch)
+        out["fwd_bwd_over_fwd"] = ad["fwd_bwd_over_fwd"]
+        out["fwd_bwd"] = {"workload": "cantilever", "particles": ad["config"]["particles"],
+                          "fwd_ms_per_step": ad["fwd"]["ms_per_step"], "fwd_bwd_ms_per_step": ad["ms_per_step"],
+                          "checkpoint_every": 25, "timing": ad["timing"]}
+        bargs = argparse.Namespace(**{**vars(args), "steps": 40, "warmup": 4})
+        bt = bench_batch(bargs, torch, only=("batched", "sequential"))
+        out["batched_over_sequential"] = bt["batched_over_sequential"]
+        out["batched"] = {"workload": "batch64", "scenes": bt["config"]["scenes"], "particles": bt["config"]["particles"],
+                          "batched_ms_per_step": bt["ms_per_step"], "sequential_ms_per_step": bt["sequential"]["ms_per_step"],
+                          "value": bt["value"], "substeps": 40}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
