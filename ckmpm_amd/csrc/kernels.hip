@@ -978,12 +978,17 @@ __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint3
             const uint32_t rn = min(end - r0, kRoundParticles);
             const uint32_t nch = (rn + 63u) >> 6;
             const bool last = r0 + rn == end;
+            // the ordering word of the wave's next chunk is loaded a chunk ahead, so the
+            // state loads of a chunk wait for one memory latency instead of two
+            uint32_t pe_next = 0u;
+            if (wave < nch && wave * 64u + lane < rn) pe_next = a.perm[r0 + wave * 64u + lane];
             for (uint32_t c = wave; c < nch; c += kWaves) {
                 const uint32_t t = r0 + c * 64u + lane;
                 const bool active = t < r0 + rn;
+                const uint32_t pe = pe_next;
+                if (c + kWaves < nch && (c + kWaves) * 64u + lane < rn) pe_next = a.perm[t + kWaves * 64u];
                 uint32_t dir = 31u;
                 if (active) {
-                    const uint32_t pe = a.perm[t];
                     uint32_t s = pe & kSlotMask;
                     int mv = static_cast<int>(pe >> kMoveShift);
                     if (s >= a.n || mv >= kDirections) {
