@@ -70,7 +70,7 @@ struct FusedRec {
         p->grids.push_back(grid);
     }
 };
-enum Stage : int { kStGrid, kStG2P2G, kStCount, kStScan, kStBase, kStScatter, kStMark0, kStEmit0, kStMark1, kStEmit1 };
+enum Stage : int { kStGrid, kStG2P2G, kStCount, kStScan, kStScatter, kStMark, kStEmit };
 
 // Launch now on s, or record for a fused batch.
 template <class T>
@@ -80,19 +80,16 @@ void Run(FusedRec* rec, int stage, int kernel, int variant, const T& a, uint32_t
 }
 
 // Neighbour and exterior extension of layout L (its A keys already written, and in
-// the table unless insert_a), on stream s.
+// the table unless insert_a), on stream s.  The map tile counts must be zero: a substep's
+// rebin count kernel clears them, the setup before its call.
 void ExtendLayout(ckmpm_engine* e, LayoutBuf& L, bool insert_a, hipStream_t s, FusedRec* rec = nullptr) {
     if (insert_a) LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
     // marking grids sized for the expected block count (the capacity before the first read)
     const uint32_t hint = e->a_hint ? e->a_hint + e->a_hint / 4 + 64u : e->cap;
-    for (int pass = 0; pass < 2; ++pass) {
-        uint32_t* cnt = e->tile_cnt + pass * e->emit_tiles;
-        uint32_t* other = e->tile_cnt + (1 - pass) * e->emit_tiles;
-        const MarkArgs m{e->dom, L.keys, L.lc, e->cap, L.table, e->bitmap, cnt, pass == 0 ? 0 : -1};
-        Run(rec, kStMark0 + 2 * pass, kFkMark, 0, m, MarkGrid(e->cap, hint, pass), s);
-        const EmitArgs em{e->bitmap, cnt, other, e->emit_tiles, L.keys, L.table, L.lc, e->cap, pass, e->status};
-        Run(rec, kStEmit0 + 2 * pass, kFkEmit, 0, em, e->emit_tiles, s);
-    }
+    const MarkArgs m{e->dom, L.keys, L.lc, e->cap, L.table, e->ext_map, e->ext_cnt};
+    Run(rec, kStMark, kFkMark, 0, m, MarkGrid(e->cap, hint), s);
+    const EmitArgs em{e->ext_map, e->ext_cnt, e->ext_tiles, L.keys, L.table, L.lc, e->cap, e->status};
+    Run(rec, kStEmit, kFkEmit, 0, em, e->ext_tiles, s);
 }
 
 void Mark(ckmpm_engine* e) {
@@ -238,26 +235,24 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
         rm.n[i] = m.n;
     }
     // exterior blocks are about 1.4 A; the kernels loop over any count
-    const RebinCountArgs ca{e->dom, C.keys, C.table, C.lc, rm, e->occ, e->tsum, P.keys, P.lc, P.table, e->cap};
+    const RebinCountArgs ca{e->dom, C.keys, C.table, C.lc, rm, e->tsum, P.keys, P.lc, P.table, e->cap,
+                            e->ext_cnt, 2u * e->ext_tiles};
     Run(rec, kStCount, kFkRebinCount, 0, ca, RebinCountGrid(e->cap, 2u * e->a_hint + 64u), s);
     const RebinScanArgs sa{C.keys, C.lc, rm, e->tsum, e->newidx, P.keys, P.table, P.lc, e->cap, e->status};
     Run(rec, kStScan, kFkRebinScan, 0, sa, RebinScanGrid(e->cap), s);
-    // Two independent chains from here: the particle permutation (RebinBase, Scatter:
-    // current layout, orderings) and the extension of the next layout (its neighbour and
-    // exterior blocks, bitmap, tile counts).  For large layouts the extension runs on the
-    // side stream: the overlap saves ~24 us per substep at 17k particle blocks.
+    // Two independent chains from here: the particle permutation (RebinScatter: current
+    // layout, orderings) and the extension of the next layout (its neighbour and exterior
+    // blocks, map, tile counts).  For large layouts the extension runs on the side stream,
+    // beside the scatter.
     const bool fork = !rec && SideStreamReady(e, s);
     if (fork) {
         (void)hipEventRecord(e->fork_ev, s);
         (void)hipStreamWaitEvent(e->side, e->fork_ev, 0);
         ExtendLayout(e, P, false, e->side);
     }
-    const RebinBaseArgs ba{e->dom, C.keys, C.table, C.lc, e->occ, e->newidx, rm, e->cap};
-    Run(rec, kStBase, kFkRebinBase, 0, ba, RebinBaseGrid(e->cap, 2u * e->a_hint + 64u), s);
-    // a wavefront per (model, block), 4 per workgroup
-    const RebinScatterArgs xa{C.lc, rm};
-    Run(rec, kStScatter, kFkRebinScatter, 0, xa,
-        GridFor((e->a_hint + e->a_hint / 4) * static_cast<uint32_t>(rm.nmodels) / 4u + 8u), s);
+    // a wavefront per particle block, 4 per workgroup
+    const RebinScatterArgs xa{e->dom, C.keys, C.table, C.lc, rm, e->newidx, e->cap};
+    Run(rec, kStScatter, kFkRebinScatter, 0, xa, GridFor((e->a_hint + e->a_hint / 4) / 4u + 8u), s);
     if (fork) {
         (void)hipEventRecord(e->join_ev, e->side);
         (void)hipStreamWaitEvent(s, e->join_ev, 0);
@@ -322,12 +317,12 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
     check(Alloc(e, &e->V, static_cast<size_t>(e->cap) * kVFloats));
     check(Alloc(e, &e->status, 1));
     check(Alloc(e, &e->bc, 1));
-    e->emit_tiles = (e->nwords + kEmitTileWords - 1) / kEmitTileWords;
-    check(Alloc(e, &e->bitmap, static_cast<size_t>(e->emit_tiles) * kEmitTileWords));
-    check(Alloc(e, &e->tile_cnt, 2 * static_cast<size_t>(e->emit_tiles)));
+    check(Alloc(e, &e->bitmap, e->nwords));
+    e->ext_tiles = ExtTiles(nblocks);
+    check(Alloc(e, &e->ext_map, static_cast<size_t>(e->ext_tiles) * kEmitTileWords));
+    check(Alloc(e, &e->ext_cnt, 2 * static_cast<size_t>(e->ext_tiles)));
     check(Alloc(e, &e->tsum, RebinTileSumWords(e->cap)));
     check(Alloc(e, &e->word_off, e->nwords));
-    check(Alloc(e, &e->occ, e->cap));
     check(Alloc(e, &e->newidx, e->cap));
     check(Alloc(e, &e->total, 4));
     check(Alloc(e, &e->scan_tmp, ScanTempWords(std::max<size_t>(e->cap, e->nwords))));
@@ -336,9 +331,9 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
             (void)hipMemsetAsync(e->L[i].table, 0xFF, nblocks * sizeof(uint32_t), e->stream);
             (void)hipMemsetAsync(e->L[i].lc, 0, sizeof(LayoutCounts), e->stream);
         }
-        (void)hipMemsetAsync(e->bitmap, 0, static_cast<size_t>(e->emit_tiles) * kEmitTileWords * sizeof(uint32_t),
+        (void)hipMemsetAsync(e->bitmap, 0, static_cast<size_t>(e->nwords) * sizeof(uint32_t), e->stream);
+        (void)hipMemsetAsync(e->ext_map, 0, static_cast<size_t>(e->ext_tiles) * kEmitTileWords * sizeof(uint32_t),
                              e->stream);
-        (void)hipMemsetAsync(e->tile_cnt, 0, 2 * static_cast<size_t>(e->emit_tiles) * sizeof(uint32_t), e->stream);
         (void)hipMemsetAsync(e->status, 0, sizeof(Status), e->stream);
         (void)hipMemsetAsync(e->bc, 0, sizeof(BoundaryDev), e->stream);
         if (hipStreamSynchronize(e->stream) != hipSuccess) rc = Fail(e, CKMPM_ERR_HIP, "init");
@@ -493,6 +488,7 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
     LaunchBitmapToKeysA(e->bitmap, e->nwords, e->word_off, e->total, L.keys, L.lc, e->cap, e->status, s);
     for (size_t i = 0; i < e->models.size(); ++i)
         LaunchSetupOffsets(sorted[i], e->models[i].n, L.keys, L.lc, e->cap, e->models[i].off[0], s);
+    CK_HIP(e, hipMemsetAsync(e->ext_cnt, 0, 2 * static_cast<size_t>(e->ext_tiles) * sizeof(uint32_t), s));
     ExtendLayout(e, L, true, s);
     CK_HIP(e, hipMemsetAsync(e->M[0], 0, static_cast<size_t>(e->cap) * kMFloats * sizeof(float), s));
     for (ModelBuf& m : e->models) {
@@ -979,7 +975,7 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
 // runs once for all scenes, its workgroups split by scene (LaunchFused), from argument
 // tables built here on the host and copied once.  The stages run in order on the capture
 // stream, so every engine's launches keep their order (independent launches run side by
-// side: the G2P2G launches of a substep, and base + scatter beside the extension passes);
+// side: the G2P2G launches of a substep, and the scatter beside the layout extension);
 // the scenes' buffers are disjoint.
 // Captures nsub (1 or 2) substeps of every engine as fused launches.
 int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
@@ -1070,14 +1066,14 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
     };
     BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
     for (size_t j = 0; j < launches.size();) {
-        if (launches[j].stage == kStBase) {
-            // after the scan, two independent chains: base + scatter (particle permutation)
-            // on the capture stream, the layout extension's passes on an auxiliary stream
+        if (launches[j].stage == kStScatter) {
+            // after the scan, two independent chains: the scatter (particle permutation)
+            // on the capture stream, the layout extension on an auxiliary stream
             size_t k = j;
-            while (k < launches.size() && launches[k].stage >= kStBase && launches[k].stage <= kStEmit1) ++k;
+            while (k < launches.size() && launches[k].stage >= kStScatter && launches[k].stage <= kStEmit) ++k;
             (void)hipEventRecord(b->fork, b->cs);
             (void)hipStreamWaitEvent(b->aux[0], b->fork, 0);
-            for (size_t q = j; q < k; ++q) launch(launches[q], launches[q].stage >= kStMark0 ? b->aux[0] : b->cs);
+            for (size_t q = j; q < k; ++q) launch(launches[q], launches[q].stage >= kStMark ? b->aux[0] : b->cs);
             (void)hipEventRecord(b->aux_done[0], b->aux[0]);
             (void)hipStreamWaitEvent(b->cs, b->aux_done[0], 0);
             j = k;

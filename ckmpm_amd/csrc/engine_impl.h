@@ -72,13 +72,13 @@ struct ckmpm_engine {
     Status* status = nullptr;
     BoundaryDev* bc = nullptr;   // grid boundary description (custom = 0: the default rule)
     bool bc_custom = false;      // host copy of bc->custom
-    uint32_t* bitmap = nullptr;
+    uint32_t* bitmap = nullptr;       // setup: one bit per domain block
     uint32_t nwords = 0;
     uint32_t* word_off = nullptr;
-    uint32_t emit_tiles = 0;          // bitmap tiles of kEmitTileWords words
-    uint32_t* tile_cnt = nullptr;     // [2][emit_tiles] newly marked bits per tile and pass
+    uint32_t* ext_map = nullptr;      // layout extension: two bits per domain block
+    uint32_t ext_tiles = 0;           // map tiles of kEmitTileWords words
+    uint32_t* ext_cnt = nullptr;      // [ext_tiles][2] neighbour / exterior blocks marked per tile
     uint32_t* tsum = nullptr;         // rebin count tile sums
-    uint32_t* occ = nullptr;
     uint32_t* newidx = nullptr;
     uint32_t* scan_tmp = nullptr;
     uint32_t* total = nullptr;

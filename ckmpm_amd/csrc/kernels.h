@@ -56,8 +56,8 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
                      const uint32_t* table, const LayoutCounts* lc, float* M, Status* st, hipStream_t s);
 
 // ---------------------------------------------------------------- layout
-// Bitmap words per emit tile; the domain bitmap is allocated in whole tiles (zero
-// padded) and each extension pass has emit_tiles counters of newly set bits.
+// Map words per emit tile: the extension map is allocated in whole tiles (zero padded),
+// with two counters per tile (neighbour and exterior blocks marked).
 constexpr uint32_t kEmitTileWords = 2048;
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
@@ -127,7 +127,9 @@ struct RebinModels {
     int nmodels;
     const uint32_t* cnt[kMaxModels];   // [A_cur][27] direction counts per old block
     uint32_t* cnt_e[kMaxModels];       // [Ex_cur] particles entering each exterior block
-    uint32_t* base[kMaxModels];        // [A_cur][27] first destination of (old block, direction)
+    // [Ex_cur][27] per exterior block e and move d: particles of the model entering e
+    // with a move before d
+    uint32_t* base[kMaxModels];
     uint32_t* off_next[kMaxModels];    // [A_new + 1] block offsets of the next layout
     const uint32_t* off_cur[kMaxModels];
     const uint32_t* code[kMaxModels];  // [n] move | rank << 5 per processing position
@@ -138,27 +140,28 @@ struct RebinModels {
 constexpr int kTileSums = 1 + kMaxModels;
 size_t RebinTileSumWords(uint32_t cap);
 
-// Rebin (RebinCount: entering counts per exterior block of the current layout, occ,
-// cnt_e, tile sums, and the next layout's table cleared; RebinScan: next layout's
-// particle blocks (keys, table, A) and every model's block offsets; RebinBase: first
-// destination of each (old block, move); RebinScatter: the next particle permutation)
-// and layout extension (Mark: candidates b + {0,1}^3 (lo = 0) or b + {-1,0,1}^3 (lo = -1)
-// not in the table, counted per emit tile; Emit: the marked blocks in ascending key
-// order after A (pass 0, sets Nb) or Nb (pass 1, sets Ex), registered in the table, the
-// bitmap cleared; tile_cnt holds 2 x emit_tiles counters, zero on the first pass, and
-// each pass clears the other's).
+// Rebin (RebinCount: entering counts per exterior block of the current layout, cnt_e,
+// tile sums, the earlier-move counts of every (exterior block, move), the next
+// layout's table cleared and the extension tile counts zeroed; RebinScan: next
+// layout's particle blocks (keys, table, A) and every model's block offsets;
+// RebinScatter: the next particle permutation) and layout extension (Mark: neighbour
+// candidates b + {0,1}^3 and exterior candidates b + {-1,0,1}^3 not in the table, in a
+// map of two bits per block, counted per tile; Emit: the neighbour blocks in ascending
+// key order after A (sets Nb), then the exterior blocks after Nb (sets Ex), registered in
+// the table, the map cleared).  ext_cnt holds 2 counters per map tile, zero before Mark.
 struct RebinCountArgs {
     DomainDev d;
     const uint32_t* keys;
     const uint32_t* table;
     const LayoutCounts* lc;
     RebinModels rm;
-    uint32_t* occ;
     uint32_t* tsum;
     const uint32_t* keys_next;
     const LayoutCounts* lc_next;
     uint32_t* table_next;
     uint32_t cap;
+    uint32_t* ext_cnt;       // cleared by workgroup 0 for the extension that follows
+    uint32_t ext_cnt_words;
 };
 struct RebinScanArgs {
     const uint32_t* keys;
@@ -172,40 +175,34 @@ struct RebinScanArgs {
     uint32_t cap;
     Status* st;
 };
-struct RebinBaseArgs {
+struct RebinScatterArgs {
     DomainDev d;
-    const uint32_t* keys;
+    const uint32_t* keys;    // current layout
     const uint32_t* table;
     const LayoutCounts* lc;
-    const uint32_t* occ;
-    const uint32_t* newidx;
     RebinModels rm;
+    const uint32_t* newidx;  // next-layout index of each occupied exterior block
     uint32_t cap;
 };
-struct RebinScatterArgs {
-    const LayoutCounts* lc;
-    RebinModels rm;
-};
+// Extension map: two bits per domain block, kExtKeysPerWord blocks per word.
+constexpr uint32_t kExtKeysPerWord = 16;
 struct MarkArgs {
     DomainDev d;
     const uint32_t* keys;
     const LayoutCounts* lc;
     uint32_t cap;
     const uint32_t* table;
-    uint32_t* bitmap;
-    uint32_t* tile_cnt;
-    int lo;
+    uint32_t* map;
+    uint32_t* tile_cnt;  // [tiles][2]: neighbour, exterior blocks newly marked per tile
 };
 struct EmitArgs {
-    uint32_t* bitmap;
+    uint32_t* map;
     const uint32_t* tile_cnt;
-    uint32_t* tile_cnt_other;
     uint32_t ntiles;
     uint32_t* keys;
     uint32_t* table;
     LayoutCounts* lc;
     uint32_t cap;
-    int pass;
     Status* st;
 };
 
@@ -216,7 +213,6 @@ enum FusedKernel : int {
     kFkG2P2G,         // variant: kind * 2 + (threads == 512)
     kFkRebinCount,
     kFkRebinScan,
-    kFkRebinBase,
     kFkRebinScatter,
     kFkMark,
     kFkEmit,
@@ -231,13 +227,14 @@ void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStre
 // buffers.
 void LaunchFused(int kernel, int variant, const void* table, const uint32_t* begin, uint32_t n, uint32_t total,
                  hipStream_t s);
-// Workgroup counts: RebinCount and RebinBase for about ex_hint exterior blocks, Mark for
-// about a_hint particle blocks (the kernels loop over the device counts, so any hint is
-// correct; a close one saves idle workgroups); RebinScan must cover the capacity.
+// Workgroup counts: RebinCount for about ex_hint exterior blocks, Mark for about a_hint
+// particle blocks (the kernels loop over the device counts, so any hint is correct; a
+// close one saves idle workgroups); RebinScan must cover the capacity.
 uint32_t RebinCountGrid(uint32_t cap, uint32_t ex_hint);
 uint32_t RebinScanGrid(uint32_t cap);
-uint32_t RebinBaseGrid(uint32_t cap, uint32_t ex_hint);
-uint32_t MarkGrid(uint32_t cap, uint32_t a_hint, int pass);
+uint32_t MarkGrid(uint32_t cap, uint32_t a_hint);
+// map tiles (one emit workgroup each) and words for a domain of nblocks blocks
+uint32_t ExtTiles(uint64_t nblocks);
 uint32_t G2P2GGrid(uint32_t grid);
 
 }  // namespace ckmpm

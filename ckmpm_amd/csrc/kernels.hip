@@ -380,48 +380,75 @@ __global__ void TableInsertAKernel(const uint32_t* keys, const LayoutCounts* lc,
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) table[keys[i]] = i;
 }
 
-// Neighbour (lo = 0: b + {0,1}^3) or exterior (lo = -1: b + {-1,0,1}^3) candidates of
-// the particle blocks that are not yet in the table: mpm_algorithm.cuh:657-695.  Each
-// bit this kernel sets is counted in tile_cnt[word / kEmitTileWords], so that the emit
-// kernel knows every tile's offset without a separate scan pass over the bitmap.  The
-// counts are aggregated per wavefront (its candidates span one to three tiles): one
-// atomic per distinct tile instead of one per bit, which would serialize on the few
-// counter addresses.
+// offset of move d (d = (ox + 1) * 9 + (oy + 1) * 3 + oz + 1)
+__device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
+    ox = d / 9 - 1;
+    oy = (d / 3) % 3 - 1;
+    oz = d % 3 - 1;
+}
+
+// Neighbour (b + {0,1}^3) and exterior (b + {-1,0,1}^3) candidates of the particle blocks
+// that are not in the table, in one pass over the 27 offsets (mpm_algorithm.cuh:657-695
+// marks them in two passes, the exterior one after the neighbours are registered).
+// Block key k owns bits 2 (k % 16) (neighbour mark) and 2 (k % 16) + 1 (exterior mark)
+// of map word k / 16: a block with the neighbour mark is a neighbour block, one with
+// only the exterior mark an exterior block.  Each tile of kEmitTileWords map words
+// counts both kinds (tile_cnt[2t], tile_cnt[2t+1]) as marks are set: the atomic OR on
+// the key's word orders the marks of one block, so the first neighbour mark that finds
+// the exterior mark set takes that block's exterior count back, and an exterior mark
+// counts only where neither mark was set.  The emit kernel then knows every tile's
+// offsets without a separate scan pass.  The counts are aggregated per wavefront (its
+// candidates span one to three tiles): one atomic per distinct tile and kind instead of
+// one per mark, which would serialize on the few counter addresses.
 // wg / ng: workgroup index and workgroup count of the launch (the scene's part of a fused
 // launch); the same holds for every kernel body below.
 __device__ __forceinline__ void MarkExtensionBody(const MarkArgs& a, uint32_t wg, uint32_t ng) {
     const DomainDev& d = a.d;
     const uint32_t* keys = a.keys;
     const uint32_t* table = a.table;
-    uint32_t* bitmap = a.bitmap;
+    uint32_t* map = a.map;
     uint32_t* tile_cnt = a.tile_cnt;
-    const int lo = a.lo;
     const uint32_t A = min(a.lc->A, a.cap);
-    const int span = 2 - lo;
-    const int per = span * span * span;
-    const uint32_t total = A * static_cast<uint32_t>(per);
+    const uint32_t total = A * static_cast<uint32_t>(kDirections);
     for (uint32_t q = wg * kThreads + threadIdx.x; q < total; q += ng * kThreads) {
-        const uint32_t i = q / per;
-        const int o = static_cast<int>(q % per);
-        int bx, by, bz;
+        const uint32_t i = q / static_cast<uint32_t>(kDirections);
+        const int o = static_cast<int>(q - i * static_cast<uint32_t>(kDirections));
+        int bx, by, bz, ox, oy, oz;
         CoordsOf(d, keys[i], bx, by, bz);
-        bx += o / (span * span) + lo;
-        by += (o / span) % span + lo;
-        bz += o % span + lo;
-        uint32_t tile = kEmpty;  // tile of a bit set by this lane
+        DirOffset(o, ox, oy, oz);
+        const bool nbr = ox >= 0 && oy >= 0 && oz >= 0;
+        bx += ox;
+        by += oy;
+        bz += oz;
+        uint32_t tile = kEmpty;  // tile of a count changed by this lane
+        int dn = 0, de = 0;      // neighbour and exterior count changes
         if (InDomain(d, bx, by, bz)) {
             const uint32_t key = KeyOf(d, bx, by, bz);
             if (table[key] == kEmpty) {
-                const uint32_t bit = 1u << (key & 31u);
-                if (!(atomicOr(&bitmap[key >> 5], bit) & bit)) tile = (key >> 5) / kEmitTileWords;
+                const uint32_t sh = 2u * (key % kExtKeysPerWord);
+                const uint32_t old = atomicOr(&map[key / kExtKeysPerWord], (nbr ? 1u : 2u) << sh) >> sh;
+                if (nbr) {
+                    dn = (old & 1u) ? 0 : 1;
+                    de = (dn && (old & 2u)) ? -1 : 0;
+                } else {
+                    de = (old & 3u) ? 0 : 1;
+                }
+                if (dn || de) tile = key / kExtKeysPerWord / kEmitTileWords;
             }
         }
         uint64_t pending = __ballot(tile != kEmpty);
         while (pending) {
             const int leader = __builtin_ctzll(pending);
             const uint32_t t = __shfl(tile, leader, 64);
-            const uint64_t same = __ballot(tile == t);
-            if (LaneId() == static_cast<uint32_t>(leader)) atomicAdd(&tile_cnt[t], static_cast<uint32_t>(__popcll(same)));
+            const bool mine = tile == t;
+            const uint64_t same = __ballot(mine);
+            const uint32_t nn = static_cast<uint32_t>(__popcll(__ballot(mine && dn > 0)));
+            const uint32_t ep = static_cast<uint32_t>(__popcll(__ballot(mine && de > 0)));
+            const uint32_t em = static_cast<uint32_t>(__popcll(__ballot(mine && de < 0)));
+            if (LaneId() == static_cast<uint32_t>(leader)) {
+                if (nn) atomicAdd(&tile_cnt[2u * t], nn);
+                if (ep != em) atomicAdd(&tile_cnt[2u * t + 1u], ep - em);  // modulo 2^32
+            }
             pending &= ~same;
         }
     }
@@ -447,71 +474,87 @@ __device__ __forceinline__ void BlockSums(const uint32_t (&v)[NQ], int nq, uint3
     __syncthreads();
 }
 
-// Extension pass (0: neighbours after A, sets Nb; 1: exterior after Nb, sets Ex) in one
-// kernel: workgroup w owns bitmap words [w, w+1) * kEmitTileWords, takes its first
-// position from the tile counts of the marking kernel, scans its words' popcounts and
-// writes the set bits in ascending key order to keys[] and the table, clearing the
-// bitmap.  Tiles without a set bit return at once.  Workgroup 0 also sets the count
-// and clears the other pass's tile counts (zero again before that pass marks).
+// Layout extension in one kernel: workgroup w owns map words [w, w+1) * kEmitTileWords,
+// takes its first positions from the mark kernel's tile counts (neighbour blocks follow
+// A, exterior blocks follow all neighbour blocks), scans its words' mark counts and
+// writes the blocks of each kind in ascending key order to keys[] and the table,
+// clearing the map.  Tiles without a mark return at once.  Workgroup 0 sets Nb and Ex.
 __device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t wg, uint32_t) {
     __shared__ uint32_t sh[kThreads / 64];
-    __shared__ uint32_t red[1];
-    uint32_t* bitmap = a.bitmap;
+    __shared__ uint32_t red[4];
+    uint32_t* map = a.map;
     const uint32_t* tile_cnt = a.tile_cnt;
-    uint32_t* tile_cnt_other = a.tile_cnt_other;
     const uint32_t ntiles = a.ntiles, cap = a.cap;
     uint32_t* keys = a.keys;
     uint32_t* table = a.table;
     LayoutCounts* lc = a.lc;
-    const int pass = a.pass;
     Status* st = a.st;
     const uint32_t w = wg, tid = threadIdx.x;
-    const uint32_t base = pass == 0 ? lc->A : lc->Nb;
-    if (w == 0) {
-        uint32_t v[1] = {0u};
-        for (uint32_t t = tid; t < ntiles; t += kThreads) v[0] += tile_cnt[t];
-        BlockSums<1>(v, 1, red);
-        if (tid == 0) {
-            uint32_t c = base + red[0];
-            if (c > cap) {
-                SetFlag(st, kFlagCapacity);
-                c = cap;
-            }
-            if (pass == 0) lc->Nb = c;
-            else lc->Ex = c;
+    const bool marked = tile_cnt[2u * w] != 0u || tile_cnt[2u * w + 1u] != 0u;
+    if (w != 0 && !marked) return;
+    // neighbour total; neighbour and exterior blocks of the tiles before w; exterior total
+    uint32_t v[4] = {0u, 0u, 0u, 0u};
+    for (uint32_t t = tid; t < ntiles; t += kThreads) {
+        const uint32_t cn = tile_cnt[2u * t], ce = tile_cnt[2u * t + 1u];
+        v[0] += cn;
+        v[3] += ce;
+        if (t < w) {
+            v[1] += cn;
+            v[2] += ce;
         }
-        for (uint32_t t = tid; t < ntiles; t += kThreads) tile_cnt_other[t] = 0u;
     }
-    if (tile_cnt[w] == 0u) return;
-    uint32_t v[1] = {0u};
-    for (uint32_t t = tid; t < w; t += kThreads) v[0] += tile_cnt[t];
-    BlockSums<1>(v, 1, red);
+    BlockSums<4>(v, 4, red);
+    const uint32_t A = lc->A;
+    uint32_t nb = A + red[0];
+    if (nb > cap) nb = cap;
+    if (w == 0 && tid == 0) {
+        uint32_t ex = nb + red[3];
+        if (A + red[0] > cap || ex > cap) {
+            SetFlag(st, kFlagCapacity);
+            ex = min(ex, cap);
+        }
+        lc->Nb = nb;
+        lc->Ex = ex;
+    }
+    if (!marked) return;
     const uint32_t w0 = w * kEmitTileWords + tid * kEmitItems;
-    const uint4* src = reinterpret_cast<const uint4*>(bitmap + w0);
+    const uint4* src = reinterpret_cast<const uint4*>(map + w0);
     const uint4 qa = src[0], qb = src[1];
     const uint32_t words[kEmitItems] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-    uint32_t pc = 0;
-#pragma unroll
-    for (int k = 0; k < kEmitItems; ++k) pc += static_cast<uint32_t>(__popc(words[k]));
-    uint32_t tot;
-    uint32_t pos = base + red[0] + BlockExclusiveScan(pc, tot, sh);
-    if (pc == 0) return;
+    constexpr uint32_t kLow = 0x55555555u;  // the neighbour mark of each block of a word
+    uint32_t pn = 0, pe = 0;
 #pragma unroll
     for (int k = 0; k < kEmitItems; ++k) {
-        uint32_t bits = words[k];
-        if (!bits) continue;
-        bitmap[w0 + k] = 0u;
-        while (bits) {
-            const uint32_t b = static_cast<uint32_t>(__ffs(bits) - 1);
-            bits &= bits - 1u;
-            const uint32_t key = (w0 + k) * 32u + b;
-            if (pos < cap) {
-                keys[pos] = key;
-                table[key] = pos;
-            } else {
-                SetFlag(st, kFlagCapacity);
+        pn += static_cast<uint32_t>(__popc(words[k] & kLow));
+        pe += static_cast<uint32_t>(__popc((words[k] >> 1) & ~words[k] & kLow));
+    }
+    uint32_t tot;
+    uint32_t posn = A + red[1] + BlockExclusiveScan(pn, tot, sh);
+    uint32_t pose = nb + red[2] + BlockExclusiveScan(pe, tot, sh);
+    if (pn + pe == 0) return;
+#pragma unroll
+    for (int k = 0; k < kEmitItems; ++k) {
+        if (!words[k]) continue;
+        map[w0 + k] = 0u;
+        uint32_t bn = words[k] & kLow, be = (words[k] >> 1) & ~words[k] & kLow;
+        const uint32_t key0 = (w0 + k) * kExtKeysPerWord;
+        while (bn) {
+            const uint32_t key = key0 + static_cast<uint32_t>(__ffs(bn) - 1) / 2u;
+            bn &= bn - 1u;
+            if (posn < cap) {
+                keys[posn] = key;
+                table[key] = posn;
             }
-            ++pos;
+            ++posn;
+        }
+        while (be) {
+            const uint32_t key = key0 + static_cast<uint32_t>(__ffs(be) - 1) / 2u;
+            be &= be - 1u;
+            if (pose < cap) {
+                keys[pose] = key;
+                table[key] = pose;
+            }
+            ++pose;
         }
     }
 }
@@ -1332,32 +1375,31 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 }
 
 // ============================================================== rebin
-__device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
-    ox = d / 9 - 1;
-    oy = (d / 3) % 3 - 1;
-    oz = d % 3 - 1;
-}
-
 // Particles entering each exterior block e: sum over the 27 possible source blocks
 // (e - delta, particle blocks of the current layout) of their count for move delta.
-// A 32-lane group per exterior block, lane = direction (27 independent lookups instead
-// of a dependent chain of 27 per thread); a workgroup owns a tile of kCountTile
-// consecutive exterior blocks and writes the tile's sums (occupied blocks, entering
-// particles per model) for the scan kernel.  The same launch clears the next layout's
-// table entries (its keys are still those of two substeps ago).
+// A 32-lane group per exterior block, lane = move (27 independent lookups instead of a
+// dependent chain of 27 per thread); a workgroup owns a tile of kCountTile consecutive
+// exterior blocks and writes the tile's sums (occupied blocks, entering particles per
+// model) for the scan kernel.  A block's particles are ordered by the move that brought
+// them in, so the group also writes base[m][e][d], the particles of model m entering e
+// with a move before d (an exclusive scan over the lanes): the first destination of the
+// particles of source e - d with move d is e's offset in the next layout plus that count.
+// The same launch clears the next layout's table entries (its keys are still those of
+// two substeps ago) and, in workgroup 0, the tile counts of the extension marking.
 __device__ __forceinline__ void RebinCountBody(const RebinCountArgs& a, uint32_t wg, uint32_t ng) {
     __shared__ uint32_t sG[kCountTile][kTileSums];
     const DomainDev& d = a.d;
     const uint32_t* keys = a.keys;
     const uint32_t* table = a.table;
     const RebinModels& rm = a.rm;
-    uint32_t* occ = a.occ;
     uint32_t* tsum = a.tsum;
     const uint32_t cap = a.cap;
     const LayoutCounts* lc = a.lc;
     const uint32_t exn = min(a.lc_next->Ex, cap);
     for (uint32_t i = wg * kCountThreads + threadIdx.x; i < exn; i += ng * kCountThreads)
         a.table_next[a.keys_next[i]] = kEmpty;
+    if (wg == 0)
+        for (uint32_t i = threadIdx.x; i < a.ext_cnt_words; i += kCountThreads) a.ext_cnt[i] = 0u;
     const uint32_t A = lc->A, Ex = min(lc->Ex, cap);
     const uint32_t lane = threadIdx.x & 31u, grp = threadIdx.x >> 5;
     const int nq = 1 + rm.nmodels;
@@ -1376,19 +1418,24 @@ __device__ __forceinline__ void RebinCountBody(const RebinCountArgs& a, uint32_t
             }
             uint32_t any = 0;
             for (int m = 0; m < rm.nmodels; ++m) {
-                uint32_t v = js < A ? rm.cnt[m][static_cast<size_t>(js) * kDirections + lane] : 0u;  // also kEmpty
+                const uint32_t v = js < A ? rm.cnt[m][static_cast<size_t>(js) * kDirections + lane] : 0u;  // also kEmpty
+                // inclusive scan over the group's lanes (lane = move)
+                uint32_t inc = v;
 #pragma unroll
-                for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
-                if (lane == 0) {
-                    rm.cnt_e[m][e] = v;
-                    sG[grp][1 + m] = v;
+                for (int o = 1; o < 32; o <<= 1) {
+                    const uint32_t u = __shfl_up(inc, o, 32);
+                    if (lane >= static_cast<uint32_t>(o)) inc += u;
                 }
-                any |= v;
+                if (lane < static_cast<uint32_t>(kDirections))
+                    rm.base[m][static_cast<size_t>(e) * kDirections + lane] = inc - v;
+                const uint32_t total = __shfl(inc, 31, 32);
+                if (lane == 0) {
+                    rm.cnt_e[m][e] = total;
+                    sG[grp][1 + m] = total;
+                }
+                any |= total;
             }
-            if (lane == 0) {
-                occ[e] = any ? 1u : 0u;
-                sG[grp][0] = any ? 1u : 0u;
-            }
+            if (lane == 0) sG[grp][0] = any ? 1u : 0u;
         } else if (lane < static_cast<uint32_t>(nq)) {
             sG[grp][lane] = 0u;
         }
@@ -1505,84 +1552,57 @@ __device__ __forceinline__ void RebinScanBody(const RebinScanArgs& a, uint32_t w
 }
 __global__ __launch_bounds__(kThreads) void RebinScanKernel(RebinScanArgs a) { RebinScanBody(a, blockIdx.x, gridDim.x); }
 
-// First destination of every (source block, move) pair entering an occupied block:
-// sources are ordered by their list index, so the destination order is the stable
-// order of the previous processing position.  Half a wave (32 lanes) per exterior
-// block: lane d owns move d; the ordering sum is done with lane shuffles.
-__device__ __forceinline__ void RebinBaseBody(const RebinBaseArgs& a, uint32_t wg, uint32_t ng) {
-    const DomainDev& d = a.d;
-    const uint32_t* keys = a.keys;
-    const uint32_t* table = a.table;
-    const uint32_t* occ = a.occ;
-    const uint32_t* newidx = a.newidx;
-    const RebinModels& rm = a.rm;
-    const uint32_t cap = a.cap;
-    const uint32_t A = a.lc->A, Ex = min(a.lc->Ex, cap);
-    const uint32_t lane = LaneId(), half = lane >> 5, dd = lane & 31u;
-    const uint32_t waves = kThreads / 64;
-    const uint32_t step = ng * waves * 2;
-    for (uint32_t e0 = (wg * waves + (threadIdx.x >> 6)) * 2; e0 < Ex; e0 += step) {
-        const uint32_t e = e0 + half;
-        const bool valid = e < Ex && occ[e] != 0u;
-        // the extension blocks at the end of the list are mostly unoccupied: whole waves skip
-        if (!__any(valid)) continue;
-        uint32_t js = kEmpty;
-        if (valid && dd < static_cast<uint32_t>(kDirections)) {
-            int bx, by, bz, ox, oy, oz;
-            CoordsOf(d, keys[e], bx, by, bz);
-            DirOffset(static_cast<int>(dd), ox, oy, oz);
-            const int sx = bx - ox, sy = by - oy, sz = bz - oz;
-            if (InDomain(d, sx, sy, sz)) {
-                const uint32_t v = table[KeyOf(d, sx, sy, sz)];
-                js = v < A ? v : kEmpty;
-            }
-        }
-        const uint32_t ni = valid ? newidx[e] : 0u;
-        for (int m = 0; m < rm.nmodels; ++m) {
-            const uint32_t c = js != kEmpty ? rm.cnt[m][static_cast<size_t>(js) * kDirections + dd] : 0u;
-            uint32_t before = 0;
-            for (int k = 0; k < kDirections; ++k) {
-                const int src = static_cast<int>(half << 5) + k;
-                const uint32_t jk = __shfl(js, src, 64);
-                const uint32_t ck = __shfl(c, src, 64);
-                before += jk < js ? ck : 0u;
-            }
-            if (js != kEmpty && ni < cap) rm.base[m][static_cast<size_t>(js) * kDirections + dd] = rm.off_next[m][ni] + before;
-        }
-    }
-}
-__global__ __launch_bounds__(kThreads) void RebinBaseKernel(RebinBaseArgs a) { RebinBaseBody(a, blockIdx.x, gridDim.x); }
-
-// Particle permutation of the next layout for every model: a wavefront per (model,
-// block) pair, kScatterUnroll particles per lane in flight (a block holds a few hundred
-// particles per model, so most pairs take one pass of dependent loads: 512 per pass).
+// Particle permutation of the next layout: a wavefront per block j of the current
+// layout, for every model.  Lane d < 27 first finds the destination e = j + d (an
+// exterior block of the current layout) and its index in the next layout; per model,
+// the first destination of move d is e's offset there plus base[m][e][d] (the particles
+// entering e with an earlier move), and the particle loop takes it by lane shuffle.
+// kScatterUnroll particles per lane are in flight (a block holds a few hundred
+// particles per model, so most blocks take one pass of dependent loads: 512 per pass).
 constexpr int kScatterUnroll = 8;
 __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint32_t wg, uint32_t ng) {
     const RebinModels& rm = a.rm;
-    const uint32_t A = a.lc->A;
-    const uint32_t pairs = A * static_cast<uint32_t>(rm.nmodels);
+    const uint32_t A = a.lc->A, cap = a.cap;
     const uint32_t lane = LaneId(), waves = kThreads / 64;
-    for (uint32_t p = wg * waves + (threadIdx.x >> 6); p < pairs; p += ng * waves) {
-        const uint32_t m = p / A, j = p - m * A;
-        const uint32_t* off = rm.off_cur[m];
-        const uint32_t beg = off[j], end = off[j + 1];
-        const uint32_t* code = rm.code[m];
-        const uint32_t* bj = rm.base[m] + static_cast<size_t>(j) * kDirections;
-        uint32_t* perm_next = rm.perm_next[m];
-        const uint32_t n = rm.n[m];
-        for (uint32_t t0 = beg + lane; t0 < end; t0 += kScatterUnroll * 64) {
-            uint32_t c[kScatterUnroll];
-#pragma unroll
-            for (int u = 0; u < kScatterUnroll; ++u) {
-                const uint32_t t = t0 + u * 64;
-                c[u] = t < end ? code[t] : 31u;
+    for (uint32_t j = wg * waves + (threadIdx.x >> 6); j < A; j += ng * waves) {
+        uint32_t e = kEmpty, ni = kEmpty;
+        if (lane < static_cast<uint32_t>(kDirections)) {
+            int bx, by, bz, ox, oy, oz;
+            CoordsOf(a.d, a.keys[j], bx, by, bz);
+            DirOffset(static_cast<int>(lane), ox, oy, oz);
+            bx += ox;
+            by += oy;
+            bz += oz;
+            if (InDomain(a.d, bx, by, bz)) {
+                e = a.table[KeyOf(a.d, bx, by, bz)];
+                if (e < cap) ni = a.newidx[e];
             }
+        }
+        for (int m = 0; m < rm.nmodels; ++m) {
+            const uint32_t* off = rm.off_cur[m];
+            const uint32_t beg = off[j], end = off[j + 1];
+            if (beg >= end) continue;
+            const uint32_t* code = rm.code[m];
+            uint32_t* perm_next = rm.perm_next[m];
+            const uint32_t n = rm.n[m];
+            // a move without a destination (no particle takes it) keeps n: its positions drop
+            uint32_t first = n;
+            if (ni < cap) first = rm.off_next[m][ni] + rm.base[m][static_cast<size_t>(e) * kDirections + lane];
+            for (uint32_t t0 = beg; t0 < end; t0 += kScatterUnroll * 64) {
+                uint32_t c[kScatterUnroll];
 #pragma unroll
-            for (int u = 0; u < kScatterUnroll; ++u) {
-                const uint32_t dir = c[u] & 31u;
-                if (dir >= static_cast<uint32_t>(kDirections)) continue;
-                const uint32_t pos = bj[dir] + (c[u] >> 5);
-                if (pos < n) perm_next[pos] = (t0 + u * 64) | (dir << kMoveShift);
+                for (int u = 0; u < kScatterUnroll; ++u) {
+                    const uint32_t t = t0 + u * 64 + lane;
+                    c[u] = t < end ? code[t] : 31u;
+                }
+#pragma unroll
+                for (int u = 0; u < kScatterUnroll; ++u) {
+                    const uint32_t dir = c[u] & 31u;
+                    const uint32_t f = __shfl(first, static_cast<int>(dir), 64);  // every lane active here
+                    const uint32_t pos = f + (c[u] >> 5);
+                    if (dir < static_cast<uint32_t>(kDirections) && pos < n)
+                        perm_next[pos] = (t0 + u * 64 + lane) | (dir << kMoveShift);
+                }
             }
         }
     }
@@ -1720,9 +1740,12 @@ uint32_t G2P2GGrid(uint32_t grid) { return ((grid + 7) / 8) * 8; }
 // one tile per workgroup for up to ex_hint exterior blocks; the loops cover any count
 uint32_t RebinCountGrid(uint32_t cap, uint32_t ex_hint) { return Blocks(std::min(ex_hint, cap), kCountTile, 2048); }
 uint32_t RebinScanGrid(uint32_t cap) { return std::max<uint32_t>(1u, (cap + kRebinScanTile - 1) / kRebinScanTile); }
-uint32_t RebinBaseGrid(uint32_t cap, uint32_t ex_hint) { return Blocks(std::min(cap, ex_hint), 8); }
-uint32_t MarkGrid(uint32_t cap, uint32_t a_hint, int pass) {
-    return Blocks(static_cast<uint64_t>(std::min(cap, a_hint)) * (pass == 0 ? 8 : 27), kThreads);
+uint32_t MarkGrid(uint32_t cap, uint32_t a_hint) {
+    return Blocks(static_cast<uint64_t>(std::min(cap, a_hint)) * kDirections, kThreads);
+}
+uint32_t ExtTiles(uint64_t nblocks) {
+    const uint64_t words = (nblocks + kExtKeysPerWord - 1) / kExtKeysPerWord;
+    return static_cast<uint32_t>((words + kEmitTileWords - 1) / kEmitTileWords);
 }
 
 void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStream_t s) {
@@ -1743,7 +1766,6 @@ void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStre
             RebinCountKernel<<<grid, kCountThreads, 0, s>>>(*static_cast<const RebinCountArgs*>(args));
             break;
         case kFkRebinScan: RebinScanKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinScanArgs*>(args)); break;
-        case kFkRebinBase: RebinBaseKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinBaseArgs*>(args)); break;
         case kFkRebinScatter:
             RebinScatterKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinScatterArgs*>(args));
             break;
@@ -1805,7 +1827,6 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 CKMPM_FUSED(RebinCountFused, RebinCountBody, RebinCountArgs, kCountThreads)
 CKMPM_FUSED(RebinScanFused, RebinScanBody, RebinScanArgs, kThreads)
-CKMPM_FUSED(RebinBaseFused, RebinBaseBody, RebinBaseArgs, kThreads)
 CKMPM_FUSED(RebinScatterFused, RebinScatterBody, RebinScatterArgs, kThreads)
 CKMPM_FUSED(MarkExtensionFused, MarkExtensionBody, MarkArgs, kThreads)
 CKMPM_FUSED(BitmapScanEmitFused, BitmapScanEmitBody, EmitArgs, kThreads)
@@ -1817,7 +1838,6 @@ size_t FusedArgsSize(int kernel) {
         case kFkG2P2G: return sizeof(G2P2GArgs);
         case kFkRebinCount: return sizeof(RebinCountArgs);
         case kFkRebinScan: return sizeof(RebinScanArgs);
-        case kFkRebinBase: return sizeof(RebinBaseArgs);
         case kFkRebinScatter: return sizeof(RebinScatterArgs);
         case kFkMark: return sizeof(MarkArgs);
         default: return sizeof(EmitArgs);
@@ -1857,9 +1877,6 @@ void LaunchFused(int kernel, int variant, const void* table, const uint32_t* beg
             break;
         case kFkRebinScan:
             RebinScanFused<<<total, kThreads, 0, s>>>(static_cast<const RebinScanArgs*>(table), begin, n);
-            break;
-        case kFkRebinBase:
-            RebinBaseFused<<<total, kThreads, 0, s>>>(static_cast<const RebinBaseArgs*>(table), begin, n);
             break;
         case kFkRebinScatter:
             RebinScatterFused<<<total, kThreads, 0, s>>>(static_cast<const RebinScatterArgs*>(table), begin, n);

@@ -18,9 +18,11 @@
 //   * a block list is "previous order with empty blocks removed, then newly
 //     occupied blocks in list order", and the neighbour / exterior extensions are
 //     appended in ascending linear key;
-//   * particle order inside a block is the stable order of the previous slot
-//     (a stable counting sort by new block index); an ordering word is
-//     slot | move << 27, move = (dx+1)*9 + (dy+1)*3 + (dz+1) of the block change.
+//   * particle order inside a block is by the move that brought the particle in, then
+//     by the previous slot (a stable counting sort by (new block index, move)); an
+//     ordering word is slot | move << 27, move = (dx+1)*9 + (dy+1)*3 + (dz+1) of the
+//     block change.  Each move into a block comes from one source block, so the
+//     particles that stay (move 13) keep their previous order.
 // Physics (weights, G2P, F update, stress, P2G, grid update) runs in FP64.
 //
 // Parity pinning: see the header of oracle/pyoracle.py and tests/test_oracle_*.py.
@@ -983,7 +985,7 @@ void Physics(Engine& e, double dt, double next_dt) {
 // order: each particle's new block is its FP32 grid-0 block, clamped to the 27
 // neighbours of its current block and to the domain; the new particle-block list is
 // the current exterior list with unoccupied blocks removed; particles are stably
-// sorted by new block index.  xf[m] holds each model's FP32 positions relative to the
+// sorted by (new block index, move).  xf[m] holds each model's FP32 positions relative to the
 // origin of the block that processed them, in processing order (3 per particle, AoS).
 void Rebin(Engine& e, const std::vector<const float*>& xf) {
     const Layout& L = e.cur;
@@ -1032,13 +1034,16 @@ void Rebin(Engine& e, const std::vector<const float*>& xf) {
     const uint32_t A = static_cast<uint32_t>(akeys.size());
     for (size_t mi = 0; mi < e.models.size(); ++mi) {
         Model& m = e.models[mi];
-        std::vector<uint32_t> cnt(A + 1, 0);
-        for (uint32_t t = 0; t < m.n; ++t) cnt[newidx[target[mi][t]]]++;
+        // counting sort by (new block, move): bucket j * 27 + move
+        std::vector<uint32_t> cnt(static_cast<size_t>(A) * 27 + 1, 0);
+        for (uint32_t t = 0; t < m.n; ++t) cnt[static_cast<size_t>(newidx[target[mi][t]]) * 27 + move[mi][t]]++;
+        std::vector<uint32_t> fill(static_cast<size_t>(A) * 27 + 1, 0);
+        for (size_t q = 0; q < static_cast<size_t>(A) * 27; ++q) fill[q + 1] = fill[q] + cnt[q];
         std::vector<uint32_t> off(A + 1, 0);
-        for (uint32_t j = 0; j < A; ++j) off[j + 1] = off[j] + cnt[j];
-        std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+        for (uint32_t j = 0; j <= A; ++j) off[j] = fill[static_cast<size_t>(j) * 27];
         std::vector<uint32_t> perm(m.n);
-        for (uint32_t t = 0; t < m.n; ++t) perm[fill[newidx[target[mi][t]]]++] = t | (move[mi][t] << kMoveShift);
+        for (uint32_t t = 0; t < m.n; ++t)
+            perm[fill[static_cast<size_t>(newidx[target[mi][t]]) * 27 + move[mi][t]]++] = t | (move[mi][t] << kMoveShift);
         m.perm.swap(perm);
         m.off.swap(off);
     }

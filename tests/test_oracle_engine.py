@@ -180,7 +180,7 @@ def _expected_rebin(o, nmodels, dim, dx):
     out = []
     for m in range(nmodels):
         ni = newidx[targets[m]]
-        order = np.argsort(ni, kind="stable")
+        order = np.argsort(ni.astype(np.int64) * 27 + moves[m], kind="stable")
         perm = order | (moves[m][order] << 27)
         counts = np.bincount(ni, minlength=int(occupied.sum()))
         off = np.concatenate([[0], np.cumsum(counts)])
