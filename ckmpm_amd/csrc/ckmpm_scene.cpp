@@ -19,9 +19,10 @@
 // With --conservation DIR the per-substep conservation metric of model i goes to
 // DIR/conservationMetric_model_<i>.bin, the reference's file (mpm_engine.cuh:494-603).
 // With --export DIR and --frames, the positions of model i after frame f go to
-// DIR/model_<i>_particle_frame_<f>.txt, raw float32 xyz, as the reference's
-// exportParticlePositionAsBinary files (mpm_engine.cuh:437-447, written at the end of
-// every frame, :880-883); its partio .bgeo copies are not written (partio is absent).
+// DIR/model_<i>_particle_frame_<f>.bgeo (Houdini binary geometry, as the reference's
+// partio export, mpm_engine.cuh:446-452) and DIR/model_<i>_particle_frame_<f>.txt, raw
+// float32 xyz, as its exportParticlePositionAsBinary files (mpm_engine.cuh:432-444,
+// written at the end of every frame, :880-883).
 // One JSON line on stdout reports the run.  Exit status 0 on success, 2 on a usage error,
 // 1 on a library error (its CKMPM_ERR_* code and message on stderr).
 #include <algorithm>
@@ -54,8 +55,8 @@ const char kUsage[] =
     "  --region fixed,X0,Y0,Z0,X1,Y1,Z1 | velocity,<box>,VX,VY,VZ | rotation,<box>,WX,WY,WZ,CX,CY,CZ\n"
     "                      grid region on cells [X0,X1) x [Y0,Y1) x [Z0,Z1), applied in order\n"
     "                      before the walls; any --walls / --region replaces the default rule\n"
-    "  --export DIR        with --frames: after frame f write DIR/model_<i>_particle_frame_<f>.txt\n"
-    "                      (positions, float32 xyz, as the reference's binary export)\n"
+    "  --export DIR        with --frames: after frame f write DIR/model_<i>_particle_frame_<f>.bgeo\n"
+    "                      (Houdini binary geometry) and .txt (raw float32 xyz), as the reference\n"
     "a model (options before the --positions that closes it):\n"
     "  --material fc|nh|dp|fluid|vm|nacc\n"
     "  --lambda L --mu M   or  --E E --nu NU   (solids)\n"
@@ -267,10 +268,13 @@ int Run(int argc, char** argv) {
         for (long f = 0; f < frames; ++f) {
             engine.Simulate(1);
             if (!cons.empty()) flush_cons();
-            for (size_t i = 0; !export_dir.empty() && i < models.size(); ++i)
-                ckmpm::WriteRawPositions(export_dir + "/model_" + std::to_string(i) + "_particle_frame_" +
-                                             std::to_string(f) + ".txt",
-                                         engine.Positions(static_cast<int32_t>(i)));
+            for (size_t i = 0; !export_dir.empty() && i < models.size(); ++i) {
+                const std::string stem =
+                    export_dir + "/model_" + std::to_string(i) + "_particle_frame_" + std::to_string(f);
+                const std::vector<float> xyz = engine.Positions(static_cast<int32_t>(i));
+                ckmpm::WriteRawPositions(stem + ".txt", xyz);
+                ckmpm::WriteBgeoPositions(stem + ".bgeo", xyz);
+            }
         }
     }
     engine.Synchronize();

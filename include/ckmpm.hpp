@@ -20,6 +20,7 @@
 #include <array>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -171,6 +172,48 @@ inline void WriteRawPositions(const std::string& path, const std::vector<float>&
     std::FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) throw Error(CKMPM_ERR_INVALID, "cannot create " + path);
     const bool ok = xyz.empty() || std::fwrite(xyz.data(), 4, xyz.size(), f) == xyz.size();
+    if (std::fclose(f) != 0 || !ok) throw Error(CKMPM_ERR_INVALID, "cannot write " + path);
+}
+
+/* The reference's per-frame particle export (mpm_engine.cuh:446-452: a "position"
+ * attribute, written by partio as model_<i>_particle_frame_<f>.bgeo, mpm_io.h:93-131).
+ * partio (a git submodule of the reference, not vendored) writes the classic Houdini
+ * binary geometry layout, all numbers big-endian: the magic 'Bgeo' (int32), 'V', the
+ * version 5 (int32), then eight int32 counts (points, primitives, point groups, primitive
+ * groups, point attributes, vertex attributes, primitive attributes, detail attributes;
+ * only the points are non-zero here, the position being the point's own P), every point
+ * as x y z w with w = 1 (float32), and the bytes 0x00 0xff that close the (empty) extra
+ * section.  The file is not compressed (a name ending in .bgeo, not .bgeo.gz). */
+inline void WriteBgeoPositions(const std::string& path, const std::vector<float>& xyz) {
+    if (xyz.size() % 3 != 0) throw Error(CKMPM_ERR_INVALID, "positions are not xyz triples");
+    const uint64_t n = xyz.size() / 3;
+    if (n > 0x7FFFFFFFu) throw Error(CKMPM_ERR_INVALID, "too many points for a bgeo file");
+    std::vector<unsigned char> buf;
+    buf.reserve(41 + 16 * n + 2);
+    auto put32 = [&](uint32_t v) {
+        for (int s = 24; s >= 0; s -= 8) buf.push_back(static_cast<unsigned char>(v >> s));
+    };
+    auto putf = [&](float f) {
+        uint32_t v;
+        std::memcpy(&v, &f, 4);
+        put32(v);
+    };
+    put32(0x4267656Fu);  // 'B' 'g' 'e' 'o'
+    buf.push_back('V');
+    put32(5u);
+    put32(static_cast<uint32_t>(n));
+    for (int k = 0; k < 7; ++k) put32(0u);
+    for (uint64_t i = 0; i < n; ++i) {
+        putf(xyz[3 * i]);
+        putf(xyz[3 * i + 1]);
+        putf(xyz[3 * i + 2]);
+        putf(1.0f);
+    }
+    buf.push_back(0x00);
+    buf.push_back(0xFF);
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw Error(CKMPM_ERR_INVALID, "cannot create " + path);
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
     if (std::fclose(f) != 0 || !ok) throw Error(CKMPM_ERR_INVALID, "cannot write " + path);
 }
 

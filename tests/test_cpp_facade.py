@@ -185,9 +185,12 @@ def test_driver_frame_loop_matches_simulate(tmp_path):
     r = subprocess.run(_command(sc, tmp_path) + ["--frames", "1", "--export", str(tmp_path)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    from ckmpm_amd import bgeo
     for i in range(len(sc.models)):  # the per-frame export of the last frame is the final state
-        np.testing.assert_array_equal(np.fromfile(tmp_path / f"model_{i}_particle_frame_0.txt", np.float32),
-                                      np.fromfile(tmp_path / f"out{i}.bin", np.float32))
+        final = np.fromfile(tmp_path / f"out{i}.bin", np.float32)
+        np.testing.assert_array_equal(np.fromfile(tmp_path / f"model_{i}_particle_frame_0.txt", np.float32), final)
+        np.testing.assert_array_equal(bgeo.read_positions(tmp_path / f"model_{i}_particle_frame_0.bgeo"),
+                                      final.reshape(-1, 3))
     eng = engine_from_scene(sc)
     eng.Simulate(1)
     eng.Synchronize()
