@@ -702,8 +702,20 @@ struct Snapshot {
 struct AdjointState {
     uint32_t seg = 0;
     bool recording = false;
-    uint32_t len = 0;
-    std::vector<Snapshot> tape;
+    // Two substep tapes.  ckmpm_adjoint_substep runs on its own stream (ustream) and
+    // undoes tape und while the replay of the previous segment records into the other
+    // tape on the engine's stream: at a few hundred particle blocks both are launch- and
+    // latency-bound, so they overlap.  A tape is allocated when first recorded into.
+    std::vector<Snapshot> tape[2];
+    uint32_t tlen[2] = {0, 0};
+    int rec = 0;                     // tape being recorded into
+    int und = 0;                     // tape being undone
+    bool undone[2] = {false, false};  // free_ev[t] follows the last undo queued from tape t
+    hipStream_t ustream = nullptr;
+    hipEvent_t join_ev = nullptr;    // engine stream -> ustream
+    hipEvent_t back_ev = nullptr;    // ustream -> engine stream
+    hipEvent_t free_ev[2] = {nullptr, nullptr};
+    bool need_join = true;           // the engine stream holds work the next undo reads
     std::vector<uint32_t*> ids[2];
     int idc = 0;
     std::vector<uint32_t*> setup_order;
@@ -871,6 +883,13 @@ int Put(ckmpm_engine* e, const Snapshot& s) {
     return CKMPM_OK;
 }
 
+// The engine stream waits for every undo queued so far.
+void JoinBack(ckmpm_engine* e) {
+    AdjointState* A = e->adj;
+    (void)hipEventRecord(A->back_ev, A->ustream);
+    (void)hipStreamWaitEvent(e->stream, A->back_ev, 0);
+}
+
 uint32_t GridSize(const ckmpm_engine* e) {
     uint32_t g = std::max<uint32_t>(e->a_hint + e->a_hint / 4 + 8, 8u);
     g = std::min<uint32_t>(g, 16384u);
@@ -906,19 +925,21 @@ void AdjointAfterSetup(ckmpm_engine* e) {
 
 int AdjointCheckTape(ckmpm_engine* e) {
     AdjointState* A = e->adj;
-    if (A && A->recording && A->len >= A->seg) return Fail(e, CKMPM_ERR_CAPACITY, "substep tape full");
+    if (A && A->recording && A->tlen[A->rec] >= A->seg) return Fail(e, CKMPM_ERR_CAPACITY, "substep tape full");
     return CKMPM_OK;
 }
 
 int AdjointBeforeSubstep(ckmpm_engine* e, double dt, double nd) {
     AdjointState* A = e->adj;
     if (!A || !A->recording) return CKMPM_OK;
-    if (A->len >= A->seg) return Fail(e, CKMPM_ERR_CAPACITY, "substep tape full");
-    Snapshot& s = A->tape[A->len];
+    uint32_t& len = A->tlen[A->rec];
+    if (len >= A->seg) return Fail(e, CKMPM_ERR_CAPACITY, "substep tape full");
+    Snapshot& s = A->tape[A->rec][len];
     CK_TRY(Take(e, s));
     s.dt = dt;
     s.nd = nd;
-    ++A->len;
+    ++len;
+    A->need_join = true;
     return CKMPM_OK;
 }
 
@@ -944,8 +965,12 @@ void AdjointDestroy(ckmpm_engine* e) {
         c->mem = nullptr;
         c->eng = nullptr;
     }
+    if (A->ustream) (void)hipStreamSynchronize(A->ustream);
     for (void* p : A->ck_pool) (void)hipFree(p);
     for (void* p : A->allocs) (void)hipFree(p);
+    for (hipEvent_t ev : {A->join_ev, A->back_ev, A->free_ev[0], A->free_ev[1]})
+        if (ev) (void)hipEventDestroy(ev);
+    if (A->ustream) (void)hipStreamDestroy(A->ustream);
     delete A;
     e->adj = nullptr;
 }
@@ -969,8 +994,9 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
     }
     AdjointState* A = new AdjointState();
     A->seg = segment;
-    A->tape.resize(segment);
-    bool ok = AllocSnapshots(e, A->tape.data(), segment, A->allocs, true);
+    bool ok = hipStreamCreateWithFlags(&A->ustream, hipStreamNonBlocking) == hipSuccess;
+    for (hipEvent_t* ev : {&A->join_ev, &A->back_ev, &A->free_ev[0], &A->free_ev[1]})
+        ok = ok && hipEventCreateWithFlags(ev, hipEventDisableTiming) == hipSuccess;
     const size_t nm = e->models.size();
     for (int k = 0; k < 2; ++k) {
         A->ids[k].assign(nm, nullptr);
@@ -1082,7 +1108,7 @@ int ckmpm_adjoint_rewind(ckmpm_engine* e, const float* vel, uint64_t nvel, const
     LaunchRasterize(e->dom, A->x0[0], A->host_io, m.n, static_cast<uint32_t>(nvel), m.mat.mass, L.table, L.lc,
                     e->M[e->mprev], e->status, st);
     CK_HIP(e, hipMemsetAsync(m.blkmax, 0, static_cast<size_t>(e->cap) * sizeof(float), st));
-    A->len = 0;
+    A->tlen[0] = A->tlen[1] = 0;
     A->recording = false;
     A->at_setup = false;
     CK_HIP(e, hipStreamSynchronize(st));
@@ -1091,14 +1117,33 @@ int ckmpm_adjoint_rewind(ckmpm_engine* e, const float* vel, uint64_t nvel, const
 
 int ckmpm_adjoint_record(ckmpm_engine* e, int32_t on) {
     if (!e || !e->adj) return CKMPM_ERR_INVALID;
-    e->adj->recording = on != 0;
-    if (on) e->adj->len = 0;
+    AdjointState* A = e->adj;
+    if (on && !A->recording) {
+        // the tape that is not being undone; its last undos must be done before it is
+        // overwritten
+        const int t = 1 - A->und;
+        if (A->tape[t].empty()) {
+            A->tape[t].resize(A->seg);
+            if (!AllocSnapshots(e, A->tape[t].data(), A->seg, A->allocs, true)) {
+                A->tape[t].clear();
+                return Fail(e, CKMPM_ERR_HIP, "substep tape allocation failed");
+            }
+        }
+        if (A->undone[t]) CK_HIP(e, hipStreamWaitEvent(e->stream, A->free_ev[t], 0));
+        A->rec = t;
+    } else if (!on && A->recording) {
+        A->und = A->rec;  // undone next
+    }
+    if (on) A->tlen[A->rec] = 0;
+    A->recording = on != 0;
     return CKMPM_OK;
 }
 
 int ckmpm_adjoint_reset(ckmpm_engine* e) {
     if (!e || !e->adj) return CKMPM_ERR_INVALID;
     AdjointState* A = e->adj;
+    JoinBack(e);  // undos still queued write the buffers cleared here
+    A->need_join = true;
     for (size_t i = 0; i < e->models.size(); ++i) {
         const uint32_t n = e->models[i].n;
         FillZeroKernel<<<AGrid(3ull * n), kAT, 0, e->stream>>>(A->xb[A->ab][i], 3ull * n);
@@ -1120,7 +1165,9 @@ int ckmpm_adjoint_seed(ckmpm_engine* e, int32_t model, const float* xbar) {
     AdjointState* A = e->adj;
     const uint32_t n = e->models[model].n;
     CK_HIP(e, hipMemcpyAsync(A->host_io, xbar, sizeof(float) * 3 * n, hipMemcpyHostToDevice, e->stream));
+    JoinBack(e);
     SeedKernel<<<AGrid(n), kAT, 0, e->stream>>>(A->ids[A->idc][model], A->host_io, n, A->xb[A->ab][model]);
+    A->need_join = true;
     CK_HIP(e, hipStreamSynchronize(e->stream));
     return CKMPM_OK;
 }
@@ -1128,9 +1175,16 @@ int ckmpm_adjoint_seed(ckmpm_engine* e, int32_t model, const float* xbar) {
 int ckmpm_adjoint_substep(ckmpm_engine* e) {
     if (!e || !e->adj) return CKMPM_ERR_INVALID;
     AdjointState* A = e->adj;
-    if (A->len == 0) return Fail(e, CKMPM_ERR_INVALID, "no recorded substep to undo");
-    const Snapshot& s = A->tape[A->len - 1];
-    hipStream_t st = e->stream;
+    const int tp = A->recording ? A->rec : A->und;
+    if (A->tlen[tp] == 0) return Fail(e, CKMPM_ERR_INVALID, "no recorded substep to undo");
+    const Snapshot& s = A->tape[tp][A->tlen[tp] - 1];
+    hipStream_t st = A->ustream;
+    if (A->need_join) {
+        // the recording of this tape (or the reset / seed) is queued on the engine stream
+        CK_HIP(e, hipEventRecord(A->join_ev, e->stream));
+        CK_HIP(e, hipStreamWaitEvent(st, A->join_ev, 0));
+        A->need_join = false;
+    }
     const uint32_t grid = GridSize(e);
     float* Mout = A->Mbar[1 - A->mb];
     const int nb = 1 - A->ab;
@@ -1186,7 +1240,9 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
     A->at_setup = s.substeps == 0;
     A->setup_table = s.table[1];
     A->setup_lc = s.lc[1];
-    --A->len;
+    --A->tlen[tp];
+    CK_HIP(e, hipEventRecord(A->free_ev[tp], st));
+    A->undone[tp] = true;
     CK_HIP(e, hipGetLastError());
     return CKMPM_OK;
 }
@@ -1199,6 +1255,7 @@ int ckmpm_adjoint_gradients(ckmpm_engine* e, int32_t model, float* lambda_bar, f
     hipStream_t st = e->stream;
     const uint32_t n = m.n;
     std::vector<float> tmp(3 * static_cast<size_t>(n));
+    JoinBack(e);
     if (v0_bar) {
         RasterizeBackwardKernel<<<AGrid(n), kAT, 0, st>>>(e->dom, A->x0[model], n, m.mat.mass, A->setup_table, A->setup_lc,
                                                           A->Mbar[A->mb], A->host_io);

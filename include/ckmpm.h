@@ -274,13 +274,18 @@ int ckmpm_checkpoint_free(ckmpm_checkpoint* ck);
  * 0.  Allocates nothing, so a gradient loop on one scene reuses one engine. */
 int ckmpm_adjoint_rewind(ckmpm_engine* eng, const float* vel, uint64_t nvel, const float* lambda_p,
                          const float* mu_p);
-/* on != 0: the following substeps are recorded (the tape restarts empty) */
+/* on != 0: the following substeps are recorded (the tape restarts empty).  The engine
+ * keeps two tapes of `segment` substeps (the second allocated on first use): a
+ * recording goes to the tape not being undone, so the replay of one segment runs beside
+ * the undo of the next one (ckmpm_adjoint_substep runs on the engine's second stream). */
 int ckmpm_adjoint_record(ckmpm_engine* eng, int32_t on);
 /* zero every particle and grid adjoint */
 int ckmpm_adjoint_reset(ckmpm_engine* eng);
 /* dL/dx of the model's particles at the current state, input order, 3 floats each */
 int ckmpm_adjoint_seed(ckmpm_engine* eng, int32_t model, const float* xbar);
-/* back-propagate through the last recorded substep and drop it from the tape */
+/* back-propagate through the last substep of the tape recorded last and drop it from
+ * the tape; queued on the engine's second stream after the recording (asynchronous: the
+ * calls that read or clear the adjoints wait for it) */
 int ckmpm_adjoint_substep(ckmpm_engine* eng);
 /* after the first substep has been undone: dL/dlambda, dL/dmu (n each) and dL/dv0
  * (3 n), input order; any pointer may be null */
