@@ -162,6 +162,66 @@ __global__ __launch_bounds__(kHashThreads) void HashFindKernel(HashView h, const
     }
 }
 
+// LDS-probed Find for tables of up to kStagedBuckets buckets.  Each workgroup (1024
+// threads, one per CU) first copies the table's keys - the high words of its slots, 4
+// bytes per slot, at most the CU's 160 KiB of LDS - into LDS with 16-byte loads, then every thread
+// probes its own keys there: a bucket is four 16-byte LDS reads and the same bucket rule
+// as HashView::Find (a bucket with an empty slot ends the search).  A hit reads its value
+// from the slot in global memory; a key that finds all its buckets full scans the stash
+// there.  The copy costs the table's size per workgroup, so this pays off when the keys
+// outnumber the slots by more than the workgroup count.
+constexpr uint32_t kStagedThreads = 1024;
+constexpr uint32_t kStagedBuckets = 2560;  // 40960 slot keys: 160 KiB
+__global__ __launch_bounds__(kStagedThreads) void HashFindStagedKernel(HashView h, const uint32_t* keys, uint32_t n,
+                                                                       uint32_t* out) {
+    __shared__ __align__(16) uint32_t sK[kStagedBuckets * kHashBucket];
+    const uint32_t pairs = h.buckets * kHashBucket / 2;  // two slots per 16-byte load
+    const uint4* src = reinterpret_cast<const uint4*>(h.slots);
+    for (uint32_t i = threadIdx.x; i < pairs; i += kStagedThreads) {
+        const uint4 q = src[i];
+        *reinterpret_cast<uint2*>(sK + 2 * i) = make_uint2(q.y, q.w);
+    }
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * kStagedThreads + threadIdx.x; i < n; i += gridDim.x * kStagedThreads) {
+        const uint32_t key = keys[i];
+        uint32_t found = kEmpty;
+        bool done = false;
+        for (int r = 0; r < kHashRounds && !done; ++r) {
+            const uint32_t b = h.Bucket(key, r);
+            const uint4* row = reinterpret_cast<const uint4*>(sK + static_cast<size_t>(b) * kHashBucket);
+            int hit = -1;
+            bool full = true;
+#pragma unroll
+            for (int k = 0; k < kHashBucket / 4; ++k) {
+                const uint4 q = row[k];
+                const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (hit < 0 && w[c] == key) hit = 4 * k + c;
+                    full &= w[c] != 0xFFFFFFFFu;
+                }
+            }
+            if (hit >= 0) {
+                found = static_cast<uint32_t>(h.slots[static_cast<size_t>(b) * kHashBucket + hit]);
+                done = true;
+            } else if (!full) {
+                done = true;
+            }
+        }
+        if (!done) {
+            const uint32_t ns = *h.nstash;
+            for (uint32_t j = 0; j < ns; ++j) {
+                const uint64_t p = h.stash[j];
+                if (static_cast<uint32_t>(p >> 32) == key) {
+                    found = static_cast<uint32_t>(p);
+                    break;
+                }
+            }
+        }
+        out[i] = found;
+    }
+}
+
 // One key per thread (HashView::Find), for comparison with the cooperative form.
 __global__ __launch_bounds__(kHashThreads) void HashFindScalarKernel(HashView h, const uint32_t* keys, uint32_t n,
                                                                      uint32_t* out) {
@@ -442,10 +502,23 @@ int ckmpm_hash_build(ckmpm_hash* h, const uint32_t* keys, const uint32_t* values
 
 int ckmpm_hash_find_device(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values, int32_t cooperative) {
     if (!h || (n && (!keys || !values))) return CKMPM_ERR_INVALID;
+    if (cooperative < 0 || cooperative > 2) return HashFail(h, CKMPM_ERR_INVALID, "find mode is 0, 1 or 2");
+    if (cooperative == 2 && h->buckets > kStagedBuckets)
+        return HashFail(h, CKMPM_ERR_INVALID, "table too large for the LDS-probed find (more than 2560 buckets)");
     if (n == 0) return CKMPM_OK;
     const HashView v = View(h);
-    if (cooperative) HashFindKernel<<<LaunchBlocks(n), kHashThreads, 0, h->stream>>>(v, keys, n, values);
-    else HashFindScalarKernel<<<LaunchBlocks(n), kHashThreads, 0, h->stream>>>(v, keys, n, values);
+    if (cooperative == 2) {
+        int dev = 0, cus = 0;
+        HASH_HIP(h, hipGetDevice(&dev));
+        HASH_HIP(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const uint32_t grid = std::max<uint32_t>(
+            1, std::min<uint32_t>(static_cast<uint32_t>(std::max(cus, 1)), (n + kStagedThreads - 1) / kStagedThreads));
+        HashFindStagedKernel<<<grid, kStagedThreads, 0, h->stream>>>(v, keys, n, values);
+    } else if (cooperative) {
+        HashFindKernel<<<LaunchBlocks(n), kHashThreads, 0, h->stream>>>(v, keys, n, values);
+    } else {
+        HashFindScalarKernel<<<LaunchBlocks(n), kHashThreads, 0, h->stream>>>(v, keys, n, values);
+    }
     HASH_HIP(h, hipGetLastError());
     return CKMPM_OK;
 }

@@ -54,8 +54,14 @@ class BlockHashTable:
         self._check(self._lib.ckmpm_hash_build_device(self._h, ctypes.c_void_p(keys_ptr), ctypes.c_void_p(values_ptr),
                                                       int(n)), "ckmpm_hash_build_device")
 
-    def FindDevice(self, keys_ptr: int, n: int, out_ptr: int, cooperative: bool = True) -> None:
-        """Enqueue a lookup of n device keys; call Synchronize before reading out_ptr."""
+    # lookup forms of FindDevice
+    SCALAR, COOPERATIVE, LDS = 0, 1, 2
+
+    def FindDevice(self, keys_ptr: int, n: int, out_ptr: int, cooperative=True) -> None:
+        """Enqueue a lookup of n device keys; call Synchronize before reading out_ptr.
+        cooperative: True / COOPERATIVE (a 16-lane group per probe), False / SCALAR (a
+        thread per key) or LDS (the table's keys staged in LDS; tables of at most 2560
+        buckets)."""
         self._check(self._lib.ckmpm_hash_find_device(self._h, ctypes.c_void_p(keys_ptr), int(n),
                                                      ctypes.c_void_p(out_ptr), int(cooperative)),
                     "ckmpm_hash_find_device")

@@ -361,7 +361,10 @@ int ckmpm_hash_build_noalloc(const ckmpm_domain_desc* domain, uint32_t max_keys,
 /* Find (CooperativeFind, gpu_hash_table.cuh:463-513) of host keys; missing -> 0xFFFFFFFF */
 int ckmpm_hash_find(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values);
 /* Same on device arrays, enqueued on the table's stream (no sync).  cooperative = 1: a
- * 16-lane quarter wavefront per probe (the reference's tile of 16); 0: one thread per key */
+ * 16-lane quarter wavefront per probe (the reference's tile of 16); 0: one thread per key;
+ * 2: LDS-probed, for tables of at most 2560 buckets (created for up to 28.6k keys;
+ * CKMPM_ERR_INVALID above): each workgroup stages the table's keys in LDS once, then
+ * probes there. */
 int ckmpm_hash_find_device(ckmpm_hash* h, const uint32_t* keys, uint32_t n, uint32_t* values, int32_t cooperative);
 int ckmpm_hash_synchronize(ckmpm_hash* h);
 int ckmpm_hash_info(ckmpm_hash* h, ckmpm_hash_info_t* info);

@@ -191,12 +191,46 @@ def test_gpu_stash_and_both_find_kernels():
     q = np.arange(16 * 16 * 16, dtype=np.uint32)
     expect = o.find(q)
     dq = torch.from_numpy(q.view(np.int32)).cuda()
-    for coop in (True, False):
+    for mode in (g.COOPERATIVE, g.SCALAR, g.LDS):
         out = torch.empty_like(dq)
-        g.FindDevice(dq.data_ptr(), q.shape[0], out.data_ptr(), cooperative=coop)
+        g.FindDevice(dq.data_ptr(), q.shape[0], out.data_ptr(), cooperative=mode)
         g.Synchronize()
-        np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), expect)
+        np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), expect, err_msg=f"find mode {mode}")
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nkeys", [1000, 28000])
+def test_gpu_lds_find_matches_the_oracle(nkeys):
+    """The LDS-probed find against the oracle's find, on a table near the LDS limit (2560
+    buckets hold 28.6k keys at the creation load) and a small one, with 2M queries: every
+    key, keys that are absent and keys outside the domain."""
+    import torch
+    from ckmpm_amd.hashtable import BlockHashTable
+    from ckmpm_amd.native import CkmpmError
+
+    dim = (128, 128, 128)
+    rng = np.random.default_rng(nkeys)
+    keys = rng.choice(dim[0] * dim[1] * dim[2], nkeys, replace=False).astype(np.uint32)
+    vals = rng.integers(0, 2 ** 31, nkeys).astype(np.uint32)
+    o = HashOracle(dim, nkeys)
+    o.build(keys, vals)
+    g = BlockHashTable(dim, nkeys)
+    g.Insert(keys, vals)
+    q = np.concatenate([keys, rng.integers(0, dim[0] * dim[1] * dim[2] + 5000, 2_000_000 - nkeys).astype(np.uint32)])
+    print(f"LDS find: {nkeys} keys, {o.buckets} buckets, {q.shape[0]} queries", flush=True)
+    expect = o.find(q)
+    dq = torch.from_numpy(q.view(np.int32)).cuda()
+    out = torch.empty_like(dq)
+    g.FindDevice(dq.data_ptr(), q.shape[0], out.data_ptr(), cooperative=g.LDS)
+    g.Synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), expect)
+    assert (expect[:nkeys] == vals).all()
+    g.close()
+    big = BlockHashTable(dim, 30000)  # 2679 buckets: beyond the LDS-probed form
+    with pytest.raises(CkmpmError, match="too large"):
+        big.FindDevice(dq.data_ptr(), 16, out.data_ptr(), cooperative=big.LDS)
+    big.close()
 
 
 @pytest.mark.gpu
