@@ -1217,8 +1217,12 @@ int ckmpm_adjoint_substep(ckmpm_engine* e) {
         a.lc_zero = s.lc[1];
         a.cap = e->cap;
         // few particle blocks (small scenes): several workgroups per block so that the
-        // launch still covers the 256 CUs
-        a.split = std::min<uint32_t>(8, std::max<uint32_t>(1, (2048 + e->a_hint) / std::max<uint32_t>(e->a_hint, 1)));
+        // launch still covers the 256 CUs, but no more than the block's 256-particle
+        // chunks (a workgroup without a chunk would only hold a CU slot)
+        const uint32_t blocks = std::max<uint32_t>(e->a_hint, 1);
+        const uint32_t chunks = static_cast<uint32_t>((static_cast<uint64_t>(m.n) + static_cast<uint64_t>(blocks) * kAT - 1) /
+                                                      (static_cast<uint64_t>(blocks) * kAT));
+        a.split = std::min<uint32_t>({8u, std::max<uint32_t>(1, (2048 + blocks) / blocks), std::max<uint32_t>(chunks, 1)});
         const uint32_t bgrid = std::min<uint32_t>(grid * a.split, 65536u / a.split * a.split);
         if (m.kind == kFixedCorotated) G2P2GBackwardKernel<kFixedCorotated><<<bgrid, kAT, 0, st>>>(a);
         else G2P2GBackwardKernel<kNeoHookean><<<bgrid, kAT, 0, st>>>(a);

@@ -227,6 +227,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
         ModelBuf& m = e->models[i];
         rm.cnt[i] = m.cnt;
         rm.cnt_e[i] = m.cnt_e;
+        rm.eoff[i] = m.eoff;
         rm.base[i] = m.base;
         rm.off_next[i] = m.off[sn];
         rm.off_cur[i] = m.off[sc];
@@ -238,7 +239,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
     const RebinCountArgs ca{e->dom, C.keys, C.table, C.lc, rm, e->tsum, P.keys, P.lc, P.table, e->cap,
                             e->ext_cnt, 2u * e->ext_tiles};
     Run(rec, kStCount, kFkRebinCount, 0, ca, RebinCountGrid(e->cap, 2u * e->a_hint + 64u), s);
-    const RebinScanArgs sa{C.keys, C.lc, rm, e->tsum, e->newidx, P.keys, P.table, P.lc, e->cap, e->status};
+    const RebinScanArgs sa{C.keys, C.lc, rm, e->tsum, P.keys, P.table, P.lc, e->cap, e->status};
     Run(rec, kStScan, kFkRebinScan, 0, sa, RebinScanGrid(e->cap), s);
     // Two independent chains from here: the particle permutation (RebinScatter: current
     // layout, orderings) and the extension of the next layout (its neighbour and exterior
@@ -251,7 +252,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
         ExtendLayout(e, P, false, e->side);
     }
     // a wavefront per particle block, 4 per workgroup
-    const RebinScatterArgs xa{e->dom, C.keys, C.table, C.lc, rm, e->newidx, e->cap};
+    const RebinScatterArgs xa{e->dom, C.keys, C.table, C.lc, rm, e->cap};
     Run(rec, kStScatter, kFkRebinScatter, 0, xa, GridFor((e->a_hint + e->a_hint / 4) / 4u + 8u), s);
     if (fork) {
         (void)hipEventRecord(e->join_ev, e->side);
@@ -323,7 +324,6 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
     check(Alloc(e, &e->ext_cnt, 2 * static_cast<size_t>(e->ext_tiles)));
     check(Alloc(e, &e->tsum, RebinTileSumWords(e->cap)));
     check(Alloc(e, &e->word_off, e->nwords));
-    check(Alloc(e, &e->newidx, e->cap));
     check(Alloc(e, &e->total, 4));
     check(Alloc(e, &e->scan_tmp, ScanTempWords(std::max<size_t>(e->cap, e->nwords))));
     if (rc == CKMPM_OK) {
@@ -404,6 +404,7 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     CK_TRY(Alloc(e, &m.cnt, static_cast<size_t>(e->cap) * kDirections));
     CK_TRY(Alloc(e, &m.base, static_cast<size_t>(e->cap) * kDirections));
     CK_TRY(Alloc(e, &m.cnt_e, e->cap));
+    CK_TRY(Alloc(e, &m.eoff, e->cap));
     CK_TRY(Alloc(e, &m.blkmax, e->cap));
     CK_HIP(e, hipMemsetAsync(m.blkmax, 0, static_cast<size_t>(e->cap) * sizeof(float), e->stream));
     e->models.push_back(std::move(m));

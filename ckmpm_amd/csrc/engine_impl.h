@@ -22,6 +22,7 @@ struct ModelBuf {
     uint32_t* cnt = nullptr;
     uint32_t* base = nullptr;
     uint32_t* cnt_e = nullptr;
+    uint32_t* eoff = nullptr;
     float* blkmax = nullptr;  // [cap] largest P2G deposit bound per particle block
     int pp_attr = 0;      // state attribute of per-particle (lambda, mu); 0: none
     int rigid_attr = 0;   // state attribute of the rigid flag (1: rigid); 0: none
@@ -79,7 +80,6 @@ struct ckmpm_engine {
     uint32_t ext_tiles = 0;           // map tiles of kEmitTileWords words
     uint32_t* ext_cnt = nullptr;      // [ext_tiles][2] neighbour / exterior blocks marked per tile
     uint32_t* tsum = nullptr;         // rebin count tile sums
-    uint32_t* newidx = nullptr;
     uint32_t* scan_tmp = nullptr;
     uint32_t* total = nullptr;
     float* pos_stage = nullptr;  // ckmpm_get_positions staging (3 floats per particle of the largest model)

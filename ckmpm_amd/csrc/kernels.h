@@ -127,6 +127,7 @@ struct RebinModels {
     int nmodels;
     const uint32_t* cnt[kMaxModels];   // [A_cur][27] direction counts per old block
     uint32_t* cnt_e[kMaxModels];       // [Ex_cur] particles entering each exterior block
+    uint32_t* eoff[kMaxModels];        // [Ex_cur] next-layout offset of each occupied exterior block
     // [Ex_cur][27] per exterior block e and move d: particles of the model entering e
     // with a move before d
     uint32_t* base[kMaxModels];
@@ -168,7 +169,6 @@ struct RebinScanArgs {
     const LayoutCounts* lc;
     RebinModels rm;
     const uint32_t* tsum;
-    uint32_t* newidx;
     uint32_t* keys_next;
     uint32_t* table_next;
     LayoutCounts* lc_next;
@@ -181,7 +181,6 @@ struct RebinScatterArgs {
     const uint32_t* table;
     const LayoutCounts* lc;
     RebinModels rm;
-    const uint32_t* newidx;  // next-layout index of each occupied exterior block
     uint32_t cap;
 };
 // Extension map: two bits per domain block, kExtKeysPerWord blocks per word.

@@ -1467,7 +1467,6 @@ __device__ __forceinline__ void RebinScanBody(const RebinScanArgs& a, uint32_t w
     const uint32_t* keys = a.keys;
     const RebinModels& rm = a.rm;
     const uint32_t* tsum = a.tsum;
-    uint32_t* newidx = a.newidx;
     uint32_t* keys_next = a.keys_next;
     uint32_t* table_next = a.table_next;
     const uint32_t cap = a.cap;
@@ -1527,7 +1526,6 @@ __device__ __forceinline__ void RebinScanBody(const RebinScanArgs& a, uint32_t w
         if (i < cap) {
             keys_next[i] = key[k];
             table_next[key[k]] = i;
-            newidx[e0 + k] = i;
         }
         ++i;
     }
@@ -1543,6 +1541,7 @@ __device__ __forceinline__ void RebinScanBody(const RebinScanArgs& a, uint32_t w
             for (int k = 0; k < kRebinScanItems; ++k) {
                 if ((omask >> k) & 1u) {
                     if (i < cap) rm.off_next[m][i] = run;
+                    rm.eoff[m][e0 + k] = run;
                     ++i;
                 }
                 run += c[m][k];
@@ -1554,9 +1553,9 @@ __global__ __launch_bounds__(kThreads) void RebinScanKernel(RebinScanArgs a) { R
 
 // Particle permutation of the next layout: a wavefront per block j of the current
 // layout, for every model.  Lane d < 27 first finds the destination e = j + d (an
-// exterior block of the current layout) and its index in the next layout; per model,
-// the first destination of move d is e's offset there plus base[m][e][d] (the particles
-// entering e with an earlier move), and the particle loop takes it by lane shuffle.
+// exterior block of the current layout); per model, the first destination of move d is
+// e's offset in the next layout (eoff) plus base[m][e][d] (the particles entering e with
+// an earlier move), and the particle loop takes it by lane shuffle.
 // kScatterUnroll particles per lane are in flight (a block holds a few hundred
 // particles per model, so most blocks take one pass of dependent loads: 512 per pass).
 constexpr int kScatterUnroll = 8;
@@ -1565,7 +1564,7 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
     const uint32_t A = a.lc->A, cap = a.cap;
     const uint32_t lane = LaneId(), waves = kThreads / 64;
     for (uint32_t j = wg * waves + (threadIdx.x >> 6); j < A; j += ng * waves) {
-        uint32_t e = kEmpty, ni = kEmpty;
+        uint32_t e = kEmpty;
         if (lane < static_cast<uint32_t>(kDirections)) {
             int bx, by, bz, ox, oy, oz;
             CoordsOf(a.d, a.keys[j], bx, by, bz);
@@ -1573,10 +1572,7 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
             bx += ox;
             by += oy;
             bz += oz;
-            if (InDomain(a.d, bx, by, bz)) {
-                e = a.table[KeyOf(a.d, bx, by, bz)];
-                if (e < cap) ni = a.newidx[e];
-            }
+            if (InDomain(a.d, bx, by, bz)) e = a.table[KeyOf(a.d, bx, by, bz)];
         }
         for (int m = 0; m < rm.nmodels; ++m) {
             const uint32_t* off = rm.off_cur[m];
@@ -1587,7 +1583,7 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
             const uint32_t n = rm.n[m];
             // a move without a destination (no particle takes it) keeps n: its positions drop
             uint32_t first = n;
-            if (ni < cap) first = rm.off_next[m][ni] + rm.base[m][static_cast<size_t>(e) * kDirections + lane];
+            if (e < cap) first = rm.eoff[m][e] + rm.base[m][static_cast<size_t>(e) * kDirections + lane];
             for (uint32_t t0 = beg; t0 < end; t0 += kScatterUnroll * 64) {
                 uint32_t c[kScatterUnroll];
 #pragma unroll
