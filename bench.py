@@ -4,7 +4,9 @@ Workload (default): the 8M-particle mixed-material scene - 5M weakly compressibl
 particles in a dam-break column plus a 3M-particle Drucker-Prager sand cone, sparse
 512^3 grid (128^3 blocks), per-cell counts log-uniform in [4, 16], dt = 2e-5, FP32.
 A step is one substep (grid update, fused G2P2G for both materials, rebin) over all
-particles; inputs are resident in HBM before timing starts.
+particles; inputs are resident in HBM before timing starts.  The timed substeps replay
+hipGraphs of two substeps, captured before timing; the per-kernel durations come from a
+second pass of as many substeps launched with HIP events at the phase boundaries.
 
 Prints one JSON line.  roofline: the G2P2G phase (the dominant kernels), algorithmic
 bytes per substep = sum_m N_m (2 S_m + 8) + 3584 G (state read + write, old-slot read and
@@ -351,7 +353,9 @@ def bench_hash(args, torch):
     dense = torch.full((nblocks,), -1, dtype=torch.int32, device="cuda")
     dense[dk.long()] = dv
     res = {}
-    for name in ("cooperative", "scalar", "dense"):
+    forms = {"cooperative": h.COOPERATIVE, "scalar": h.SCALAR, "lds": h.LDS}
+    names = ["cooperative", "scalar", "dense"] + (["lds"] if h.info().buckets <= 2560 else [])
+    for name in names:
         for rep in range(2):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -359,7 +363,7 @@ def bench_hash(args, torch):
                 if name == "dense":
                     out = dense[dq.long()]
                 else:
-                    h.FindDevice(dq.data_ptr(), dq.numel(), out.data_ptr(), cooperative=(name == "cooperative"))
+                    h.FindDevice(dq.data_ptr(), dq.numel(), out.data_ptr(), cooperative=forms[name])
             h.Synchronize()
             torch.cuda.synchronize()
             el = (time.perf_counter() - t0) / reps
@@ -406,7 +410,7 @@ def main():
     import torch
 
     from ckmpm_amd import scenes
-    from ckmpm_amd.engine import engine_from_scene
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
 
     if args.workload == "batch64":
         print(json.dumps(bench_batch(args, torch)), flush=True)
@@ -442,21 +446,36 @@ def main():
     eng.Synchronize()
     torch.cuda.synchronize()
     print(f"[bench] {args.warmup} warmup substeps done, timing {args.steps}", file=sys.stderr, flush=True)
+    # the job: K substeps replayed as hipGraphs of two substeps (one launch per kernel,
+    # the layout extension on a second stream of the graph), captured before the timed
+    # region
+    b = MPMBatch([eng])
+    b.Run(2, MPMBatch.FUSED)
+    b.Synchronize()
     c0 = eng.counters()
-    eng.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    b.Run(args.steps, MPMBatch.FUSED)
+    b.Synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    c1 = eng.counters()
+    b.close()
+    elapsed = t1 - t0
+    # per-kernel durations: K more substeps as plain launches on the engine stream with
+    # HIP events at the phase boundaries (the events add gaps between launches, so this
+    # pass is not the throughput)
+    eng.set_profiling(True)
     for _ in range(args.steps):
         eng.Substep(dt, dt)
     eng.Synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     (ms_grid, ms_g2p2g, ms_rebin), nsub = eng.phase_times()
-    c1 = eng.counters()
-    elapsed = t1 - t0
+    c2 = eng.counters()
+    eng.set_profiling(False)
     N = sc.n_particles
     G = 0.5 * (c0.neighbor_blocks + c1.neighbor_blocks)
-    g2p2g_bytes = sum(m.x.shape[0] * (2 * STATE_BYTES[m.material.kind] + 8) for m in sc.models) + 3584 * G
+    G_ev = 0.5 * (c1.neighbor_blocks + c2.neighbor_blocks)
+    g2p2g_bytes = sum(m.x.shape[0] * (2 * STATE_BYTES[m.material.kind] + 8) for m in sc.models) + 3584 * G_ev
     g2p2g_s = ms_g2p2g / max(nsub, 1) * 1e-3
     achieved = g2p2g_bytes / g2p2g_s / 1e9
     substep_bytes = sum(m.x.shape[0] * (2 * STATE_BYTES[m.material.kind] + 12) for m in sc.models) + 7168 * G
@@ -486,6 +505,9 @@ def main():
         "substep_hbm": {"bytes": substep_bytes, "frac": substep_bytes * sps / (HBM_PEAK_GBS * 1e9)},
         "phase_ms_per_substep": {"grid_update": ms_grid / max(nsub, 1), "g2p2g": ms_g2p2g / max(nsub, 1),
                                  "rebin": ms_rebin / max(nsub, 1)},
+        "timing": {"value": f"{args.steps} substeps replayed as two-substep hipGraphs (fused launches)",
+                   "phases": f"{nsub} further substeps as plain launches with HIP events at phase boundaries",
+                   "events_ms_per_step": (ms_grid + ms_g2p2g + ms_rebin) / max(nsub, 1)},
         "blocks": {"active": c1.active_blocks, "neighbor": c1.neighbor_blocks, "exterior": c1.exterior_blocks},
         "g2p2g_workgroup": [eng.tuning(m) for m in range(len(sc.models))],
     }

@@ -240,3 +240,37 @@ def test_fused_batch_of_64_scenes_matches_separate_engines_and_the_oracle():
             assert o.flags == 0
     assert batch.captures == _graphs_captured([substeps, 1, 1], True) == 3
     batch.close()
+
+
+@pytest.mark.gpu
+def test_single_engine_fused_graphs_match_plain_launches_at_8m():
+    """The benchmark's timed path: the 8M-particle mixed scene advanced by one-engine
+    batches of fused launches replayed as hipGraphs, against plain launches."""
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+
+    sc = scenes.mixed_dam_sand()
+    substeps = 7
+    print(f"8M mixed scene, {substeps} substeps: plain launches vs fused hipGraphs", flush=True)
+    a = engine_from_scene(sc)
+    for k in range(substeps):
+        a.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+    a.Synchronize()
+    b = engine_from_scene(sc)
+    b.Substep(0.0, sc.dt)
+    batch = MPMBatch([b])
+    batch.Run(substeps - 1, MPMBatch.FUSED)
+    batch.Synchronize()
+    ka, ca = a.layout(0)
+    kb, cb = b.layout(0)
+    assert ca == cb
+    np.testing.assert_array_equal(ka, kb)
+    for m in range(len(sc.models)):
+        pa, oa = a.ordering(m)
+        pb, ob = b.ordering(m)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(oa, ob)
+        np.testing.assert_allclose(b.state(m), a.state(m), rtol=0, atol=1e-5)
+    assert a.counters().flags == 0 and b.counters().flags == 0
+    batch.close()
+    a.close()
+    b.close()
