@@ -232,7 +232,7 @@ def bench_adjoint(args, torch):
     import numpy as np
 
     from ckmpm_amd import scenes
-    from ckmpm_amd.autodiff import simulate
+    from ckmpm_amd.autodiff import clear_engine_cache, simulate
     from ckmpm_amd.engine import engine_from_scene
 
     sc = scenes.cantilever()
@@ -286,6 +286,9 @@ def bench_adjoint(args, torch):
     t_all, loss = float(np.median(ck_samples)), ck[-1][1]
     tape_vs_ck = max(float((g_tape[0] - logE.grad).norm() / logE.grad.norm()),
                      float((g_tape[1] - v0.grad).norm() / v0.grad.norm()))
+    # the kept engine holds a stream on a hardware queue of its own, which slows other
+    # work's graph replays while it exists: release it before anything else is timed
+    clear_engine_cache()
     return {
         "metric": "particle-substeps/s fwd+bwd (cantilever, per-particle log E and v0 gradients)",
         "value": N * steps / t_all,

@@ -994,7 +994,7 @@ int ckmpm_adjoint_enable(ckmpm_engine* e, uint32_t segment) {
     }
     AdjointState* A = new AdjointState();
     A->seg = segment;
-    bool ok = hipStreamCreateWithFlags(&A->ustream, hipStreamNonBlocking) == hipSuccess;
+    bool ok = CreateOwnQueueStream(&A->ustream) == hipSuccess;
     for (hipEvent_t* ev : {&A->join_ev, &A->back_ev, &A->free_ev[0], &A->free_ev[1]})
         ok = ok && hipEventCreateWithFlags(ev, hipEventDisableTiming) == hipSuccess;
     const size_t nm = e->models.size();
