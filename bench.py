@@ -181,6 +181,14 @@ def bench_batch(args, torch, only=None):
             continue
         b = MPMBatch(engines)
         b.Run(args.warmup, mode)
+        tuned = None
+        if mode == MPMBatch.FUSED:
+            # the batch's own tuning (per kind G2P2G workgroup size, launch order), untimed
+            b.autotune(2)
+            tuned = {"serial_g2p2g": b.serial_g2p2g,
+                     "g2p2g_512_by_kind": {str(k): sum(1 for e, sc in zip(engines, scs) for m in range(len(sc.models))
+                                                       if sc.models[m].material.kind == k and e.tuning(m) == 512)
+                                           for k in sorted({m.material.kind for sc in scs for m in sc.models})}}
         b.Synchronize()
         torch.cuda.synchronize()
         c0 = [e.counters() for e in engines]
@@ -193,6 +201,8 @@ def bench_batch(args, torch, only=None):
         G = sum(0.5 * (a.neighbor_blocks + c.neighbor_blocks) for a, c in zip(c0, c1))
         res[name] = {"value": N * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "G": G,
                      "captures": b.captures}
+        if tuned is not None:
+            res[name]["tuning"] = tuned
         print(f"[bench] {name}: {res[name]['ms_per_step']:.3f} ms per batch substep", file=sys.stderr, flush=True)
         b.close()
     substep_bytes = sum(m.x.shape[0] * (2 * STATE_BYTES[m.material.kind] + 12) for sc in scs for m in sc.models)
@@ -223,6 +233,7 @@ def bench_batch(args, torch, only=None):
         "modes_ms_per_step": {k: v["ms_per_step"] for k, v in res.items()},
         "batched_over_sequential": res["batched"]["value"] / res["sequential"]["value"],
         "graph_captures": res["batched"]["captures"],
+        "tuning": res["batched"].get("tuning"),
     }
 
 
@@ -541,7 +552,7 @@ def main():
         out["batched_over_sequential"] = bt["batched_over_sequential"]
         out["batched"] = {"workload": "batch64", "scenes": bt["config"]["scenes"], "particles": bt["config"]["particles"],
                           "batched_ms_per_step": bt["ms_per_step"], "sequential_ms_per_step": bt["sequential"]["ms_per_step"],
-                          "value": bt["value"], "substeps": 40}
+                          "value": bt["value"], "substeps": 40, "tuning": bt["tuning"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)

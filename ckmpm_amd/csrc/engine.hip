@@ -644,6 +644,7 @@ int ckmpm_autotune(ckmpm_engine* e, uint32_t substeps) {
 int ckmpm_set_tuning(ckmpm_engine* e, int32_t model, int32_t threads) {
     if (!e || model < 0 || model >= static_cast<int32_t>(e->models.size())) return CKMPM_ERR_INVALID;
     if (threads != 256 && threads != 512) return Fail(e, CKMPM_ERR_INVALID, "G2P2G workgroup size must be 256 or 512");
+    if (e->models[model].threads != threads) ++e->args_version;  // captured graphs hold the old size
     e->models[model].threads = threads;
     return CKMPM_OK;
 }
@@ -863,6 +864,7 @@ struct ckmpm_batch {
     // fused mode: streams and events that run a substep's G2P2G launches side by side
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> aux_done;
+    bool g2p2g_serial = false;  // fused mode: the G2P2G launches of a substep one after the other
     // mode 3: one graph per engine and parity, replayed on the engine's own stream
     std::vector<BatchGraph> pe[2];
     uint64_t captures = 0;
@@ -1082,7 +1084,7 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
         }
         size_t k = j;
         while (k < launches.size() && launches[k].kernel == kFkG2P2G) ++k;
-        if (k - j < 2) {
+        if (k - j < 2 || b->g2p2g_serial) {
             const Launch& l = launches[j];
             LaunchFused(l.kernel, l.variant, dev + l.args, reinterpret_cast<const uint32_t*>(dev + l.begin), l.n,
                         l.total, b->cs);
@@ -1238,6 +1240,89 @@ int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode) {
         for (ckmpm_engine* e : b->eng) EagerSubstep(e);
     const hipError_t r = hipGetLastError();
     if (r != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, std::string("launch: ") + hipGetErrorString(r));
+    return CKMPM_OK;
+}
+
+int ckmpm_batch_autotune(ckmpm_batch* b, uint32_t pairs) {
+    if (!b) return CKMPM_ERR_INVALID;
+    if (pairs == 0) return CKMPM_OK;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    BATCH_HIP(b, hipEventCreate(&t0));
+    BATCH_HIP(b, hipEventCreate(&t1));
+    // device time of `pairs` replays of the two-substep fused graph, after one untimed
+    // replay (which captures the graph for the candidate)
+    auto timed = [&](double* ms) -> int {
+        int rc = ckmpm_batch_run(b, 2, 4);
+        if (rc != CKMPM_OK) return rc;
+        if (hipEventRecord(t0, b->cs) != hipSuccess) return BatchFail(b, CKMPM_ERR_HIP, "hipEventRecord");
+        rc = ckmpm_batch_run(b, 2 * pairs, 4);
+        if (rc != CKMPM_OK) return rc;
+        float f = 0.0f;
+        if (hipEventRecord(t1, b->cs) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+            hipEventElapsedTime(&f, t0, t1) != hipSuccess)
+            return BatchFail(b, CKMPM_ERR_HIP, "autotune timing");
+        *ms = f;
+        return CKMPM_OK;
+    };
+    auto set_kind = [&](int kind, int threads) {
+        for (ckmpm_engine* e : b->eng)
+            for (size_t i = 0; i < e->models.size(); ++i)
+                if (e->models[i].kind == kind) (void)ckmpm_set_tuning(e, static_cast<int32_t>(i), threads);
+    };
+    auto set_serial = [&](bool serial) { (void)ckmpm_batch_set_serial(b, serial ? 1 : 0); };
+    std::vector<int> kinds;
+    for (ckmpm_engine* e : b->eng)
+        for (const ModelBuf& m : e->models)
+            if (std::find(kinds.begin(), kinds.end(), m.kind) == kinds.end()) kinds.push_back(m.kind);
+    int rc = CKMPM_OK;
+    // one coordinate at a time: each kind's workgroup size, then the G2P2G launch order
+    for (int kind : kinds) {
+        double best = 0.0;
+        int best_threads = 256;
+        for (int threads : {256, 512}) {
+            set_kind(kind, threads);
+            double ms = 0.0;
+            if ((rc = timed(&ms)) != CKMPM_OK) break;
+            if (threads == 256 || ms < best) {
+                best = ms;
+                best_threads = threads;
+            }
+        }
+        if (rc != CKMPM_OK) break;
+        set_kind(kind, best_threads);
+    }
+    if (rc == CKMPM_OK) {
+        double side = 0.0, serial = 0.0;
+        set_serial(false);
+        rc = timed(&side);
+        if (rc == CKMPM_OK) {
+            set_serial(true);
+            rc = timed(&serial);
+        }
+        if (rc == CKMPM_OK) set_serial(serial < side);
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc != CKMPM_OK) return rc;
+    return ckmpm_batch_synchronize(b);
+}
+
+int ckmpm_batch_get_serial(const ckmpm_batch* b, int32_t* serial) {
+    if (!b || !serial) return CKMPM_ERR_INVALID;
+    *serial = b->g2p2g_serial ? 1 : 0;
+    return CKMPM_OK;
+}
+
+int ckmpm_batch_set_serial(ckmpm_batch* b, int32_t serial) {
+    if (!b) return CKMPM_ERR_INVALID;
+    if (b->g2p2g_serial != (serial != 0)) {
+        BATCH_HIP(b, hipStreamSynchronize(b->cs));  // the graphs are rebuilt for the new order
+        b->g2p2g_serial = serial != 0;
+        for (int p = 0; p < 2; ++p) {
+            ReleaseGraph(b->fz[p]);
+            ReleaseGraph(b->fz1[p]);
+        }
+    }
     return CKMPM_OK;
 }
 

@@ -371,6 +371,22 @@ class MPMBatch:
         self._check(self._lib.ckmpm_batch_captures(self._h, ctypes.byref(c)), "ckmpm_batch_captures")
         return int(c.value)
 
+    def autotune(self, pairs: int = 3):
+        """Times the fused mode's candidates (each material kind's G2P2G workgroup size,
+        then side-by-side or serial G2P2G launches) and keeps the fastest; advances the
+        engines 2 (pairs + 1) substeps per candidate."""
+        self._check(self._lib.ckmpm_batch_autotune(self._h, int(pairs)), "ckmpm_batch_autotune")
+
+    @property
+    def serial_g2p2g(self) -> bool:
+        v = ctypes.c_int32()
+        self._check(self._lib.ckmpm_batch_get_serial(self._h, ctypes.byref(v)), "ckmpm_batch_get_serial")
+        return bool(v.value)
+
+    @serial_g2p2g.setter
+    def serial_g2p2g(self, on: bool):
+        self._check(self._lib.ckmpm_batch_set_serial(self._h, int(bool(on))), "ckmpm_batch_set_serial")
+
     @property
     def n_particles(self) -> int:
         return int(sum(e.n(m) for e in self.engines for m in range(len(e.models))))

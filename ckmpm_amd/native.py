@@ -34,6 +34,7 @@ EXPORTS = [
     "ckmpm_hash_find_device", "ckmpm_hash_synchronize", "ckmpm_hash_info", "ckmpm_hash_export",
     "ckmpm_hash_destroy", "ckmpm_hash_last_error", "ckmpm_hash_build_noalloc",
     "ckmpm_batch_create", "ckmpm_batch_run", "ckmpm_batch_synchronize", "ckmpm_batch_captures",
+    "ckmpm_batch_autotune", "ckmpm_batch_get_serial", "ckmpm_batch_set_serial",
     "ckmpm_batch_destroy", "ckmpm_batch_last_error",
     "ckmpm_adjoint_enable", "ckmpm_checkpoint_save", "ckmpm_checkpoint_restore", "ckmpm_checkpoint_free",
     "ckmpm_adjoint_record", "ckmpm_adjoint_reset", "ckmpm_adjoint_seed", "ckmpm_adjoint_substep",
@@ -150,6 +151,9 @@ def load():
     L.ckmpm_batch_run.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int32]
     L.ckmpm_batch_synchronize.argtypes = [_vp]
     L.ckmpm_batch_captures.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.ckmpm_batch_autotune.argtypes = [_vp, ctypes.c_uint32]
+    L.ckmpm_batch_get_serial.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32)]
+    L.ckmpm_batch_set_serial.argtypes = [_vp, ctypes.c_int32]
     L.ckmpm_batch_destroy.argtypes = [_vp]
     L.ckmpm_batch_last_error.argtypes = [_vp]
     L.ckmpm_batch_last_error.restype = ctypes.c_char_p

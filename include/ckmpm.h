@@ -315,6 +315,16 @@ int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode);
 /* waits for every engine; returns the first engine error */
 int ckmpm_batch_synchronize(ckmpm_batch* b);
 int ckmpm_batch_captures(const ckmpm_batch* b, uint64_t* captures);
+/* Tuning of mode 4 (no reference counterpart): per material kind in the batch, the G2P2G
+ * workgroup size (256 or 512 threads) of every model of that kind, then whether a
+ * substep's G2P2G launches (one per kind and size) run side by side or one after the
+ * other.  Each candidate is timed with HIP events over `pairs` replays of the two-substep
+ * fused graph after one untimed replay; the engines advance 2 (pairs + 1) substeps per
+ * candidate (2 kinds + 2 candidates in all).  The choice stays with the batch and its
+ * engines (ckmpm_get_tuning, ckmpm_batch_get_serial). */
+int ckmpm_batch_autotune(ckmpm_batch* b, uint32_t pairs);
+int ckmpm_batch_get_serial(const ckmpm_batch* b, int32_t* serial);
+int ckmpm_batch_set_serial(ckmpm_batch* b, int32_t serial);
 int ckmpm_batch_destroy(ckmpm_batch* b);
 const char* ckmpm_batch_last_error(const ckmpm_batch* b);
 

@@ -113,7 +113,8 @@ def test_sequential_and_stream_modes_match_graph():
 
 
 @pytest.mark.gpu
-def test_fused_batch_variants_match_separate_engines():
+@pytest.mark.parametrize("serial", [False, True])
+def test_fused_batch_variants_match_separate_engines(serial):
     """The fused launches' variants: a custom boundary (second grid-update instantiation),
     two models of one engine in one G2P2G launch (two fixed-corotated spheres, with the
     corner box's model: three entries), two kinds of one engine (fluid + sand) and
@@ -140,6 +141,7 @@ def test_fused_batch_variants_match_separate_engines():
         e.Synchronize()
     together = engines()
     batch = MPMBatch(together)
+    batch.serial_g2p2g = serial
     batch.Run(substeps, MPMBatch.FUSED)
     batch.Synchronize()
     assert batch.captures == _graphs_captured([substeps], True)
@@ -274,3 +276,32 @@ def test_single_engine_fused_graphs_match_plain_launches_at_8m():
     batch.close()
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_autotune_runs_every_candidate_and_keeps_one():
+    """The fused mode's tuner: each kind's G2P2G workgroup size, then side-by-side or
+    serial G2P2G launches, each candidate timed over replays of the fused graph (the
+    configurations it chooses among are checked against separate engines in
+    test_fused_batch_variants_match_separate_engines)."""
+    from ckmpm_amd.engine import MPMBatch, engine_from_scene
+
+    engs = [engine_from_scene(sc) for sc in (scenes.colliding_spheres(),
+                                              scenes.mixed_dam_sand(n_fluid=9_000, n_sand=5_000, scale=8))]
+    batch = MPMBatch(engs)
+    batch.Run(1, MPMBatch.FUSED)
+    print("batch autotune: spheres (FC) + dam (fluid, sand), 1 timed pair per candidate", flush=True)
+    batch.autotune(1)
+    batch.Run(3, MPMBatch.FUSED)
+    batch.Synchronize()
+    # 3 kinds x 2 sizes + 2 launch orders, 2 (1 + 1) substeps each
+    assert all(e.counters().substeps == 1 + 4 * (2 * 3 + 2) + 3 for e in engs)
+    for e in engs:
+        assert e.counters().flags == 0
+        for m in range(len(e.models)):
+            assert e.tuning(m) in (256, 512)
+            assert np.isfinite(e.state(m)).all()
+    # one size per kind: the two fixed-corotated spheres share theirs
+    assert engs[0].tuning(0) == engs[0].tuning(1)
+    assert isinstance(batch.serial_g2p2g, bool)
+    batch.close()
