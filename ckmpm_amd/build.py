@@ -12,7 +12,7 @@ LIB = os.path.join(HERE, "libckmpm.so")
 DRIVER = os.path.join(HERE, "ckmpm_scene")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 CXX = os.environ.get("CXX", "g++")
-SOURCES = ["kernels.hip", "engine.hip", "hash.hip", "adjoint.hip", "conservation.hip"]
+SOURCES = ["kernels.hip", "engine.hip", "hash.hip", "adjoint.hip", "conservation.hip", "setup_wide.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

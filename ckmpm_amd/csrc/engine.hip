@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "engine_impl.h"
+#include "setup_wide.h"
 
 using namespace ckmpm;
 
@@ -289,8 +290,8 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
     for (int a = 0; a < 3; ++a)
         if (domain->dim[a] <= 0 || domain->dim[a] > 1024) return CKMPM_ERR_INVALID;
     if (!(sim->dx > 0.0f) || sim->block_capacity == 0) return CKMPM_ERR_INVALID;
+    // at most 1024^3 = 2^30 blocks: a block key fits 32 bits with room for the empty marker
     const uint64_t nblocks = static_cast<uint64_t>(domain->dim[0]) * domain->dim[1] * domain->dim[2];
-    if (nblocks > (1ull << 26)) return CKMPM_ERR_INVALID;  // block key << 6 | cell fits 32 bits
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CKMPM_ERR_NO_DEVICE;
     ckmpm_engine* e = new ckmpm_engine();
@@ -475,7 +476,10 @@ int ckmpm_initial_setup(ckmpm_engine* e) {
         sorted[i] = keys;
         uint32_t* vals = m.perm[0];
         CK_HIP(e, hipMemcpyAsync(xyz, m.x0.data(), 3 * sizeof(float) * m.n, hipMemcpyHostToDevice, s));
-        SetupOrder(e->dom, xyz, m.n, key_bits, keys, vals, tmp, e->bitmap, e->status, s);
+        // the cell key block << 6 | cell fits 32 bits up to 2^26 blocks; larger domains
+        // sort by cell and block in two passes
+        if (key_bits + 6 <= 32) SetupOrder(e->dom, xyz, m.n, key_bits, keys, vals, tmp, e->bitmap, e->status, s);
+        else SetupOrderWide(e->dom, xyz, m.n, key_bits, keys, vals, tmp, e->bitmap, e->status, s);
         AdjointOnSetupOrder(e, i, vals, xyz);
         if (m.pp_attr) {
             CK_HIP(e, hipMemcpyAsync(lam, m.lam0.data(), sizeof(float) * m.n, hipMemcpyHostToDevice, s));

@@ -54,7 +54,8 @@ enum {
 };
 
 /* MPMDomain<MPMDomainRange<Dx,Dy,Dz>, MPMDomainOffset<Ox,Oy,Oz>> (mpm_domain.h:9-61):
- * extent and offset in 4^3-cell blocks */
+ * extent and offset in 4^3-cell blocks; each extent 1..1024 (up to 2^30 blocks: the two
+ * dense block tables of an engine then take 8 GiB) */
 typedef struct {
     int32_t dim[3];
     int32_t offset[3];

@@ -45,11 +45,21 @@ def test_invalid_descriptors_are_refused_before_any_device_call():
     lib = native.load()
     good_d, good_s = ((16, 16, 16), (0, 0, 0)), (1 / 64, 0, 0, 0.5, 1e-4, 48, 4096)
     cases = [(((0, 16, 16), (0, 0, 0)), good_s), (((16, 2048, 16), (0, 0, 0)), good_s),
-             (((1024, 1024, 128), (0, 0, 0)), good_s),
+             (((1025, 16, 16), (0, 0, 0)), good_s),
              (good_d, (0.0, 0, 0, 0.5, 1e-4, 48, 4096)), (good_d, (1 / 64, 0, 0, 0.5, 1e-4, 48, 0))]
     for (dim, off), sim in cases:
         d = native.DomainDesc((ctypes.c_int32 * 3)(*dim), (ctypes.c_int32 * 3)(*off))
         s = native.SimDesc(*sim)
         h = ctypes.c_void_p()
         assert lib.ckmpm_create(ctypes.byref(d), ctypes.byref(s), ctypes.byref(h)) == native.ERR_INVALID, (dim, sim)
+        assert not h
+    # the largest domain, 1024^3 blocks, passes the descriptor checks (a key stays 32-bit);
+    # only where there is no device, which then refuses it (a device would allocate 8 GiB)
+    import torch
+
+    if torch.cuda.device_count() == 0:
+        d = native.DomainDesc((ctypes.c_int32 * 3)(1024, 1024, 1024), (ctypes.c_int32 * 3)(0, 0, 0))
+        h = ctypes.c_void_p()
+        assert lib.ckmpm_create(ctypes.byref(d), ctypes.byref(native.SimDesc(*good_s)), ctypes.byref(h)) in (
+            native.ERR_NO_DEVICE, native.ERR_HIP)
         assert not h

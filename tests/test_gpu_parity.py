@@ -98,6 +98,27 @@ def test_colliding_spheres_lockstep():
     _lockstep(sc, 8, lambda k: (0.0 if k == 0 else 1e-4, 1e-4), (2e-7, 1e-5))
 
 
+@pytest.mark.gpu
+def test_wide_domain_lockstep():
+    """A domain of 2^28 blocks (1024 x 512 x 512): the block key no longer fits beside the
+    cell in one 32-bit word, so the setup sorts by cell and by block in two passes. The
+    colliding spheres sit 600 blocks along x (keys above 1.5e8): setup and 4 substeps in
+    lockstep with the oracle."""
+    import copy
+
+    sc = scenes.colliding_spheres()
+    big = copy.copy(sc)
+    big.dim = (1024, 512, 512)
+    shift = np.array([600 * 4 * sc.dx, 0.0, 0.0], np.float32)
+    big.models = []
+    for m in sc.models:
+        mm = copy.copy(m)
+        mm.x = (m.x + shift).astype(np.float32)
+        big.models.append(mm)
+    print(f"wide domain: {np.prod(big.dim)} blocks, {big.n_particles} particles", flush=True)
+    _lockstep(big, 4, lambda k: (0.0 if k == 0 else 1e-4, 1e-4), (2e-6, 1e-5))
+
+
 def _ids_after(eng_or_orc, ids, nm, is_gpu):
     out = []
     for m in range(nm):

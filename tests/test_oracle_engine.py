@@ -209,10 +209,27 @@ def test_rebin_matches_numpy_restatement():
     assert o.flags == 0
 
 
-def test_setup_order_matches_numpy_restatement():
+def _wide_spheres():
+    """The colliding spheres 600 blocks along x of a 1024 x 512 x 512-block domain (2^28
+    blocks, keys above 1.5e8: beyond a 32-bit block << 6 | cell key)."""
+    import copy
+
+    sc = scenes.colliding_spheres()
+    big = copy.copy(sc)
+    big.dim = (1024, 512, 512)
+    big.models = []
+    for m in sc.models:
+        mm = copy.copy(m)
+        mm.x = (m.x + np.array([600 * 4 * sc.dx, 0.0, 0.0], np.float32)).astype(np.float32)
+        big.models.append(mm)
+    return big
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_setup_order_matches_numpy_restatement(wide):
     """Setup order = sort by (block, rank of the particle among the earlier input
     particles of its binning cell, cell within the block)."""
-    sc = scenes.mixed_dam_sand(n_fluid=9000, n_sand=5000, scale=8)
+    sc = _wide_spheres() if wide else scenes.mixed_dam_sand(n_fluid=9000, n_sand=5000, scale=8)
     o = Oracle.from_scene(sc)
     o.setup()
     dim = sc.dim
@@ -235,6 +252,8 @@ def test_setup_order_matches_numpy_restatement():
         expected = np.lexsort((cb, rank, key))
         got = o.setup_order(mi)
         np.testing.assert_array_equal(got, expected)
+        if wide:
+            assert key.min() > (1 << 26)
         # consecutive particles of a block sit in distinct cells while the block has
         # unused cells left for that rank
         assert (np.diff(ck[got]) != 0).mean() > 0.95
