@@ -149,33 +149,6 @@ void PhaseGrid(ckmpm_engine* e, double dt, FusedRec* rec = nullptr) {
     Mark(e);
 }
 
-// particle blocks from which the rebin's layout extension runs on a side stream
-constexpr uint32_t kForkBlocks = 8192;
-
-// Whether work of this substep may fork to the side stream: large layouts only (the fork
-// and join cost ~13 us, a loss for small scenes) and never under graph capture (a
-// two-stream fork captured into the batch's graphs crashed in the HIP runtime).  The
-// side stream is created on first use, so engines that never fork (small or batched
-// scenes) hold one stream and keep their hardware-queue assignment.
-bool SideStreamReady(ckmpm_engine* e, hipStream_t s) {
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    (void)hipStreamIsCapturing(s, &cap);
-    bool fork = cap == hipStreamCaptureStatusNone && e->a_hint >= kForkBlocks && !e->side_failed;
-    if (fork && !e->side) {
-        if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&e->join_ev, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();  // no side stream: stay on one
-            if (e->side) (void)hipStreamDestroy(e->side);
-            if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
-            e->side = nullptr;
-            e->fork_ev = nullptr;
-            e->side_failed = true;
-        }
-    }
-    return fork && e->side && e->fork_ev && e->join_ev;
-}
-
 // G2P2G(dt, next_dt) for every model, then the rebin into the next layout.
 void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = nullptr) {
     hipStream_t s = e->stream;
@@ -185,7 +158,7 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
     const int sc = e->sc, sn = 1 - e->sc;
     const uint32_t grid = GridFor(e->a_hint + e->a_hint / 4 + 8);
     // The models' launches run one after the other on the engine stream.  Running the
-    // second model's on the side stream, concurrently with the first (the launches are
+    // second model's on a second stream, concurrently with the first (the launches are
     // independent; their grid deposits are commutative atomic adds), measured slower on
     // the 8M fluid + sand scene: 0.614 against 0.607 ms.
     for (size_t mi = 0; mi < e->models.size(); ++mi) {
@@ -244,23 +217,17 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
     Run(rec, kStScan, kFkRebinScan, 0, sa, RebinScanGrid(e->cap), s);
     // Two independent chains from here: the particle permutation (RebinScatter: current
     // layout, orderings) and the extension of the next layout (its neighbour and exterior
-    // blocks, map, tile counts).  For large layouts the extension runs on the side stream,
-    // beside the scatter.
-    const bool fork = !rec && SideStreamReady(e, s);
-    if (fork) {
-        (void)hipEventRecord(e->fork_ev, s);
-        (void)hipStreamWaitEvent(e->side, e->fork_ev, 0);
-        ExtendLayout(e, P, false, e->side);
-    }
+    // blocks, map, tile counts).  The scatter and the extension's mark pass share one
+    // launch (the first workgroups mark), then the emit pass runs.
+    const uint32_t hint = e->a_hint ? e->a_hint + e->a_hint / 4 + 64u : e->cap;
+    ScatterMarkArgs sm{};
+    sm.x = RebinScatterArgs{e->dom, C.keys, C.table, C.lc, rm, e->cap};
+    sm.m = MarkArgs{e->dom, P.keys, P.lc, e->cap, P.table, e->ext_map, e->ext_cnt};
+    sm.mark_wgs = MarkGrid(e->cap, hint);
     // a wavefront per particle block, 4 per workgroup
-    const RebinScatterArgs xa{e->dom, C.keys, C.table, C.lc, rm, e->cap};
-    Run(rec, kStScatter, kFkRebinScatter, 0, xa, GridFor((e->a_hint + e->a_hint / 4) / 4u + 8u), s);
-    if (fork) {
-        (void)hipEventRecord(e->join_ev, e->side);
-        (void)hipStreamWaitEvent(s, e->join_ev, 0);
-    } else {
-        ExtendLayout(e, P, false, s, rec);
-    }
+    Run(rec, kStScatter, kFkScatterMark, 0, sm, sm.mark_wgs + GridFor((e->a_hint + e->a_hint / 4) / 4u + 8u), s);
+    const EmitArgs em{e->ext_map, e->ext_cnt, e->ext_tiles, P.keys, P.table, P.lc, e->cap, e->status};
+    Run(rec, kStEmit, kFkEmit, 0, em, e->ext_tiles, s);
     Mark(e);
     FlipRoles(e);
     e->substeps++;
@@ -827,9 +794,6 @@ int ckmpm_destroy(ckmpm_engine* e) {
             if (ev) (void)hipEventDestroy(ev);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
-    if (e->side) (void)hipStreamDestroy(e->side);
-    if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
-    if (e->join_ev) (void)hipEventDestroy(e->join_ev);
     delete e;
     return CKMPM_OK;
 }
@@ -981,9 +945,8 @@ int CaptureEngine(ckmpm_batch* b, ckmpm_engine* e, BatchGraph& g) {
 // Two substeps of every engine as fused launches: each (kernel, variant) of a substep
 // runs once for all scenes, its workgroups split by scene (LaunchFused), from argument
 // tables built here on the host and copied once.  The stages run in order on the capture
-// stream, so every engine's launches keep their order (independent launches run side by
-// side: the G2P2G launches of a substep, and the scatter beside the layout extension);
-// the scenes' buffers are disjoint.
+// stream, so every engine's launches keep their order (the G2P2G launches of a substep,
+// which are independent, run side by side); the scenes' buffers are disjoint.
 // Captures nsub (1 or 2) substeps of every engine as fused launches.
 int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
     ReleaseGraph(g);
@@ -1073,19 +1036,6 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
     };
     BATCH_HIP(b, hipStreamBeginCapture(b->cs, hipStreamCaptureModeRelaxed));
     for (size_t j = 0; j < launches.size();) {
-        if (launches[j].stage == kStScatter) {
-            // after the scan, two independent chains: the scatter (particle permutation)
-            // on the capture stream, the layout extension on an auxiliary stream
-            size_t k = j;
-            while (k < launches.size() && launches[k].stage >= kStScatter && launches[k].stage <= kStEmit) ++k;
-            (void)hipEventRecord(b->fork, b->cs);
-            (void)hipStreamWaitEvent(b->aux[0], b->fork, 0);
-            for (size_t q = j; q < k; ++q) launch(launches[q], launches[q].stage >= kStMark ? b->aux[0] : b->cs);
-            (void)hipEventRecord(b->aux_done[0], b->aux[0]);
-            (void)hipStreamWaitEvent(b->cs, b->aux_done[0], 0);
-            j = k;
-            continue;
-        }
         size_t k = j;
         while (k < launches.size() && launches[k].kernel == kFkG2P2G) ++k;
         if (k - j < 2 || b->g2p2g_serial) {

@@ -59,12 +59,6 @@ struct ckmpm_engine {
     ckmpm_sim_desc sim{};
     uint32_t cap = 0;
     hipStream_t stream = nullptr;
-    // second stream for the rebin's layout extension of large layouts, forked from and
-    // joined back to `stream` inside a substep (default priority, like `stream`; created
-    // on first use)
-    hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-    bool side_failed = false;  // creating them failed: stay on one stream
     LayoutBuf L[2];
     int cur = 0;          // current layout; 1 - cur is the previous one
     float* M[2] = {nullptr, nullptr};
@@ -124,8 +118,8 @@ inline int Fail(ckmpm_engine* e, int code, const std::string& msg) {
 // the replay).  The runtime shares a few hardware queues among the process's streams,
 // and two streams on one queue run one after the other; a stream created with a CU mask
 // (here every CU) gets a queue of its own.  Such a queue is not free: while one exists,
-// the 8M-particle substep's graph replays ran 7% slower (0.709 -> 0.757 ms), so the
-// layout extension's side stream stays an ordinary one.  Falls back to a shared stream.
+// the 8M-particle substep's graph replays ran 7% slower (0.709 -> 0.757 ms), so it is
+// used only where the overlap pays for that.  Falls back to a shared stream.
 inline hipError_t CreateOwnQueueStream(hipStream_t* s) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&

@@ -204,6 +204,13 @@ struct EmitArgs {
     uint32_t cap;
     Status* st;
 };
+// RebinScatter and the extension's Mark in one launch (both need only the scan's
+// outputs): workgroups [0, mark_wgs) mark, the others scatter; the emit pass follows.
+struct ScatterMarkArgs {
+    RebinScatterArgs x;
+    MarkArgs m;
+    uint32_t mark_wgs;
+};
 
 // Kernels of one substep.  The grid-update and G2P2G entries have variants (the boundary
 // instantiation; material kind x workgroup size of 256 or 512 threads).
@@ -215,6 +222,7 @@ enum FusedKernel : int {
     kFkRebinScatter,
     kFkMark,
     kFkEmit,
+    kFkScatterMark,
 };
 size_t FusedArgsSize(int kernel);  // bytes of the kernel's argument struct
 // One launch of grid workgroups; args points to the kernel's argument struct.

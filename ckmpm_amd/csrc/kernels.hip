@@ -1552,10 +1552,11 @@ __device__ __forceinline__ void RebinScanBody(const RebinScanArgs& a, uint32_t w
 __global__ __launch_bounds__(kThreads) void RebinScanKernel(RebinScanArgs a) { RebinScanBody(a, blockIdx.x, gridDim.x); }
 
 // Particle permutation of the next layout: a wavefront per block j of the current
-// layout, for every model.  Lane d < 27 first finds the destination e = j + d (an
-// exterior block of the current layout); per model, the first destination of move d is
-// e's offset in the next layout (eoff) plus base[m][e][d] (the particles entering e with
-// an earlier move), and the particle loop takes it by lane shuffle.
+// layout, for every model.  Lane d < 27 finds the destination e = j + d (an exterior
+// block of the current layout); per model, the first destination of move d is e's offset
+// in the next layout (eoff) plus base[m][e][d] (the particles entering e with an earlier
+// move), and the particle loop takes it by lane shuffle.  The destination lookups start
+// after the first pass's move codes are requested, so the two chains of loads overlap.
 // kScatterUnroll particles per lane are in flight (a block holds a few hundred
 // particles per model, so most blocks take one pass of dependent loads: 512 per pass).
 constexpr int kScatterUnroll = 8;
@@ -1565,15 +1566,7 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
     const uint32_t lane = LaneId(), waves = kThreads / 64;
     for (uint32_t j = wg * waves + (threadIdx.x >> 6); j < A; j += ng * waves) {
         uint32_t e = kEmpty;
-        if (lane < static_cast<uint32_t>(kDirections)) {
-            int bx, by, bz, ox, oy, oz;
-            CoordsOf(a.d, a.keys[j], bx, by, bz);
-            DirOffset(static_cast<int>(lane), ox, oy, oz);
-            bx += ox;
-            by += oy;
-            bz += oz;
-            if (InDomain(a.d, bx, by, bz)) e = a.table[KeyOf(a.d, bx, by, bz)];
-        }
+        bool have_e = false;  // wave-uniform
         for (int m = 0; m < rm.nmodels; ++m) {
             const uint32_t* off = rm.off_cur[m];
             const uint32_t beg = off[j], end = off[j + 1];
@@ -1583,7 +1576,6 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
             const uint32_t n = rm.n[m];
             // a move without a destination (no particle takes it) keeps n: its positions drop
             uint32_t first = n;
-            if (e < cap) first = rm.eoff[m][e] + rm.base[m][static_cast<size_t>(e) * kDirections + lane];
             for (uint32_t t0 = beg; t0 < end; t0 += kScatterUnroll * 64) {
                 uint32_t c[kScatterUnroll];
 #pragma unroll
@@ -1591,6 +1583,20 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
                     const uint32_t t = t0 + u * 64 + lane;
                     c[u] = t < end ? code[t] : 31u;
                 }
+                if (!have_e) {
+                    if (lane < static_cast<uint32_t>(kDirections)) {
+                        int bx, by, bz, ox, oy, oz;
+                        CoordsOf(a.d, a.keys[j], bx, by, bz);
+                        DirOffset(static_cast<int>(lane), ox, oy, oz);
+                        bx += ox;
+                        by += oy;
+                        bz += oz;
+                        if (InDomain(a.d, bx, by, bz)) e = a.table[KeyOf(a.d, bx, by, bz)];
+                    }
+                    have_e = true;
+                }
+                if (t0 == beg && e < cap)
+                    first = rm.eoff[m][e] + rm.base[m][static_cast<size_t>(e) * kDirections + lane];
 #pragma unroll
                 for (int u = 0; u < kScatterUnroll; ++u) {
                     const uint32_t dir = c[u] & 31u;
@@ -1605,6 +1611,16 @@ __device__ __forceinline__ void RebinScatterBody(const RebinScatterArgs& a, uint
 }
 __global__ __launch_bounds__(kThreads) void RebinScatterKernel(RebinScatterArgs a) {
     RebinScatterBody(a, blockIdx.x, gridDim.x);
+}
+
+// The scatter and the extension's mark pass side by side in one launch: a fork to a
+// second stream and its join cost more than either kernel's tail.
+__device__ __forceinline__ void ScatterMarkBody(const ScatterMarkArgs& a, uint32_t wg, uint32_t ng) {
+    if (wg < a.mark_wgs) MarkExtensionBody(a.m, wg, a.mark_wgs);
+    else RebinScatterBody(a.x, wg - a.mark_wgs, ng - a.mark_wgs);
+}
+__global__ __launch_bounds__(kThreads) void ScatterMarkKernel(ScatterMarkArgs a) {
+    ScatterMarkBody(a, blockIdx.x, gridDim.x);
 }
 
 // rank of each element inside its run of equal keys (from run heads and their positions)
@@ -1766,6 +1782,9 @@ void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStre
             RebinScatterKernel<<<grid, kThreads, 0, s>>>(*static_cast<const RebinScatterArgs*>(args));
             break;
         case kFkMark: MarkExtensionKernel<<<grid, kThreads, 0, s>>>(*static_cast<const MarkArgs*>(args)); break;
+        case kFkScatterMark:
+            ScatterMarkKernel<<<grid, kThreads, 0, s>>>(*static_cast<const ScatterMarkArgs*>(args));
+            break;
         default: BitmapScanEmitKernel<<<grid, kThreads, 0, s>>>(*static_cast<const EmitArgs*>(args)); break;
     }
 }
@@ -1826,6 +1845,7 @@ CKMPM_FUSED(RebinScanFused, RebinScanBody, RebinScanArgs, kThreads)
 CKMPM_FUSED(RebinScatterFused, RebinScatterBody, RebinScatterArgs, kThreads)
 CKMPM_FUSED(MarkExtensionFused, MarkExtensionBody, MarkArgs, kThreads)
 CKMPM_FUSED(BitmapScanEmitFused, BitmapScanEmitBody, EmitArgs, kThreads)
+CKMPM_FUSED(ScatterMarkFused, ScatterMarkBody, ScatterMarkArgs, kThreads)
 #undef CKMPM_FUSED
 
 size_t FusedArgsSize(int kernel) {
@@ -1836,6 +1856,7 @@ size_t FusedArgsSize(int kernel) {
         case kFkRebinScan: return sizeof(RebinScanArgs);
         case kFkRebinScatter: return sizeof(RebinScatterArgs);
         case kFkMark: return sizeof(MarkArgs);
+        case kFkScatterMark: return sizeof(ScatterMarkArgs);
         default: return sizeof(EmitArgs);
     }
 }
@@ -1879,6 +1900,9 @@ void LaunchFused(int kernel, int variant, const void* table, const uint32_t* beg
             break;
         case kFkMark:
             MarkExtensionFused<<<total, kThreads, 0, s>>>(static_cast<const MarkArgs*>(table), begin, n);
+            break;
+        case kFkScatterMark:
+            ScatterMarkFused<<<total, kThreads, 0, s>>>(static_cast<const ScatterMarkArgs*>(table), begin, n);
             break;
         default:
             BitmapScanEmitFused<<<total, kThreads, 0, s>>>(static_cast<const EmitArgs*>(table), begin, n);
