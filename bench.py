@@ -546,6 +546,8 @@ def main():
         out["fwd_bwd_over_fwd"] = ad["fwd_bwd_over_fwd"]
         out["fwd_bwd"] = {"workload": "cantilever", "particles": ad["config"]["particles"],
                           "fwd_ms_per_step": ad["fwd"]["ms_per_step"], "fwd_bwd_ms_per_step": ad["ms_per_step"],
+                          "fwd_samples_ms_per_step": ad["fwd"]["samples_ms_per_step"],
+                          "fwd_bwd_samples_ms_per_step": ad["samples_ms_per_step"],
                           "checkpoint_every": 25, "timing": ad["timing"]}
         bargs = argparse.Namespace(**{**vars(args), "steps": 40, "warmup": 4})
         bt = bench_batch(bargs, torch, only=("batched", "sequential"))

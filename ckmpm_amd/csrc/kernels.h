@@ -58,7 +58,7 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
 // ---------------------------------------------------------------- layout
 // Map words per emit tile: the extension map is allocated in whole tiles (zero padded),
 // with two counters per tile (neighbour and exterior blocks marked).
-constexpr uint32_t kEmitTileWords = 2048;
+constexpr uint32_t kEmitTileWords = 512;
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
