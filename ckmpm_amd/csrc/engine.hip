@@ -87,10 +87,10 @@ void ExtendLayout(ckmpm_engine* e, LayoutBuf& L, bool insert_a, hipStream_t s, F
     if (insert_a) LaunchTableInsertA(L.keys, L.lc, e->cap, L.table, s);
     // marking grids sized for the expected block count (the capacity before the first read)
     const uint32_t hint = e->a_hint ? e->a_hint + e->a_hint / 4 + 64u : e->cap;
-    const MarkArgs m{e->dom, L.keys, L.lc, e->cap, L.table, e->ext_map, e->ext_cnt};
+    const MarkArgs m{e->dom, L.keys, L.lc, e->cap, L.table, e->ext_map, e->ext_cnt, e->ext_tile_shift};
     Run(rec, kStMark, kFkMark, 0, m, MarkGrid(e->cap, hint), s);
     const EmitArgs em{e->ext_map, e->ext_cnt, e->ext_tiles, L.keys, L.table, L.lc, e->cap, e->status};
-    Run(rec, kStEmit, kFkEmit, 0, em, e->ext_tiles, s);
+    Run(rec, kStEmit, kFkEmit, e->emit_variant, em, e->ext_tiles, s);
 }
 
 void Mark(ckmpm_engine* e) {
@@ -143,9 +143,8 @@ void PhaseGrid(ckmpm_engine* e, double dt, FusedRec* rec = nullptr) {
     g.bc = e->bc;
     g.custom = e->bc_custom ? 1 : 0;
     g.slot = static_cast<int>(e->substeps & 1u);
-    // 2 blocks x 8 chains per workgroup and pass: 0.094 A workgroups cover Nb (about 1.2 A).
-    // 8 chains per thread instead of 4: 0.044 -> 0.038 ms on the 8M scene.
-    Run(rec, kStGrid, kFkGrid, g.custom, g, GridFor(e->a_hint * 3u / 32u + 8u), s);
+    // 8 blocks x 4 chains per workgroup and pass: 0.047 A workgroups cover Nb (about 1.2 A)
+    Run(rec, kStGrid, kFkGrid, g.custom, g, GridFor(e->a_hint * 3u / 64u + 8u), s);
     Mark(e);
 }
 
@@ -222,12 +221,12 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
     const uint32_t hint = e->a_hint ? e->a_hint + e->a_hint / 4 + 64u : e->cap;
     ScatterMarkArgs sm{};
     sm.x = RebinScatterArgs{e->dom, C.keys, C.table, C.lc, rm, e->cap};
-    sm.m = MarkArgs{e->dom, P.keys, P.lc, e->cap, P.table, e->ext_map, e->ext_cnt};
+    sm.m = MarkArgs{e->dom, P.keys, P.lc, e->cap, P.table, e->ext_map, e->ext_cnt, e->ext_tile_shift};
     sm.mark_wgs = MarkGrid(e->cap, hint);
     // a wavefront per particle block, 4 per workgroup
     Run(rec, kStScatter, kFkScatterMark, 0, sm, sm.mark_wgs + GridFor((e->a_hint + e->a_hint / 4) / 4u + 8u), s);
     const EmitArgs em{e->ext_map, e->ext_cnt, e->ext_tiles, P.keys, P.table, P.lc, e->cap, e->status};
-    Run(rec, kStEmit, kFkEmit, 0, em, e->ext_tiles, s);
+    Run(rec, kStEmit, kFkEmit, e->emit_variant, em, e->ext_tiles, s);
     Mark(e);
     FlipRoles(e);
     e->substeps++;
@@ -288,7 +287,10 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
     check(Alloc(e, &e->bc, 1));
     check(Alloc(e, &e->bitmap, e->nwords));
     e->ext_tiles = ExtTiles(nblocks);
-    check(Alloc(e, &e->ext_map, static_cast<size_t>(e->ext_tiles) * kEmitTileWords));
+    e->ext_tile_words = ExtTileWords(nblocks);
+    e->ext_tile_shift = ExtTileShift(nblocks);
+    e->emit_variant = EmitVariant(nblocks);
+    check(Alloc(e, &e->ext_map, static_cast<size_t>(e->ext_tiles) * e->ext_tile_words));
     check(Alloc(e, &e->ext_cnt, 2 * static_cast<size_t>(e->ext_tiles)));
     check(Alloc(e, &e->tsum, RebinTileSumWords(e->cap)));
     check(Alloc(e, &e->word_off, e->nwords));
@@ -300,7 +302,7 @@ int ckmpm_create(const ckmpm_domain_desc* domain, const ckmpm_sim_desc* sim, ckm
             (void)hipMemsetAsync(e->L[i].lc, 0, sizeof(LayoutCounts), e->stream);
         }
         (void)hipMemsetAsync(e->bitmap, 0, static_cast<size_t>(e->nwords) * sizeof(uint32_t), e->stream);
-        (void)hipMemsetAsync(e->ext_map, 0, static_cast<size_t>(e->ext_tiles) * kEmitTileWords * sizeof(uint32_t),
+        (void)hipMemsetAsync(e->ext_map, 0, static_cast<size_t>(e->ext_tiles) * e->ext_tile_words * sizeof(uint32_t),
                              e->stream);
         (void)hipMemsetAsync(e->status, 0, sizeof(Status), e->stream);
         (void)hipMemsetAsync(e->bc, 0, sizeof(BoundaryDev), e->stream);

@@ -71,7 +71,10 @@ struct ckmpm_engine {
     uint32_t nwords = 0;
     uint32_t* word_off = nullptr;
     uint32_t* ext_map = nullptr;      // layout extension: two bits per domain block
-    uint32_t ext_tiles = 0;           // map tiles of kEmitTileWords words
+    uint32_t ext_tiles = 0;           // map tiles of ext_tile_words words
+    uint32_t ext_tile_words = 0;      // ExtTileWords of the domain
+    uint32_t ext_tile_shift = 0;      // block key -> map tile shift (ExtTileShift)
+    int emit_variant = 0;             // EmitVariant of the domain
     uint32_t* ext_cnt = nullptr;      // [ext_tiles][2] neighbour / exterior blocks marked per tile
     uint32_t* tsum = nullptr;         // rebin count tile sums
     uint32_t* scan_tmp = nullptr;

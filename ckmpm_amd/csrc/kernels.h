@@ -57,8 +57,15 @@ void LaunchRasterize(const DomainDev& d, const float* xyz, const float* vel, uin
 
 // ---------------------------------------------------------------- layout
 // Map words per emit tile: the extension map is allocated in whole tiles (zero padded),
-// with two counters per tile (neighbour and exterior blocks marked).
-constexpr uint32_t kEmitTileWords = 512;
+// with two counters per tile (neighbour and exterior blocks marked).  Small tiles spread
+// the emit over more workgroups; every marked tile's workgroup reads the counters of all
+// tiles, so domains of more than kSmallTileBlocks blocks take large tiles (a domain of
+// 2^30 blocks then has 32768 tiles, not 131072).
+constexpr uint32_t kEmitTileWordsSmall = 512, kEmitTileWordsLarge = 2048;
+constexpr uint64_t kSmallTileBlocks = 1ull << 25;
+inline uint32_t ExtTileWords(uint64_t nblocks) {
+    return nblocks > kSmallTileBlocks ? kEmitTileWordsLarge : kEmitTileWordsSmall;
+}
 void LaunchTableInsertA(const uint32_t* keys, LayoutCounts* lc, uint32_t cap, uint32_t* table, hipStream_t s);
 void LaunchBitmapToKeysA(uint32_t* bitmap, uint32_t nwords, const uint32_t* word_off, const uint32_t* total,
                          uint32_t* keys, LayoutCounts* lc, uint32_t cap, Status* st, hipStream_t s);
@@ -193,6 +200,7 @@ struct MarkArgs {
     const uint32_t* table;
     uint32_t* map;
     uint32_t* tile_cnt;  // [tiles][2]: neighbour, exterior blocks newly marked per tile
+    uint32_t tile_shift; // tile of block key k: k >> tile_shift (16 keys per map word)
 };
 struct EmitArgs {
     uint32_t* map;
@@ -240,8 +248,11 @@ void LaunchFused(int kernel, int variant, const void* table, const uint32_t* beg
 uint32_t RebinCountGrid(uint32_t cap, uint32_t ex_hint);
 uint32_t RebinScanGrid(uint32_t cap);
 uint32_t MarkGrid(uint32_t cap, uint32_t a_hint);
-// map tiles (one emit workgroup each) and words for a domain of nblocks blocks
+// map tiles (one emit workgroup each) for a domain of nblocks blocks, of
+// ExtTileWords(nblocks) words each; the key shift of Mark; the emit launch variant
 uint32_t ExtTiles(uint64_t nblocks);
+uint32_t ExtTileShift(uint64_t nblocks);
+inline int EmitVariant(uint64_t nblocks) { return ExtTileWords(nblocks) == kEmitTileWordsLarge ? 1 : 0; }
 uint32_t G2P2GGrid(uint32_t grid);
 
 }  // namespace ckmpm

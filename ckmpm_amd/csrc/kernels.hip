@@ -22,12 +22,11 @@ namespace {
 
 constexpr int kThreads = 256;
 // Bitmap words per thread of the emit kernel (a thread emits the marked blocks of its
-// words one after the other) and exterior blocks per thread of the rebin scan.  Small
-// tiles spread both over more workgroups: with 8 words and 8 blocks per thread the 8M
-// scene's scan ran in 12 workgroups, and the substep took 0.706 ms against 0.690 ms.
-constexpr int kEmitItems = 2;
-static_assert(kEmitTileWords == kThreads * kEmitItems, "emit tile");
-static_assert(kEmitItems == 2 || kEmitItems % 4 == 0, "emit words per thread: 2 or a multiple of 4");
+// words one after the other: 2 for small tiles, 8 for large) and exterior blocks per
+// thread of the rebin scan.  Small tiles spread both over more workgroups: with 8 words
+// and 8 blocks per thread the 8M scene's scan ran in 12 workgroups, and the substep took
+// 0.706 ms against 0.690 ms.
+static_assert(kEmitTileWordsSmall == kThreads * 2 && kEmitTileWordsLarge == kThreads * 8, "emit tiles");
 constexpr int kCountThreads = 1024;                    // rebin count workgroup
 constexpr int kCountTile = kCountThreads / 32;         // exterior blocks per count tile
 constexpr int kRebinScanItems = 1;
@@ -397,7 +396,7 @@ __device__ __forceinline__ void DirOffset(int d, int& ox, int& oy, int& oz) {
 // marks them in two passes, the exterior one after the neighbours are registered).
 // Block key k owns bits 2 (k % 16) (neighbour mark) and 2 (k % 16) + 1 (exterior mark)
 // of map word k / 16: a block with the neighbour mark is a neighbour block, one with
-// only the exterior mark an exterior block.  Each tile of kEmitTileWords map words
+// only the exterior mark an exterior block.  Each tile of map words
 // counts both kinds (tile_cnt[2t], tile_cnt[2t+1]) as marks are set: the atomic OR on
 // the key's word orders the marks of one block, so the first neighbour mark that finds
 // the exterior mark set takes that block's exterior count back, and an exterior mark
@@ -438,7 +437,7 @@ __device__ __forceinline__ void MarkExtensionBody(const MarkArgs& a, uint32_t wg
                 } else {
                     de = (old & 3u) ? 0 : 1;
                 }
-                if (dn || de) tile = key / kExtKeysPerWord / kEmitTileWords;
+                if (dn || de) tile = key >> a.tile_shift;
             }
         }
         uint64_t pending = __ballot(tile != kEmpty);
@@ -479,12 +478,14 @@ __device__ __forceinline__ void BlockSums(const uint32_t (&v)[NQ], int nq, uint3
     __syncthreads();
 }
 
-// Layout extension in one kernel: workgroup w owns map words [w, w+1) * kEmitTileWords,
+// Layout extension in one kernel: workgroup w owns map words [w, w+1) * kThreads * ITEMS,
 // takes its first positions from the mark kernel's tile counts (neighbour blocks follow
 // A, exterior blocks follow all neighbour blocks), scans its words' mark counts and
 // writes the blocks of each kind in ascending key order to keys[] and the table,
 // clearing the map.  Tiles without a mark return at once.  Workgroup 0 sets Nb and Ex.
+template <int ITEMS>
 __device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t wg, uint32_t) {
+    static_assert(ITEMS == 2 || ITEMS % 4 == 0, "emit words per thread: 2 or a multiple of 4");
     __shared__ uint32_t sh[kThreads / 64];
     __shared__ uint32_t red[4];
     uint32_t* map = a.map;
@@ -522,11 +523,11 @@ __device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t w
         lc->Ex = ex;
     }
     if (!marked) return;
-    const uint32_t w0 = w * kEmitTileWords + tid * kEmitItems;
-    uint32_t words[kEmitItems];
-    if constexpr (kEmitItems % 4 == 0) {
+    const uint32_t w0 = (w * kThreads + tid) * ITEMS;
+    uint32_t words[ITEMS];
+    if constexpr (ITEMS % 4 == 0) {
 #pragma unroll
-        for (int q = 0; q < kEmitItems / 4; ++q) {
+        for (int q = 0; q < ITEMS / 4; ++q) {
             const uint4 v = reinterpret_cast<const uint4*>(map + w0)[q];
             words[4 * q] = v.x;
             words[4 * q + 1] = v.y;
@@ -541,7 +542,7 @@ __device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t w
     constexpr uint32_t kLow = 0x55555555u;  // the neighbour mark of each block of a word
     uint32_t pn = 0, pe = 0;
 #pragma unroll
-    for (int k = 0; k < kEmitItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         pn += static_cast<uint32_t>(__popc(words[k] & kLow));
         pe += static_cast<uint32_t>(__popc((words[k] >> 1) & ~words[k] & kLow));
     }
@@ -550,7 +551,7 @@ __device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t w
     uint32_t pose = nb + red[2] + BlockExclusiveScan(pe, tot, sh);
     if (pn + pe == 0) return;
 #pragma unroll
-    for (int k = 0; k < kEmitItems; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         if (!words[k]) continue;
         map[w0 + k] = 0u;
         uint32_t bn = words[k] & kLow, be = (words[k] >> 1) & ~words[k] & kLow;
@@ -575,7 +576,10 @@ __device__ __forceinline__ void BitmapScanEmitBody(const EmitArgs& a, uint32_t w
         }
     }
 }
-__global__ __launch_bounds__(kThreads) void BitmapScanEmitKernel(EmitArgs a) { BitmapScanEmitBody(a, blockIdx.x, gridDim.x); }
+template <int ITEMS>
+__global__ __launch_bounds__(kThreads) void BitmapScanEmitKernel(EmitArgs a) {
+    BitmapScanEmitBody<ITEMS>(a, blockIdx.x, gridDim.x);
+}
 
 // Emit set bits in ascending key order at position base + word_off[w] + rank.
 // pass 0 / 1: append after A / after Nb and register in the table; pass 2: particle
@@ -633,11 +637,11 @@ __device__ __forceinline__ void MaxBits(uint32_t* p, float v) {
 // through the previous key table, writes velocities (p/m, gravity on y, sticky
 // boundary within 2 blocks of the domain edge), clears the next P2G target and
 // reduces max |v|^2 (NaN -> +inf) and the P2G deposit bounds of the previous substep.
-// 128 threads per block, 2 blocks per workgroup.
-// Each thread serves one node of kGridIlp blocks at a time: the key -> table -> mass /
-// momentum load chains of those blocks are issued together, so a wave has kGridIlp
-// independent latency chains in flight instead of one.
-constexpr int kGridIlp = 8;
+// 32 threads per block (4 consecutive nodes each, 16-byte loads and stores), 8 blocks
+// per workgroup and step.  Each thread serves kGridIlp blocks at a time: the key ->
+// table -> mass / momentum load chains of those blocks are issued together, so a wave
+// has kGridIlp independent latency chains in flight instead of one.
+constexpr int kGridIlp = 4;
 // A scene's boundary description (the reference's ProcessGridCellVelocity hook,
 // mpm_algorithm.cuh:795-799, as data): regions in order on the node's integer cell, then
 // the walls within 8 cells of the domain faces.
@@ -690,15 +694,16 @@ __device__ __forceinline__ void ApplyBoundary(const ckmpm_boundary_desc& bc, con
 template <bool kCustom>
 __device__ __forceinline__ void GridUpdateBody(const GridUpdateArgs& a, uint32_t wg, uint32_t ng) {
     const uint32_t Nb = min(a.lc->Nb, a.cap), Nbp = a.lc_prev->Nb;
-    const uint32_t n = threadIdx.x & 127u;
-    const uint32_t stride = ng * 2;
+    const uint32_t n0 = (threadIdx.x & 31u) * 4u;  // nodes n0 .. n0 + 3 of each of its blocks
+    constexpr uint32_t kPer = kThreads / 32;       // blocks per workgroup and step
+    const uint32_t stride = ng * kPer;
     // clear the other parity slot for the next substep (nothing else touches it now)
     if (wg == 0) {
         if (threadIdx.x == 0) a.st->vmax2_bits[a.slot ^ 1] = 0u;
         if (threadIdx.x < static_cast<uint32_t>(kMaxModels)) a.st->dep_bits[a.slot ^ 1][threadIdx.x] = 0u;
     }
     float vmax = 0.0f;
-    for (uint32_t i0 = wg * 2 + (threadIdx.x >> 7); i0 < Nb; i0 += stride * kGridIlp) {
+    for (uint32_t i0 = wg * kPer + (threadIdx.x >> 5); i0 < Nb; i0 += stride * kGridIlp) {
         uint32_t key[kGridIlp], io[kGridIlp];
 #pragma unroll
         for (int u = 0; u < kGridIlp; ++u) {
@@ -710,17 +715,16 @@ __device__ __forceinline__ void GridUpdateBody(const GridUpdateArgs& a, uint32_t
             const uint32_t i = i0 + u * stride;
             io[u] = i < Nb ? a.table_prev[key[u]] : kEmpty;
         }
-        float m[kGridIlp], p[kGridIlp][3];
+        float4 m[kGridIlp], p[kGridIlp][3];
 #pragma unroll
         for (int u = 0; u < kGridIlp; ++u) {
-            m[u] = 0.0f;
-            p[u][0] = p[u][1] = p[u][2] = 0.0f;
+            m[u] = p[u][0] = p[u][1] = p[u][2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (io[u] != kEmpty && io[u] < Nbp) {
-                const float* src = a.M_prev + static_cast<size_t>(io[u]) * kMFloats;
-                m[u] = src[n];
-                p[u][0] = src[128 + n];
-                p[u][1] = src[256 + n];
-                p[u][2] = src[384 + n];
+                const float4* src = reinterpret_cast<const float4*>(a.M_prev + static_cast<size_t>(io[u]) * kMFloats + n0);
+                m[u] = src[0];
+                p[u][0] = src[32];
+                p[u][1] = src[64];
+                p[u][2] = src[96];
             }
         }
 #pragma unroll
@@ -732,37 +736,46 @@ __device__ __forceinline__ void GridUpdateBody(const GridUpdateArgs& a, uint32_t
             bool oob = false;
 #pragma unroll
             for (int k = 0; k < 3; ++k) oob |= (b[k] + a.dom.off[k] < 2) || (b[k] + a.dom.off[k] + 2 >= a.dom.dim[k]);
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            float v2 = 0.0f;
-            if (m[u] > a.mass_clamp) {
-                const float im = __builtin_amdgcn_rcpf(m[u]);
+            const float mk[4] = {m[u].x, m[u].y, m[u].z, m[u].w};
+            const float pk[3][4] = {{p[u][0].x, p[u][0].y, p[u][0].z, p[u][0].w},
+                                    {p[u][1].x, p[u][1].y, p[u][1].z, p[u][1].w},
+                                    {p[u][2].x, p[u][2].y, p[u][2].z, p[u][2].w}};
+            float vo[3][4];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) v[k] = p[u][k] * im;
-                v[1] += a.gravity * a.dt;
-                if (kCustom) {
-                    // node (x, y, z) of either sub-grid: cell 4 b + (x, y, z), as the hook sees it
-                    const int cell[3] = {4 * b[0] + static_cast<int>((n >> 4) & 3u), 4 * b[1] + static_cast<int>((n >> 2) & 3u),
-                                         4 * b[2] + static_cast<int>(n & 3u)};
-                    ApplyBoundary(a.bc->d, a.dom, cell, v);
-                } else if (oob) {
-                    v[0] = v[1] = v[2] = 0.0f;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t n = n0 + q;
+                float v[3] = {0.0f, 0.0f, 0.0f};
+                float v2 = 0.0f;
+                if (mk[q] > a.mass_clamp) {
+                    const float im = __builtin_amdgcn_rcpf(mk[q]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) v[k] = pk[k][q] * im;
+                    v[1] += a.gravity * a.dt;
+                    if (kCustom) {
+                        // node (x, y, z) of either sub-grid: cell 4 b + (x, y, z), as the hook sees it
+                        const int cell[3] = {4 * b[0] + static_cast<int>((n >> 4) & 3u),
+                                             4 * b[1] + static_cast<int>((n >> 2) & 3u),
+                                             4 * b[2] + static_cast<int>(n & 3u)};
+                        ApplyBoundary(a.bc->d, a.dom, cell, v);
+                    } else if (oob) {
+                        v[0] = v[1] = v[2] = 0.0f;
+                    }
+                    v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
                 }
-                v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) vo[k][q] = v[k];
+                if (isnan(v2)) {
+                    v2 = INFINITY;
+                    SetFlag(a.st, kFlagNonFinite);
+                }
+                if (mk[q] >= a.vmax_mass_floor || v2 == INFINITY) vmax = fmaxf(vmax, v2);
             }
-            float* dst = a.V + static_cast<size_t>(i) * kVFloats;
-            dst[n] = v[0];
-            dst[128 + n] = v[1];
-            dst[256 + n] = v[2];
-            float* mz = a.M_zero + static_cast<size_t>(i) * kMFloats;
-            mz[n] = 0.0f;
-            mz[128 + n] = 0.0f;
-            mz[256 + n] = 0.0f;
-            mz[384 + n] = 0.0f;
-            if (isnan(v2)) {
-                v2 = INFINITY;
-                SetFlag(a.st, kFlagNonFinite);
-            }
-            if (m[u] >= a.vmax_mass_floor || v2 == INFINITY) vmax = fmaxf(vmax, v2);
+            float4* dst = reinterpret_cast<float4*>(a.V + static_cast<size_t>(i) * kVFloats + n0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dst[32 * k] = make_float4(vo[k][0], vo[k][1], vo[k][2], vo[k][3]);
+            float4* mz = reinterpret_cast<float4*>(a.M_zero + static_cast<size_t>(i) * kMFloats + n0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) mz[32 * k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
     // largest deposit bound per model over the previous layout's particle blocks
@@ -1773,8 +1786,12 @@ uint32_t MarkGrid(uint32_t cap, uint32_t a_hint) {
     return Blocks(static_cast<uint64_t>(std::min(cap, a_hint)) * kDirections, kThreads);
 }
 uint32_t ExtTiles(uint64_t nblocks) {
-    const uint64_t words = (nblocks + kExtKeysPerWord - 1) / kExtKeysPerWord;
-    return static_cast<uint32_t>((words + kEmitTileWords - 1) / kEmitTileWords);
+    const uint64_t words = (nblocks + kExtKeysPerWord - 1) / kExtKeysPerWord, tw = ExtTileWords(nblocks);
+    return static_cast<uint32_t>((words + tw - 1) / tw);
+}
+uint32_t ExtTileShift(uint64_t nblocks) {
+    static_assert(kExtKeysPerWord == 16, "16 keys per map word");
+    return 4u + static_cast<uint32_t>(__builtin_ctz(ExtTileWords(nblocks)));
 }
 
 void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStream_t s) {
@@ -1802,7 +1819,10 @@ void LaunchOne(int kernel, int variant, const void* args, uint32_t grid, hipStre
         case kFkScatterMark:
             ScatterMarkKernel<<<grid, kThreads, 0, s>>>(*static_cast<const ScatterMarkArgs*>(args));
             break;
-        default: BitmapScanEmitKernel<<<grid, kThreads, 0, s>>>(*static_cast<const EmitArgs*>(args)); break;
+        default:
+            if (variant) BitmapScanEmitKernel<8><<<grid, kThreads, 0, s>>>(*static_cast<const EmitArgs*>(args));
+            else BitmapScanEmitKernel<2><<<grid, kThreads, 0, s>>>(*static_cast<const EmitArgs*>(args));
+            break;
     }
 }
 
@@ -1861,7 +1881,8 @@ CKMPM_FUSED(RebinCountFused, RebinCountBody, RebinCountArgs, kCountThreads)
 CKMPM_FUSED(RebinScanFused, RebinScanBody, RebinScanArgs, kThreads)
 CKMPM_FUSED(RebinScatterFused, RebinScatterBody, RebinScatterArgs, kThreads)
 CKMPM_FUSED(MarkExtensionFused, MarkExtensionBody, MarkArgs, kThreads)
-CKMPM_FUSED(BitmapScanEmitFused, BitmapScanEmitBody, EmitArgs, kThreads)
+CKMPM_FUSED(BitmapScanEmitFused2, BitmapScanEmitBody<2>, EmitArgs, kThreads)
+CKMPM_FUSED(BitmapScanEmitFused8, BitmapScanEmitBody<8>, EmitArgs, kThreads)
 CKMPM_FUSED(ScatterMarkFused, ScatterMarkBody, ScatterMarkArgs, kThreads)
 #undef CKMPM_FUSED
 
@@ -1922,7 +1943,8 @@ void LaunchFused(int kernel, int variant, const void* table, const uint32_t* beg
             ScatterMarkFused<<<total, kThreads, 0, s>>>(static_cast<const ScatterMarkArgs*>(table), begin, n);
             break;
         default:
-            BitmapScanEmitFused<<<total, kThreads, 0, s>>>(static_cast<const EmitArgs*>(table), begin, n);
+            if (variant) BitmapScanEmitFused8<<<total, kThreads, 0, s>>>(static_cast<const EmitArgs*>(table), begin, n);
+            else BitmapScanEmitFused2<<<total, kThreads, 0, s>>>(static_cast<const EmitArgs*>(table), begin, n);
             break;
     }
 }
