@@ -20,6 +20,7 @@ batch against sequential launches).  cpu_baseline: on the same scene shrunk 4x p
 core, and a NumPy restatement of the reference's Python demo substep.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -258,11 +259,16 @@ def bench_adjoint(args, torch):
     for _ in range(3):
         eng = engine_from_scene(sc)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(steps):
-            eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
-        eng.Synchronize()
-        fwd_samples.append(time.perf_counter() - t0)
+        gc.collect()
+        gc.disable()
+        try:
+            t0 = time.perf_counter()
+            for k in range(steps):
+                eng.Substep(0.0 if k == 0 else sc.dt, sc.dt)
+            eng.Synchronize()
+            fwd_samples.append(time.perf_counter() - t0)
+        finally:
+            gc.enable()
         eng.close()
     t_fwd = float(np.median(fwd_samples))
     target = torch.tensor([0.35, 0.5, 0.5], dtype=torch.float64)
@@ -270,13 +276,21 @@ def bench_adjoint(args, torch):
     def fwd_bwd(segment):
         logE.grad = None
         v0.grad = None
-        t0 = time.perf_counter()
-        x = simulate(logE, v0, sc, steps, segment)
-        loss = ((x.mean(0) - target) ** 2).sum()
-        t1 = time.perf_counter()
-        loss.backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
+        # the host loop launches every substep: a pass of Python's cyclic garbage
+        # collector inside it (tens of ms) would stall the GPU, so collect first and
+        # keep the collector off during the call (as timeit does)
+        gc.collect()
+        gc.disable()
+        try:
+            t0 = time.perf_counter()
+            x = simulate(logE, v0, sc, steps, segment)
+            loss = ((x.mean(0) - target) ** 2).sum()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        finally:
+            gc.enable()
         print(f"[bench] cantilever {N} particles, segment {segment}: fwd {t_fwd * 1e3 / steps:.3f} ms/substep, "
               f"recorded fwd {(t1 - t0) * 1e3 / steps:.3f}, backward {(t2 - t1) * 1e3 / steps:.3f}",
               file=sys.stderr, flush=True)
@@ -460,9 +474,8 @@ def main():
     eng.Synchronize()
     torch.cuda.synchronize()
     print(f"[bench] {args.warmup} warmup substeps done, timing {args.steps}", file=sys.stderr, flush=True)
-    # the job: K substeps replayed as hipGraphs of two substeps (one launch per kernel,
-    # the layout extension on a second stream of the graph), captured before the timed
-    # region
+    # the job: K substeps replayed as hipGraphs of two substeps (one launch per kernel),
+    # captured before the timed region; the K substeps are one call into the library
     b = MPMBatch([eng])
     b.Run(2, MPMBatch.FUSED)
     b.Synchronize()

@@ -1030,15 +1030,27 @@ __device__ __forceinline__ void G2P2GBody(const G2P2GArgs& a, uint32_t wg, uint3
         const float inv_qs = __int_as_float((mexp - kq + 127) << 23);
         float tmax = 0.0f;  // largest deposit bound of this thread's particles
         __syncthreads();
-        // velocity halo in the same windows as the P2G accumulator (3 channels)
-        for (int e = tid; e < 8 * kNodesPerBlock; e += THREADS) {
-            const int o = e >> 7, nd = e & 127;
-            const int r = WindowOfHalo(o, nd);
-            if (r < 0) continue;
+        // velocity halo in the same windows as the P2G accumulator (3 channels): a thread
+        // takes a z column of 4 nodes of a grid block (16-byte loads), which fills 4
+        // consecutive window entries; in grid 0 the first of them lies outside the window
+        // when the block is at z offset 0 (lz = 0), and a column at lx = 0 or ly = 0
+        // lies outside altogether
+        for (int e = tid; e < 8 * kNodesPerBlock / 4; e += THREADS) {
+            const int o = e >> 5, nd0 = (e & 31) * 4;
+            const int r3 = WindowOfHalo(o, nd0 + 3);
+            if (r3 < 0) continue;
             const uint32_t bi = sHalo[o];
-            const float* src = a.V + static_cast<size_t>(bi) * kVFloats + nd;
+            const float4* src = reinterpret_cast<const float4*>(a.V + static_cast<size_t>(bi) * kVFloats + nd0);
+            const bool first = nd0 >= 64 || (o & 1);
 #pragma unroll
-            for (int ch = 0; ch < kVChannels; ++ch) sV[ch * kMWin + r] = bi != kEmpty ? src[ch * 128] : 0.0f;
+            for (int ch = 0; ch < kVChannels; ++ch) {
+                const float4 v = bi != kEmpty ? src[32 * ch] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const int r = ch * kMWin + r3;
+                if (first) sV[r - 3] = v.x;
+                sV[r - 2] = v.y;
+                sV[r - 1] = v.z;
+                sV[r] = v.w;
+            }
         }
         if (tid == 0) {
             bool miss = false;
