@@ -62,6 +62,7 @@ struct MaterialDev {
     int kind;
     float lambda, mu;
     float bulk, gamma, viscosity, J0;
+    int gamma_int;              // gamma when it is an integer in [1, 16], else 0
     float sin_phi;
     float yield;                // von Mises yield stress
     float beta, xi, msq, alpha0;  // Cam-clay: beta, hardening factor, M^2, initial logJp

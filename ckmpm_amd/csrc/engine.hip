@@ -342,6 +342,9 @@ int ckmpm_add_model(ckmpm_engine* e, const ckmpm_material_desc* mat, float mass,
     m.mat.mu = mat->mu;
     m.mat.bulk = mat->bulk;
     m.mat.gamma = mat->gamma;
+    m.mat.gamma_int = (mat->gamma >= 1.0f && mat->gamma <= 16.0f && mat->gamma == std::floor(mat->gamma))
+                          ? static_cast<int>(mat->gamma)
+                          : 0;
     m.mat.viscosity = mat->viscosity;
     m.mat.J0 = mat->J0;
     m.mat.sin_phi = static_cast<float>(std::sin(static_cast<double>(mat->friction_angle_deg) * 3.14159265358979323846 / 180.0));

@@ -329,11 +329,22 @@ __device__ __forceinline__ void StressCamClay(const MaterialDev& m, float lambda
 // J <- J (1 + dt tr C); p = K (J^-gamma - 1); tau = J (nu (C + C^T) - p I).
 // The state holds q = J - 1 (the reference stores J): near J = 1 an FP32 J keeps only
 // ~6e-8 absolute resolution, which is a large relative error of J - 1 and hence of the
-// pressure; q keeps full relative precision, and p = K expm1(-gamma log1p(q)).
+// pressure; q keeps full relative precision.  For an integer gamma g (the scenes use 7),
+// with r = 1/J: J^-g - 1 = (r - 1)(1 + r + ... + r^(g-1)) and r - 1 = -q r, a sum of
+// positive terms that keeps full relative precision near J = 1 (a rcp and g - 1 fmas);
+// otherwise p = K expm1(-gamma log1p(q)), which costs two library calls.
 __device__ __forceinline__ void StressFluid(const MaterialDev& m, const float (&C)[9], float dt, float& q, float (&tau)[9]) {
     q += (C[0] + C[4] + C[8]) * dt * (1.0f + q);
     const float J = 1.0f + q;
-    const float p = m.bulk * expm1f(-m.gamma * log1pf(q));
+    float p;
+    if (m.gamma_int > 0) {
+        const float r = __builtin_amdgcn_rcpf(J);
+        float sum = 1.0f;
+        for (int k = 1; k < m.gamma_int; ++k) sum = fmaf(sum, r, 1.0f);
+        p = m.bulk * (-q * r) * sum;
+    } else {
+        p = m.bulk * expm1f(-m.gamma * log1pf(q));
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
