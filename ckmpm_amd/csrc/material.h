@@ -71,10 +71,41 @@ __device__ __forceinline__ void StressFixedCorotated(float lambda, float mu, con
         }
 }
 
+// log(1 + d) and exp(x) of the small principal strains: for |d|, |x| <= 1/32 their
+// Taylor series (the terms fall by 1/32 or faster each, so 7 terms leave under 2^-35
+// relative), otherwise the library functions, which cost several times as many
+// instructions.  The lanes of a wavefront take the series together whenever their
+// strains are all small.
+__device__ __forceinline__ float Log1pSmall(float d) {
+    if (fabsf(d) <= 0.03125f) {
+        float p = 1.0f / 7.0f;  // d (1 - d/2 + d^2/3 - ... + d^6/7)
+        p = fmaf(p, d, -1.0f / 6.0f);
+        p = fmaf(p, d, 0.2f);
+        p = fmaf(p, d, -0.25f);
+        p = fmaf(p, d, 1.0f / 3.0f);
+        p = fmaf(p, d, -0.5f);
+        p = fmaf(p, d, 1.0f);
+        return d * p;
+    }
+    return log1pf(d);
+}
+__device__ __forceinline__ float ExpSmall(float x) {
+    if (fabsf(x) <= 0.03125f) {
+        float p = 1.0f / 720.0f;  // 1 + x + x^2/2 + ... + x^6/720
+        p = fmaf(p, x, 1.0f / 120.0f);
+        p = fmaf(p, x, 1.0f / 24.0f);
+        p = fmaf(p, x, 1.0f / 6.0f);
+        p = fmaf(p, x, 0.5f);
+        p = fmaf(p, x, 1.0f);
+        return fmaf(p, x, 1.0f);
+    }
+    return expf(x);
+}
+
 // Neo-Hookean (closed form): tau = mu (F F^T - I) + lambda ln(J) I,
 // F F^T - I = E + E^T + E E^T.
 __device__ __forceinline__ void StressNeoHookean(float lambda, float mu, const float (&E)[9], float (&tau)[9]) {
-    const float lnJ = log1pf(DetIPlusEMinus1(E));
+    const float lnJ = Log1pSmall(DetIPlusEMinus1(E));
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -103,7 +134,7 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
     SignedSvdStrain(E, sv, d);
     float eps[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) eps[a] = sv.s[a] >= 1e-4f ? log1pf(d[a]) : logf(fmaxf(fabsf(sv.s[a]), 1e-4f));
+    for (int a = 0; a < 3; ++a) eps[a] = sv.s[a] >= 1e-4f ? Log1pSmall(d[a]) : logf(fmaxf(fabsf(sv.s[a]), 1e-4f));
     const float eps_sum = eps[0] + eps[1] + eps[2];
     const float eps_tr = eps_sum + logJp;
     float eh[3];
@@ -131,22 +162,26 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
         keep = true;
     }
     const float htr = H[0] + H[1] + H[2];
-    float k[3], ns[3];
+    float k[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        ns[a] = expf(H[a]);
-        k[a] = smu * H[a] + lambda * htr;  // (2 mu log ns + lambda tr log ns) / ns * ns
-        // F kept with singular values s: tau = U diag(phat s) U^T
-        if (keep) k[a] *= sv.s[a] / ns[a];
-    }
-    if (rebuild) {
-        // E = U diag(ns) V^T - I
+    for (int a = 0; a < 3; ++a) k[a] = smu * H[a] + lambda * htr;  // (2 mu log ns + lambda tr log ns) / ns * ns
+    if (rebuild || keep) {  // the new singular values ns = exp H are needed
+        float ns[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 3; ++a) {
+            ns[a] = ExpSmall(H[a]);
+            // F kept with singular values s: tau = U diag(phat s) U^T
+            if (keep) k[a] *= sv.s[a] / ns[a];
+        }
+        if (rebuild) {
+            // E = U diag(ns) V^T - I
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                E[i * 3 + j] = sv.U[i * 3 + 0] * ns[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * ns[1] * sv.V[j * 3 + 1] +
-                               sv.U[i * 3 + 2] * ns[2] * sv.V[j * 3 + 2] - (i == j ? 1.0f : 0.0f);
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    E[i * 3 + j] = sv.U[i * 3 + 0] * ns[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * ns[1] * sv.V[j * 3 + 1] +
+                                   sv.U[i * 3 + 2] * ns[2] * sv.V[j * 3 + 2] - (i == j ? 1.0f : 0.0f);
+        }
     }
     // tau = U diag(phat) V^T F^T = U diag(phat * ns) U^T
 #pragma unroll
@@ -165,7 +200,7 @@ __device__ __forceinline__ void StressDruckerPrager(float lambda, float mu, floa
 // reference's logf of a non-positive singular value is NaN).
 __device__ __forceinline__ void HenckyStrain(const Svd3& sv, const float (&d)[3], float (&eps)[3]) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) eps[a] = sv.s[a] >= 1e-4f ? log1pf(d[a]) : logf(fmaxf(fabsf(sv.s[a]), 1e-4f));
+    for (int a = 0; a < 3; ++a) eps[a] = sv.s[a] >= 1e-4f ? Log1pSmall(d[a]) : logf(fmaxf(fabsf(sv.s[a]), 1e-4f));
 }
 
 // E = U diag(ns) V^T - I
@@ -214,7 +249,7 @@ __device__ __forceinline__ void StressVonMises(float lambda, float mu, float yie
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             eps[a] -= r * eh[a];
-            ns[a] = expf(eps[a]);
+            ns[a] = ExpSmall(eps[a]);
         }
         RebuildStrain(sv, ns, E);
     }
