@@ -310,8 +310,9 @@ int ckmpm_batch_create(ckmpm_engine* const* engines, uint32_t count, ckmpm_batch
  * mode 3: pairs of substeps replay one hipGraph per engine on the engine's own stream;
  * mode 4: as 2, but each kernel of a substep is one launch for all engines (one per
  *         material kind and workgroup size for G2P2G, per boundary kind for the grid
- *         update), its workgroups split by engine; about 12 launches per substep for the
- *         whole batch.  The graph reads argument tables that it holds in device memory. */
+ *         update), its workgroups split by engine; about 9 launches per substep for the
+ *         whole batch (grid update, G2P2G per kind, rebin count, scan, scatter with the
+ *         extension mark, emit).  The graph reads argument tables that it holds in device memory. */
 int ckmpm_batch_run(ckmpm_batch* b, uint32_t substeps, int32_t mode);
 /* waits for every engine; returns the first engine error */
 int ckmpm_batch_synchronize(ckmpm_batch* b);
