@@ -185,7 +185,7 @@ def test_colliding_spheres_trajectory_100_substeps():
         assert abs(disp - 99 * sc.dt) < 0.05 * 99 * sc.dt
 
 
-def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5, j0=1.0):
+def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5, j0=1.0, gamma=7.0):
     """A small box of one material with random initial velocities (exercises C and stress)."""
     dx = 1.0 / (4 * dim)
     rng = scenes.Splitmix(seed)
@@ -196,7 +196,7 @@ def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5,
     vol = dx ** 3 / ppc_side ** 3
     lam, mu = scenes.lame(2e5, 0.3)
     if kind == scenes.FLUID:
-        mat = scenes.Material(kind, bulk=1e4, gamma=7.0, viscosity=0.1, J0=j0)
+        mat = scenes.Material(kind, bulk=1e4, gamma=gamma, viscosity=0.1, J0=j0)
     elif kind == scenes.VON_MISES:
         # yields within the run: |dev eps| reaches ~1e-3 against yield / (2 mu) = 2e-4
         mat = scenes.Material(kind, lam, mu, yield_stress=30.0)
@@ -214,6 +214,15 @@ def _box_scene(kind, seed, dim=12, ppc_side=2, speed=0.3, gravity=-9.8, dt=2e-5,
                                   scenes.VON_MISES, scenes.CAM_CLAY])
 def test_material_box_lockstep(kind):
     sc = _box_scene(kind, seed=10 + kind)
+    _lockstep(sc, 6, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
+
+
+@pytest.mark.parametrize("gamma", [6.5, 1.0])
+def test_fluid_box_lockstep_gamma_forms(gamma):
+    """The fluid pressure takes a sum of powers of 1/J for an integer gamma (7 above; 1
+    here, the shortest sum) and expm1(-gamma log1p(J - 1)) otherwise (6.5): both forms
+    against the oracle's K (J^-gamma - 1), on a slightly compressed box (J0 = 0.999)."""
+    sc = _box_scene(scenes.FLUID, seed=31, j0=0.999, gamma=gamma)
     _lockstep(sc, 6, lambda k: (0.0 if k == 0 else sc.dt, sc.dt), (2e-7, 2e-5))
 
 
@@ -244,6 +253,36 @@ def _clamped_sand_scene():
 # (test_oracle_kernel.py::test_clamped_sand_grid_sensitivity_to_one_ulp pins that), so
 # the momentum bar of that substep is 1e-4 and every other bar is unchanged.
 CLAMPED_NEXT_SUBSTEP_TOL = 1e-4
+
+
+@pytest.mark.parametrize("kind", [scenes.NEO_HOOKEAN, scenes.DRUCKER_PRAGER, scenes.VON_MISES])
+def test_stretched_state_lockstep(kind):
+    """A box at rest whose imported state holds F = R1 diag(s) R2^T with singular values
+    s in [0.9, 1.1] on every particle: the Hencky logs and exps (and the Neo-Hookean
+    log J) leave their 7-term series for |log s| or |s - 1| above 1/32 and take the
+    library functions, which the boxes above never reach.  Two substeps against the
+    oracle from the same imported state, at the usual bars."""
+    sc = _box_scene(kind, seed=51 + kind, speed=0.0)
+    rng = np.random.default_rng(7 + kind)
+    eng = engine_from_scene(sc)
+    orc = Oracle.from_scene(sc)
+    orc.setup()
+    st = eng.state(0)
+    n = st.shape[1]
+    for p in range(n):
+        F = _rotation(rng) @ np.diag(rng.uniform(0.9, 1.1, 3)) @ _rotation(rng).T
+        st[3:12, p] = (F - np.eye(3)).reshape(9).astype(np.float32)
+    eng.set_state(0, st)
+    for k in range(2):
+        dt = 0.0 if k == 0 else sc.dt
+        _import(eng, orc, 1)
+        orc.physics(dt, sc.dt)
+        eng.Substep(dt, sc.dt)
+        eng.Synchronize()
+        _grid_close(eng.grid_m(), orc.grid_mnext(), GRID_TOL)
+        sg, so = eng.global_state(0), orc.state_next(0)
+        np.testing.assert_allclose(sg[:3], so[:3], rtol=0, atol=2e-7)
+        np.testing.assert_allclose(sg[3:], so[3:], rtol=0, atol=2e-5)
 
 
 def test_drucker_prager_inverted_and_near_singular_lockstep():
