@@ -155,7 +155,15 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
     LayoutBuf& P = e->L[1 - e->cur];
     float* Mout = e->M[1 - e->mprev];
     const int sc = e->sc, sn = 1 - e->sc;
-    const uint32_t grid = GridFor(e->a_hint + e->a_hint / 4 + 8);
+    // G2P2G workgroups, each taking particle blocks round-robin: 1.25 per block for one
+    // model; for several models, whose launches run side by side in graph replays, half
+    // a workgroup per block (but at least 2048, 8 per CU, where there are that many
+    // blocks), so that the launches share the CUs.  On the 8M fluid + sand scene 0.5 per
+    // block took the substep from 0.654 to 0.628 ms (0.625: 0.637; 0.31: 0.629 but slower
+    // as plain launches; 0.25: 0.635); on the single-model 1M jelly it cost 1.4%.
+    const uint32_t wide = e->a_hint + e->a_hint / 4u;
+    const uint32_t grid =
+        GridFor((e->models.size() > 1 ? std::max(e->a_hint / 2u, std::min(wide, 2048u)) : wide) + 8u);
     // The models' launches run one after the other on the engine stream.  Running the
     // second model's on a second stream, concurrently with the first (the launches are
     // independent; their grid deposits are commutative atomic adds), measured slower on
