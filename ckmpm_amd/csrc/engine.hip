@@ -55,6 +55,7 @@ struct FusedPhase {
 };
 struct FusedRec {
     std::vector<FusedPhase> ph;
+    bool shared = false;  // several engines: their G2P2G launches run side by side
     void Add(int stage, int kernel, int variant, const void* a, size_t size, uint32_t grid) {
         FusedPhase* p = nullptr;
         for (FusedPhase& q : ph)
@@ -156,14 +157,14 @@ void PhaseParticles(ckmpm_engine* e, double dt, double next_dt, FusedRec* rec = 
     float* Mout = e->M[1 - e->mprev];
     const int sc = e->sc, sn = 1 - e->sc;
     // G2P2G workgroups, each taking particle blocks round-robin: 1.25 per block for one
-    // model; for several models, whose launches run side by side in graph replays, half
-    // a workgroup per block (but at least 2048, 8 per CU, where there are that many
-    // blocks), so that the launches share the CUs.  On the 8M fluid + sand scene 0.5 per
-    // block took the substep from 0.654 to 0.628 ms (0.625: 0.637; 0.31: 0.629 but slower
-    // as plain launches; 0.25: 0.635); on the single-model 1M jelly it cost 1.4%.
-    const uint32_t wide = e->a_hint + e->a_hint / 4u;
-    const uint32_t grid =
-        GridFor((e->models.size() > 1 ? std::max(e->a_hint / 2u, std::min(wide, 2048u)) : wide) + 8u);
+    // model alone; half a workgroup per block where launches run side by side in graph
+    // replays (several models, or the fused launches of a batch of engines), so that the
+    // launches share the CUs.  On the 8M fluid + sand scene 0.5 per block took the
+    // substep from 0.654 to 0.628 ms (0.625: 0.637; 0.31: 0.629 but slower as plain
+    // launches; 0.25: 0.635), the two-model 30k spheres gained 5% and the 64-scene batch
+    // 1-2% (0.25: no more); the single-model 1M jelly lost 12% (one engine in a graph).
+    const bool side_by_side = e->models.size() > 1 || (rec && rec->shared);
+    const uint32_t grid = GridFor((side_by_side ? e->a_hint / 2u : e->a_hint + e->a_hint / 4u) + 8u);
     // The models' launches run one after the other on the engine stream.  Running the
     // second model's on a second stream, concurrently with the first (the launches are
     // independent; their grid deposits are commutative atomic adds), measured slower on
@@ -968,6 +969,7 @@ int CaptureFused(ckmpm_batch* b, BatchGraph& g, int nsub) {
     g.hints.assign(ne, 0);
     g.versions.assign(ne, 0);
     FusedRec rec[2];
+    rec[0].shared = rec[1].shared = ne > 1;
     for (size_t i = 0; i < ne; ++i) {
         g.roles[i] = EngineRoles(b->eng[i]);
         g.hints[i] = b->eng[i]->a_hint;
