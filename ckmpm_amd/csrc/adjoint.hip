@@ -94,6 +94,16 @@ __global__ void ToInputOrderKernel(const uint32_t* ids, const float* src, uint32
     }
 }
 
+// dst[3 ids[t] + k] = src[3 t + k]: AoS rows from processing order to input order
+__global__ void AosToInputOrderKernel(const uint32_t* ids, const float* src, uint32_t n, float* dst) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const size_t d = 3 * static_cast<size_t>(ids[t]), r = 3 * static_cast<size_t>(t);
+        dst[d] = src[r];
+        dst[d + 1] = src[r + 1];
+        dst[d + 2] = src[r + 2];
+    }
+}
+
 // ------------------------------------------------------------------ grid backward
 struct GridBwdArgs {
     DomainDev dom;
@@ -1282,16 +1292,24 @@ int ckmpm_adjoint_gradients(ckmpm_engine* e, int32_t model, float* lambda_bar, f
 
 int ckmpm_get_positions_input_order(ckmpm_engine* e, int32_t model, float* xyz) {
     if (!e || !e->adj || !xyz || model < 0 || model >= static_cast<int32_t>(e->models.size())) return CKMPM_ERR_INVALID;
+    if (!e->setup_done) return Fail(e, CKMPM_ERR_INVALID, "call ckmpm_initial_setup first");
     AdjointState* A = e->adj;
-    const uint32_t n = e->models[model].n;
-    std::vector<float> proc(3 * static_cast<size_t>(n));
-    uint64_t got = 0;
-    CK_TRY(ckmpm_get_positions(e, model, proc.data(), n, &got));
-    // processing-order AoS positions -> input order through the identities
-    std::vector<uint32_t> ids(n);
-    CK_HIP(e, hipMemcpy(ids.data(), A->ids[A->idc][model], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    for (uint32_t t = 0; t < n; ++t)
-        for (int k = 0; k < 3; ++k) xyz[3 * static_cast<size_t>(ids[t]) + k] = proc[3 * static_cast<size_t>(t) + k];
+    ModelBuf& m = e->models[model];
+    const uint32_t n = m.n;
+    if (!e->pos_stage) {  // the staging buffer of ckmpm_get_positions, kept
+        uint32_t nmax = 0;
+        for (const ModelBuf& q : e->models) nmax = std::max(nmax, q.n);
+        CK_TRY(Alloc(e, &e->pos_stage, 3 * static_cast<size_t>(nmax)));
+    }
+    // processing-order AoS positions, then to input order through the identities on the
+    // device (host_io holds 3 n floats), then one copy to the caller
+    JoinBack(e);  // host_io: after any undo still queued
+    const LayoutBuf& P = e->L[1 - e->cur];
+    const uint32_t wgs = std::max<uint32_t>(8u, std::min<uint32_t>(e->a_hint + 8u, 16384u));
+    LaunchExportPositions(e->dom, P.keys, P.lc, m.off[1 - e->sc], m.st[e->sc], n, e->pos_stage, wgs, e->stream);
+    AosToInputOrderKernel<<<AGrid(n), kAT, 0, e->stream>>>(A->ids[A->idc][model], e->pos_stage, n, A->host_io);
+    CK_HIP(e, hipMemcpyAsync(xyz, A->host_io, 3 * sizeof(float) * static_cast<size_t>(n), hipMemcpyDeviceToHost, e->stream));
+    CK_HIP(e, hipStreamSynchronize(e->stream));
     return CKMPM_OK;
 }
 
